@@ -1,0 +1,998 @@
+/*
+ * ImmerseFlow oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Plain C, FP64, OpenMP over rows.  Built with -ffp-contract=off so every
+ * expression rounds exactly as written; the product's HIP kernels evaluate
+ * the same expressions in the same order, which is what lets the parity
+ * tests demand bit equality where no reduction order is involved.
+ *
+ * Reductions that feed control flow (the AD L1 residual) are summed row by
+ * row in a fixed order so the oracle is deterministic at any thread count.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(i, j, nx) ((i) + (j) * (nx))
+
+/* ======================================================================== */
+/* grid metric: ImmerseFlow::readGridData, src/include/preSim.cu:293-355    */
+/* ======================================================================== */
+int of_grid_metric(int nf, const double *xf, double *xc, double *dx)
+{
+    const int n = nf + 1; /* nx = NX + 2 with nf = NX + 1 (main.cu:55-58) */
+    if (nf < 2) return -1;
+    /* centres, preSim.cu:294-296 */
+    for (int i = 1; i < n - 1; i++) xc[i] = (xf[i - 1] + xf[i]) / 2.0;
+    /* ghost mirrors, preSim.cu:304-307: the west mirror is about x = 0 */
+    xc[0] = -1 * xc[1];
+    xc[n - 1] = xf[nf - 1] + (xf[nf - 1] - xc[n - 2]);
+    /* widths, preSim.cu:329-334, and ghost copies, :337-355 */
+    for (int i = 1; i < n - 1; i++) dx[i] = xf[i] - xf[i - 1];
+    dx[0] = dx[1];
+    dx[n - 1] = dx[n - 2];
+    return 0;
+}
+
+/* ======================================================================== */
+/* compat mode                                                              */
+/* ======================================================================== */
+
+/* initializeKernel, preSim.cu:53-97 (vortex IC; fields zeroed first). */
+void of_compat_init(int nx, int ny, const double *xc, const double *yc,
+                    double *u, double *v, double *uf, double *vf, double *p)
+{
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            double r = sqrt(pow(xc[i] - 0.5, 2.0) + pow(yc[j] - 0.5, 2.0));
+            double r0 = 0.1;
+            u[id] = 1.0 - 0.25 * (yc[j] - 0.5) * exp((1.0 - pow(r / r0, 2.0)) / 2.0);
+            v[id] = 0.25 * (xc[i] - 0.5) * exp((1.0 - pow(r / r0, 2.0)) / 2.0);
+            p[id] = 0.0;
+        }
+    memset(uf, 0, sizeof(double) * (size_t)(nx - 1) * (ny - 2));
+    memset(vf, 0, sizeof(double) * (size_t)(nx - 2) * (ny - 1));
+}
+
+/* Per-axis viscous coefficients of calculateADCoefficients,
+ * ADSolver.cu:34-39, evaluated with the reference's operation order:
+ *   coeff   = (1.0 + (dt/Re)*(X)) + (dt/Re)*(Y)
+ *   dx2_p1  = (dt/Re) * (2 / (dx_i * (dx_i + dx_ip1)))  etc. */
+typedef struct {
+    double *xp, *xm, *yp, *ym; /* off-diagonals */
+    double *px, *py;            /* aP = px[i] + py[j] */
+} ad_coef;
+
+static void ad_coef_build(int nx, int ny, const double *dx, const double *dy,
+                          double dt, double re, ad_coef *c)
+{
+    const double k = dt / re;
+    c->xp = calloc(nx, sizeof(double));
+    c->xm = calloc(nx, sizeof(double));
+    c->px = calloc(nx, sizeof(double));
+    c->yp = calloc(ny, sizeof(double));
+    c->ym = calloc(ny, sizeof(double));
+    c->py = calloc(ny, sizeof(double));
+    for (int i = 1; i < nx - 1; i++) {
+        const double dxi = dx[i], dxp = dx[i + 1], dxm = dx[i - 1];
+        c->xp[i] = k * (2 / (dxi * (dxi + dxp)));
+        c->xm[i] = k * (2 / (dxi * (dxi + dxm)));
+        c->px[i] = 1.0 + k * ((2 / (dxi * (dxi + dxp))) + (2 / (dxi * (dxi + dxm))));
+    }
+    for (int j = 1; j < ny - 1; j++) {
+        const double dyj = dy[j], dyp = dy[j + 1], dym = dy[j - 1];
+        c->yp[j] = k * (2 / (dyj * (dyj + dyp)));
+        c->ym[j] = k * (2 / (dyj * (dyj + dym)));
+        c->py[j] = k * ((2 / (dyj * (dyj + dyp))) + (2 / (dyj * (dyj + dym))));
+    }
+}
+
+static void ad_coef_free(ad_coef *c)
+{
+    free(c->xp); free(c->xm); free(c->px);
+    free(c->yp); free(c->ym); free(c->py);
+}
+
+/* Compute_velf, ADSolver.cu:162-189, including the loop-continuation quirk:
+ * with the reference launch (one pass of the grid-stride loop per thread)
+ * the vf loop only runs for ids in [(nx-1)(ny-2), (nx-2)(ny-1)). */
+static void compat_velf(int nx, int ny, const double *dx, const double *dy,
+                        const double *u, const double *v, double *uf,
+                        double *vf)
+{
+    const int nuf = (nx - 1) * (ny - 2);
+    const int nvf = (nx - 2) * (ny - 1);
+    for (int id = 0; id < nuf; id++) {
+        const int i = id % (nx - 1), j = id / (nx - 1);
+        const int idc = j * nx + i;
+        /* dx is j-independent (preSim.cu:331): dx[idc+nx] = dx_i */
+        uf[id] = 1.0 / (dx[i] + dx[i + 1]) * (u[idc + nx + 1] * dx[i] + u[idc + nx] * dx[i + 1]);
+    }
+    for (int id = nuf; id < nvf; id++) {
+        const int i = id % (nx - 2), j = id / (nx - 2);
+        const int idc = j * nx + i;
+        vf[id] = 1.0 / (dy[j + 1] + dy[j]) * (v[idc + nx + 1] * dy[j] + v[idc + 1] * dy[j + 1]);
+    }
+}
+
+/* set_velocity_BC, ADSolver.cu:191-217.  The reference applies the four
+ * side rules in place from all threads at once, so the corner cells read
+ * neighbouring ghosts that other threads are writing.  The oracle fixes one
+ * outcome: the order of a sequential sweep over id (i fastest), side rules in
+ * source order.  Corner (0,0) then reads the pre-call value of ghost (0,1);
+ * the other three corners read freshly written edge ghosts.  Edge cells read
+ * interior cells only, which the kernel does not write.  The uf/vf loops of
+ * the kernel (:221-263) never run (loop-continuation quirk) and are omitted. */
+static void compat_bc(int nx, int ny, double *q, double qbc)
+{
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            if (i == 0) q[id] = -q[id + 1] + qbc * 2.0;
+            if (j == 0) q[id] = -q[id + nx] + qbc * 2.0;
+            if (i == nx - 1) q[id] = -q[id - 1] + qbc * 2.0;
+            if (j == ny - 1) q[id] = -q[id - nx] + qbc * 2.0;
+        }
+}
+
+/* ADSource, ADSolver.cu:46-79 (factor c; the reference uses 0.5). */
+static void source_cells(int nx, int ny, const double *dx, const double *dy,
+                         double dt, double c, const double *q,
+                         const double *uf, const double *vf, double *s)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            s[id] = 0.0;
+            if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+                const double dx_i = dx[i], dx_ip1 = dx[i + 1], dx_im1 = dx[i - 1];
+                const double dy_j = dy[j], dy_jp1 = dy[j + 1], dy_jm1 = dy[j - 1];
+                const double ufe = uf[i + (j - 1) * (nx - 1)];
+                const double ufw = uf[i - 1 + (j - 1) * (nx - 1)];
+                const double vfn = vf[i - 1 + j * (nx - 2)];
+                const double vfs = vf[i - 1 + (j - 1) * (nx - 2)];
+                s[id] = (q[id] - (dt / dx_i) * c * (((1 / (dx_i + dx_ip1)) * (q[id + 1] * dx_i + q[id] * dx_ip1) * ufe)
+                                                  - ((1 / (dx_i + dx_im1)) * (q[id] * dx_im1 + q[id - 1] * dx_i) * ufw)))
+                        - (dt / dy_j) * c * (((1 / (dy_j + dy_jp1)) * (q[id + nx] * dy_j + q[id] * dy_jp1) * vfn)
+                                            - ((1 / (dy_j + dy_jm1)) * (q[id] * dy_jm1 + q[id - nx] * dy_j) * vfs));
+            }
+        }
+}
+
+/* ADusolver_kernel / ADvsolver_kernel, ADSolver.cu:81-120: one Jacobi sweep
+ * on interior cells, masked by iBlank; ghost cells of qn are not written. */
+static void ad_sweep(int nx, int ny, const ad_coef *c, const double *iblank,
+                     const double *s, const double *q, double *qn)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            double t = s[id] + c->xp[i] * q[id + 1] + c->xm[i] * q[id - 1]
+                             + c->yp[j] * q[id + nx] + c->ym[j] * q[id - nx];
+            qn[id] = iblank[id] * t / (c->px[i] + c->py[j]);
+        }
+}
+
+/* Compute_{u,v}Residual_AD + Reduction: sum of |qn - q| over interior cells
+ * with iBlank == 1 (ADSolver.cu:122-160, preSim.cu:376-445).  Row partials
+ * summed in row order. */
+static double l1_change(int nx, int ny, const double *iblank, const double *a,
+                        const double *b)
+{
+    double *row = calloc(ny, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++) {
+        double acc = 0.0;
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            if (iblank[id] == 1) acc += fabs(a[id] - b[id]);
+        }
+        row[j] = acc;
+    }
+    double s = 0.0;
+    for (int j = 1; j < ny - 1; j++) s += row[j];
+    free(row);
+    return s;
+}
+
+/* ImmerseFlow::ADsolver, ADSolver.cu:268-395, as coded. */
+int of_compat_ad_step(int nx, int ny, const double *dx, const double *dy,
+                      double dt, double re, int ad_itermax,
+                      const double *iblank, double *u, double *v,
+                      double *uf, double *vf, double *res_out)
+{
+    const size_t n = (size_t)nx * ny;
+    ad_coef c;
+    ad_coef_build(nx, ny, dx, dy, dt, re, &c); /* :289 */
+    /* velc buffers and the per-call scratch (:275-287; fresh scratch is
+     * modelled as zeros -- its ghosts are overwritten by the second
+     * iteration's BC before they can reach the output) */
+    double *bu[2] = {malloc(n * sizeof(double)), calloc(n, sizeof(double))};
+    double *bv[2] = {malloc(n * sizeof(double)), calloc(n, sizeof(double))};
+    double *sx = calloc(n, sizeof(double)), *sy = calloc(n, sizeof(double));
+    memcpy(bu[0], u, n * sizeof(double));
+    memcpy(bv[0], v, n * sizeof(double));
+    int cur = 0;
+
+    compat_velf(nx, ny, dx, dy, bu[cur], bv[cur], uf, vf);   /* :298 stale ghosts */
+    compat_bc(nx, ny, bu[cur], 1.0);                          /* :304 */
+    compat_bc(nx, ny, bv[cur], 0.0);
+    source_cells(nx, ny, dx, dy, dt, 0.5, bu[cur], uf, vf, sx); /* :308 */
+    source_cells(nx, ny, dx, dy, dt, 0.5, bv[cur], uf, vf, sy);
+
+    double ures = 1.0, vres = 1.0;
+    int iter = 0;
+    while (ures + vres > pow(10.0, -6.0) && iter < ad_itermax) { /* :315 */
+        compat_bc(nx, ny, bu[cur], 1.0);                          /* :317 */
+        compat_bc(nx, ny, bv[cur], 0.0);
+        compat_velf(nx, ny, dx, dy, bu[cur], bv[cur], uf, vf);   /* :321 */
+        ad_sweep(nx, ny, &c, iblank, sx, bu[cur], bu[1 - cur]);   /* :326 */
+        ad_sweep(nx, ny, &c, iblank, sy, bv[cur], bv[1 - cur]);   /* :331 */
+        compat_bc(nx, ny, bu[cur], 1.0);                          /* :337 old buffer */
+        compat_bc(nx, ny, bv[cur], 0.0);
+        cur = 1 - cur;                                            /* :342-348 swap */
+        ures = l1_change(nx, ny, iblank, bu[cur], bu[1 - cur]);   /* :350-366 */
+        vres = l1_change(nx, ny, iblank, bv[cur], bv[1 - cur]);
+        iter += 1;
+    }
+    memcpy(u, bu[cur], n * sizeof(double));
+    memcpy(v, bv[cur], n * sizeof(double));
+    if (res_out) *res_out = ures + vres;
+    free(bu[0]); free(bu[1]); free(bv[0]); free(bv[1]);
+    free(sx); free(sy);
+    ad_coef_free(&c);
+    return iter;
+}
+
+/* Poisson coefficients as intended by calculatePPECoefficients,
+ * PPESolver.cu:93-99: a_E = 2/(dx_i(dx_i+dx_ip1)) ..., a_P = -(sum). */
+static void ppe_axis(int n, const double *d, double *cp, double *cm)
+{
+    for (int i = 1; i < n - 1; i++) {
+        cp[i] = (2 / (d[i] * (d[i] + d[i + 1])));
+        cm[i] = (2 / (d[i] * (d[i] + d[i - 1])));
+    }
+    cp[0] = cm[0] = cp[n - 1] = cm[n - 1] = 0.0;
+}
+
+/* set_pressure_BC, PPESolver.cu:54-71: p = 100 on the i==0 and j==0 ghosts. */
+static void compat_pbc(int nx, int ny, double *p)
+{
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            if (i == 0) p[id] = 100.0;
+            if (j == 0) p[id] = 100.0;
+        }
+}
+
+/* ImmerseFlow::PPESolver loop, PPESolver.cu:13-50,164-195. */
+int of_compat_ppe(int nx, int ny, const double *dx, const double *dy,
+                  int ppe_itermax, double *p, double *res_out)
+{
+    const size_t n = (size_t)nx * ny;
+    double *cxp = calloc(nx, sizeof(double)), *cxm = calloc(nx, sizeof(double));
+    double *cyp = calloc(ny, sizeof(double)), *cym = calloc(ny, sizeof(double));
+    ppe_axis(nx, dx, cxp, cxm);
+    ppe_axis(ny, dy, cyp, cym);
+    double *pt = calloc(n, sizeof(double)); /* managed memory reads as zero */
+    double *row = calloc(ny, sizeof(double));
+    double *pc = p;
+    compat_pbc(nx, ny, pc);
+    double residual = 1.0;
+    int iter = 0;
+    while (residual > pow(10.0, -6.0) && iter < ppe_itermax) {
+        /* jacobiIteration, :13-31 */
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int id = IDX(i, j, nx);
+                pt[id] = pc[id];
+                if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+                    const double ap = -1 * ((cxp[i] + cxm[i]) + (cyp[j] + cym[j]));
+                    pt[id] = -(pc[id + 1] * cxp[i] + pc[id - 1] * cxm[i]
+                               + pc[id + nx] * cyp[j] + pc[id - nx] * cym[j]) / ap;
+                }
+            }
+        double *t = pc; pc = pt; pt = t; /* :178-180 */
+        /* Compute_Residual (signed A.p, no RHS) + Reduction, :33-50,:182 */
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j < ny - 1; j++) {
+            double acc = 0.0;
+            for (int i = 1; i < nx - 1; i++) {
+                const int id = IDX(i, j, nx);
+                const double ap = -1 * ((cxp[i] + cxm[i]) + (cyp[j] + cym[j]));
+                acc += pc[id] * ap + pc[id + 1] * cxp[i] + pc[id - 1] * cxm[i]
+                       + pc[id + nx] * cyp[j] + pc[id - nx] * cym[j];
+            }
+            row[j] = acc;
+        }
+        residual = 0.0;
+        for (int j = 1; j < ny - 1; j++) residual += row[j];
+        iter += 1;
+    }
+    compat_pbc(nx, ny, pc); /* :195 */
+    if (pc != p) memcpy(p, pc, n * sizeof(double));
+    if (res_out) *res_out = residual;
+    free(pc == p ? pt : pc);
+    free(cxp); free(cxm); free(cyp); free(cym); free(row);
+    return iter;
+}
+
+/* ======================================================================== */
+/* physics mode: geometry                                                   */
+/* ======================================================================== */
+
+/* Inside tests: + - * / and compares only, FP64, no contraction. */
+static int inside_body(const of_body *b, double x, double y)
+{
+    const double ex = x - b->xc, ey = y - b->yc;
+    switch (b->kind) {
+    case OF_BODY_CIRCLE:
+        return ex * ex + ey * ey <= b->r * b->r;
+    case OF_BODY_ELLIPSE: {
+        const double X = ex * b->c + ey * b->s;
+        const double Y = ey * b->c - ex * b->s;
+        return (X * X) / (b->a * b->a) + (Y * Y) / (b->b * b->b) <= 1.0;
+    }
+    case OF_BODY_POLYGON: {
+        if (!(ex * ex + ey * ey <= b->r * b->r)) return 0; /* bounding circle */
+        int in = 0;
+        const int n = b->nvert;
+        for (int k = 0; k < n; k++) {
+            const int k1 = (k + 1 == n) ? 0 : k + 1;
+            const double ax = b->vx[k], ay = b->vy[k];
+            const double bx = b->vx[k1], by = b->vy[k1];
+            if ((ay > y) != (by > y)) {
+                const double cr = (bx - ax) * (y - ay) - (by - ay) * (x - ax);
+                if (by > ay) { if (cr > 0) in = !in; }
+                else         { if (cr < 0) in = !in; }
+            }
+        }
+        return in;
+    }
+    }
+    return 0;
+}
+
+void of_classify(int nx, int ny, const double *xc, const double *yc,
+                 int nbody, const of_body *bodies, uint8_t *flags,
+                 int8_t *body_id)
+{
+    /* iBlank: 1 = fluid.  The reference's iBlankComputeKernel
+     * (preSim.cu:110-136) is a stub writing 1.0; its commented-out circle
+     * test (:126-130) is the intended classification. */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            int owner = -1;
+            for (int b = 0; b < nbody && owner < 0; b++)
+                if (inside_body(&bodies[b], xc[i], yc[j])) owner = b;
+            flags[IDX(i, j, nx)] = (owner < 0) ? OF_FLUID : 0;
+            if (body_id) body_id[IDX(i, j, nx)] = (int8_t)owner;
+        }
+    /* ghost cells and open faces on interior cells */
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            uint8_t f = flags[id] & OF_FLUID;
+            const int fe = flags[id + 1] & OF_FLUID, fw = flags[id - 1] & OF_FLUID;
+            const int fn = flags[id + nx] & OF_FLUID, fs = flags[id - nx] & OF_FLUID;
+            if (f) {
+                if (i + 1 <= nx - 2 && fe) f |= OF_OPEN_E;
+                if (i - 1 >= 1 && fw) f |= OF_OPEN_W;
+                if (j + 1 <= ny - 2 && fn) f |= OF_OPEN_N;
+                if (j - 1 >= 1 && fs) f |= OF_OPEN_S;
+            } else if (fe || fw || fn || fs) {
+                f |= OF_GHOST;
+            }
+            flags[id] = f;
+        }
+}
+
+int of_ghost_list(int nx, int ny, const uint8_t *flags, int32_t *list, int cap)
+{
+    int n = 0;
+    for (int id = 0; id < nx * ny; id++)
+        if (flags[id] & OF_GHOST) {
+            if (n < cap) list[n] = id;
+            n++;
+        }
+    return n;
+}
+
+/* Distance from a point to an ellipse: D. Eberly, "Distance from a Point to
+ * an Ellipse, an Ellipsoid, or a Hyperellipsoid" (Geometric Tools, 2013),
+ * robust bisection on F(s) = (r0 z0/(s+r0))^2 + (z1/(s+1))^2 - 1.  First
+ * quadrant, e0 >= e1 > 0, y0, y1 >= 0. */
+static void ellipse_closest_q1(double e0, double e1, double y0, double y1,
+                               double *x0, double *x1)
+{
+    if (y1 > 0) {
+        if (y0 > 0) {
+            const double z0 = y0 / e0, z1 = y1 / e1;
+            double g = z0 * z0 + z1 * z1 - 1.0;
+            if (g != 0) {
+                const double r0 = (e0 / e1) * (e0 / e1);
+                const double n0 = r0 * z0;
+                double s0 = z1 - 1.0;
+                double s1 = (g < 0) ? 0.0 : sqrt(n0 * n0 + z1 * z1) - 1.0;
+                double s = 0.0;
+                for (int it = 0; it < 200; it++) {
+                    s = (s0 + s1) / 2.0;
+                    if (s == s0 || s == s1) break;
+                    const double q0 = n0 / (s + r0), q1 = z1 / (s + 1.0);
+                    g = q0 * q0 + q1 * q1 - 1.0;
+                    if (g > 0) s0 = s;
+                    else if (g < 0) s1 = s;
+                    else break;
+                }
+                *x0 = r0 * y0 / (s + r0);
+                *x1 = y1 / (s + 1.0);
+            } else {
+                *x0 = y0;
+                *x1 = y1;
+            }
+        } else {
+            *x0 = 0.0;
+            *x1 = e1;
+        }
+    } else {
+        const double numer0 = e0 * y0, denom0 = e0 * e0 - e1 * e1;
+        if (numer0 < denom0) {
+            const double xde0 = numer0 / denom0;
+            *x0 = e0 * xde0;
+            *x1 = e1 * sqrt(1.0 - xde0 * xde0);
+        } else {
+            *x0 = e0;
+            *x1 = 0.0;
+        }
+    }
+}
+
+void of_closest_point(const of_body *b, double px, double py, double *bx,
+                      double *by)
+{
+    const double ex = px - b->xc, ey = py - b->yc;
+    switch (b->kind) {
+    case OF_BODY_CIRCLE: {
+        const double L = sqrt(ex * ex + ey * ey);
+        if (L == 0.0) { *bx = b->xc + b->r; *by = b->yc; return; }
+        *bx = b->xc + b->r * (ex / L);
+        *by = b->yc + b->r * (ey / L);
+        return;
+    }
+    case OF_BODY_ELLIPSE: {
+        /* body frame */
+        const double X = ex * b->c + ey * b->s;
+        const double Y = ey * b->c - ex * b->s;
+        const int swap = b->a < b->b;
+        const double e0 = swap ? b->b : b->a, e1 = swap ? b->a : b->b;
+        const double y0 = fabs(swap ? Y : X), y1 = fabs(swap ? X : Y);
+        double x0, x1;
+        ellipse_closest_q1(e0, e1, y0, y1, &x0, &x1);
+        double cx = swap ? x1 : x0, cy = swap ? x0 : x1;
+        if (X < 0) cx = -cx;
+        if (Y < 0) cy = -cy;
+        /* back to the world frame */
+        *bx = b->xc + (cx * b->c - cy * b->s);
+        *by = b->yc + (cx * b->s + cy * b->c);
+        return;
+    }
+    case OF_BODY_POLYGON: {
+        double best = -1.0, bxx = b->vx[0], byy = b->vy[0];
+        const int n = b->nvert;
+        for (int k = 0; k < n; k++) {
+            const int k1 = (k + 1 == n) ? 0 : k + 1;
+            const double ax = b->vx[k], ay = b->vy[k];
+            const double sx = b->vx[k1] - ax, sy = b->vy[k1] - ay;
+            const double L2 = sx * sx + sy * sy;
+            double t = (L2 > 0.0) ? ((px - ax) * sx + (py - ay) * sy) / L2 : 0.0;
+            if (t < 0.0) t = 0.0;
+            if (t > 1.0) t = 1.0;
+            const double cx = ax + t * sx, cy = ay + t * sy;
+            const double d2 = (px - cx) * (px - cx) + (py - cy) * (py - cy);
+            if (best < 0.0 || d2 < best) { best = d2; bxx = cx; byy = cy; }
+        }
+        *bx = bxx;
+        *by = byy;
+        return;
+    }
+    }
+    *bx = px;
+    *by = py;
+}
+
+/* Largest k in [0, n-2] with c[k] <= x (c strictly increasing). */
+static int bracket(int n, const double *c, double x)
+{
+    int lo = 0, hi = n - 2;
+    if (x <= c[0]) return 0;
+    if (x >= c[n - 2]) return n - 2;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* Image-point records (sharp-interface ghost-cell method, Mittal et al.,
+ * J. Comput. Phys. 227 (2008) 4825-4852): boundary intercept BI = closest
+ * boundary point, image point IP = 2 BI - GC, bilinear stencil around IP.
+ * Stencil nodes that are fluid carry weights; the ghost cell itself is
+ * eliminated implicitly; other solid nodes are taken at the body value
+ * (Dirichlet) or dropped with renormalisation (Neumann). */
+void of_ghost_weights(int nx, int ny, const double *xc, const double *yc,
+                      const uint8_t *flags, int nbody, const of_body *bodies,
+                      const int8_t *body_id, int ng, const int32_t *list,
+                      of_ghost *out)
+{
+    (void)nbody;
+#pragma omp parallel for schedule(static)
+    for (int g = 0; g < ng; g++) {
+        const int id = list[g];
+        const int gi = id % nx, gj = id / nx;
+        const int b = body_id[id] < 0 ? 0 : body_id[id];
+        const double gx = xc[gi], gy = yc[gj];
+        double bix, biy;
+        of_closest_point(&bodies[b], gx, gy, &bix, &biy);
+        const double ipx = 2.0 * bix - gx, ipy = 2.0 * biy - gy;
+        const int i0 = bracket(nx, xc, ipx), j0 = bracket(ny, yc, ipy);
+        const double fx = (ipx - xc[i0]) / (xc[i0 + 1] - xc[i0]);
+        const double fy = (ipy - yc[j0]) / (yc[j0 + 1] - yc[j0]);
+        const double w[4] = {(1.0 - fx) * (1.0 - fy), fx * (1.0 - fy),
+                             (1.0 - fx) * fy, fx * fy};
+        const int nd[4] = {IDX(i0, j0, nx), IDX(i0 + 1, j0, nx),
+                           IDX(i0, j0 + 1, nx), IDX(i0 + 1, j0 + 1, nx)};
+        double wself = 0.0, wsolid = 0.0, wfluid = 0.0;
+        for (int k = 0; k < 4; k++) {
+            if (nd[k] == id) wself += w[k];
+            else if (flags[nd[k]] & OF_FLUID) wfluid += w[k];
+            else wsolid += w[k];
+        }
+        of_ghost *o = &out[g];
+        o->gidx = id;
+        o->body = b;
+        o->cd = (2.0 - wsolid) / (1.0 + wself);
+        for (int k = 0; k < 4; k++) {
+            const int fl = (nd[k] != id) && (flags[nd[k]] & OF_FLUID);
+            o->node[k] = fl ? nd[k] : id;
+            o->wd[k] = fl ? -w[k] / (1.0 + wself) : 0.0;
+            o->wn[k] = (fl && wfluid > 0.0) ? w[k] / wfluid : 0.0;
+        }
+        o->lnx = ipx - gx;
+        o->lny = ipy - gy;
+        o->bix = bix; o->biy = biy;
+        o->ipx = ipx; o->ipy = ipy;
+    }
+}
+
+void of_phys_ghost_velocity(int nx, int ny, int ng, const of_ghost *gh,
+                            int nbody, const of_body *bodies, double *u,
+                            double *v)
+{
+    (void)nx; (void)ny; (void)nbody;
+    for (int g = 0; g < ng; g++) {
+        const of_ghost *o = &gh[g];
+        const of_body *b = &bodies[o->body];
+        double su = o->cd * b->ub, sv = o->cd * b->vb;
+        for (int k = 0; k < 4; k++) {
+            su = su + o->wd[k] * u[o->node[k]];
+            sv = sv + o->wd[k] * v[o->node[k]];
+        }
+        u[o->gidx] = su;
+        v[o->gidx] = sv;
+    }
+}
+
+void of_phys_ghost_pressure(int nx, int ny, int ng, const of_ghost *gh,
+                            int nbody, const of_body *bodies, double *p)
+{
+    (void)nx; (void)ny; (void)nbody;
+    for (int g = 0; g < ng; g++) {
+        const of_ghost *o = &gh[g];
+        const of_body *b = &bodies[o->body];
+        double sp = o->wn[0] * p[o->node[0]];
+        for (int k = 1; k < 4; k++) sp = sp + o->wn[k] * p[o->node[k]];
+        p[o->gidx] = sp + (o->lnx * b->axb + o->lny * b->ayb);
+    }
+}
+
+/* ======================================================================== */
+/* physics mode: the fractional step                                        */
+/* ======================================================================== */
+
+/* Domain ghosts: q_g = -q_in + 2 q_bc (ADSolver.cu:199-217) on the four
+ * edges; corners are never written. */
+static void domain_bc_vel(int nx, int ny, double *q, double qbc)
+{
+    for (int j = 1; j < ny - 1; j++) {
+        q[IDX(0, j, nx)] = -q[IDX(1, j, nx)] + qbc * 2.0;
+        q[IDX(nx - 1, j, nx)] = -q[IDX(nx - 2, j, nx)] + qbc * 2.0;
+    }
+    for (int i = 1; i < nx - 1; i++) {
+        q[IDX(i, 0, nx)] = -q[IDX(i, 1, nx)] + qbc * 2.0;
+        q[IDX(i, ny - 1, nx)] = -q[IDX(i, ny - 2, nx)] + qbc * 2.0;
+    }
+}
+
+/* Homogeneous Neumann pressure ghosts: p_g = p_in. */
+static void domain_bc_p(int nx, int ny, double *p)
+{
+    for (int j = 1; j < ny - 1; j++) {
+        p[IDX(0, j, nx)] = p[IDX(1, j, nx)];
+        p[IDX(nx - 1, j, nx)] = p[IDX(nx - 2, j, nx)];
+    }
+    for (int i = 1; i < nx - 1; i++) {
+        p[IDX(i, 0, nx)] = p[IDX(i, 1, nx)];
+        p[IDX(i, ny - 1, nx)] = p[IDX(i, ny - 2, nx)];
+    }
+}
+
+/* Velocity on a blanked (fluid/solid) interior face: the body velocity of
+ * the solid side. */
+static double blank_vel(const int8_t *body_id, int ida, int idb,
+                        const of_body *bodies, int comp)
+{
+    int b = body_id[ida] >= 0 ? body_id[ida] : body_id[idb];
+    if (b < 0) return 0.0;
+    return comp == 0 ? bodies[b].ub : bodies[b].vb;
+}
+
+/* Face velocities from cell velocities: linear interpolation of
+ * Compute_velf (ADSolver.cu:173-174,184-185) on open faces, body velocity
+ * on blanked faces, the Dirichlet BC on domain faces. */
+static void faces_from_cells(int nx, int ny, const double *dx, const double *dy,
+                             const uint8_t *flags, const int8_t *body_id,
+                             const of_body *bodies, const of_params *prm,
+                             const double *u, const double *v, double *uf,
+                             double *vf)
+{
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 0; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            double val;
+            if (i == 0 || i == nx - 2) val = prm->ubc;
+            else if (flags[id] & OF_OPEN_E)
+                val = (1.0 / (dx[i] + dx[i + 1])) * (u[id + 1] * dx[i] + u[id] * dx[i + 1]);
+            else val = blank_vel(body_id, id, id + 1, bodies, 0);
+            uf[i + (j - 1) * (nx - 1)] = val;
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            double val;
+            if (j == 0 || j == ny - 2) val = prm->vbc;
+            else if (flags[id] & OF_OPEN_N)
+                val = (1.0 / (dy[j] + dy[j + 1])) * (v[id + nx] * dy[j] + v[id] * dy[j + 1]);
+            else val = blank_vel(body_id, id, id + nx, bodies, 1);
+            vf[(i - 1) + j * (nx - 2)] = val;
+        }
+}
+
+/* Discrete divergence of the face field at interior cell (i,j). */
+static inline double cell_div(int nx, int i, int j, const double *dx,
+                              const double *dy, const double *uf,
+                              const double *vf)
+{
+    const double ufe = uf[i + (j - 1) * (nx - 1)], ufw = uf[i - 1 + (j - 1) * (nx - 1)];
+    const double vfn = vf[i - 1 + j * (nx - 2)], vfs = vf[i - 1 + (j - 1) * (nx - 2)];
+    return (ufe - ufw) * (1.0 / dx[i]) + (vfn - vfs) * (1.0 / dy[j]);
+}
+
+void of_phys_rhs(int nx, int ny, const double *dx, const double *dy,
+                 const uint8_t *flags, double dt, const double *uf,
+                 const double *vf, double *rhs)
+{
+    const double rdt = 1.0 / dt;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            rhs[id] = 0.0;
+            if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && (flags[id] & OF_FLUID))
+                rhs[id] = cell_div(nx, i, j, dx, dy, uf, vf) * rdt;
+        }
+}
+
+double of_phys_divergence(int nx, int ny, const double *dx, const double *dy,
+                          const uint8_t *flags, const double *uf,
+                          const double *vf, double *div)
+{
+    double *row = calloc(ny, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++) {
+        double m = 0.0;
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            double d = 0.0;
+            if (flags[id] & OF_FLUID) d = cell_div(nx, i, j, dx, dy, uf, vf);
+            if (div) div[id] = d;
+            if (fabs(d) > m) m = fabs(d);
+        }
+        row[j] = m;
+    }
+    double m = 0.0;
+    for (int j = 1; j < ny - 1; j++) if (row[j] > m) m = row[j];
+    free(row);
+    return m;
+}
+
+/* Poisson operator at fluid cell (i,j): coefficients of PPESolver.cu:93-99
+ * on open faces, zero across blanked and domain faces (Neumann). */
+typedef struct { double *xp, *xm, *yp, *ym; } ppe_coef;
+
+static void ppe_coef_build(int nx, int ny, const double *dx, const double *dy,
+                           ppe_coef *c)
+{
+    c->xp = calloc(nx, sizeof(double)); c->xm = calloc(nx, sizeof(double));
+    c->yp = calloc(ny, sizeof(double)); c->ym = calloc(ny, sizeof(double));
+    ppe_axis(nx, dx, c->xp, c->xm);
+    ppe_axis(ny, dy, c->yp, c->ym);
+}
+
+static void ppe_coef_free(ppe_coef *c)
+{
+    free(c->xp); free(c->xm); free(c->yp); free(c->ym);
+}
+
+/* r = rhs - A p at one fluid cell; returns the neighbour sum in *sum and
+ * the diagonal in *ap. */
+static inline void ppe_cell(const ppe_coef *c, int nx, int i, int j,
+                            uint8_t f, const double *p, double *ap,
+                            double *sum)
+{
+    const int id = IDX(i, j, nx);
+    const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+    const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+    const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+    const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+    *ap = -((aE + aW) + (aN + aS));
+    *sum = (aE * p[id + 1] + aW * p[id - 1]) + (aN * p[id + nx] + aS * p[id - nx]);
+}
+
+static double residual_inf(const ppe_coef *c, int nx, int ny,
+                           const uint8_t *flags, const double *rhs,
+                           const double *p, double *r)
+{
+    double *row = calloc(ny, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++) {
+        double m = 0.0;
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            const uint8_t f = flags[id];
+            double rr = 0.0;
+            if (f & OF_FLUID) {
+                double ap, sum;
+                ppe_cell(c, nx, i, j, f, p, &ap, &sum);
+                rr = rhs[id] - (ap * p[id] + sum);
+            }
+            if (r) r[id] = rr;
+            if (fabs(rr) > m) m = fabs(rr);
+        }
+        row[j] = m;
+    }
+    double m = 0.0;
+    for (int j = 1; j < ny - 1; j++) if (row[j] > m) m = row[j];
+    free(row);
+    return m;
+}
+
+double of_phys_residual(int nx, int ny, const double *dx, const double *dy,
+                        const uint8_t *flags, const double *rhs,
+                        const double *p, double *r)
+{
+    ppe_coef c;
+    ppe_coef_build(nx, ny, dx, dy, &c);
+    const double m = residual_inf(&c, nx, ny, flags, rhs, p, r);
+    ppe_coef_free(&c);
+    return m;
+}
+
+/* Red-black SOR sweep (colour (i+j)&1 == 0 first), the parallel form of the
+ * deck's PPE_Solver 1 "Point GS" with relaxation w-PPE (inputs.txt:14-15);
+ * Jacobi (jacobiIteration, PPESolver.cu:13-31, with the RHS) when
+ * ppe_solver == 0. */
+static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
+                      const double *rhs, const of_params *prm, double *p,
+                      double *tmp)
+{
+    const double om = prm->ppe_omega;
+    if (prm->ppe_solver == 0) {
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 1; i < nx - 1; i++) {
+                const int id = IDX(i, j, nx);
+                const uint8_t f = flags[id];
+                double val = p[id];
+                if (f & OF_FLUID) {
+                    double ap, sum;
+                    ppe_cell(c, nx, i, j, f, p, &ap, &sum);
+                    if (ap != 0.0) val = (rhs[id] - sum) / ap;
+                }
+                tmp[id] = val;
+            }
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 1; i < nx - 1; i++) p[IDX(i, j, nx)] = tmp[IDX(i, j, nx)];
+        return;
+    }
+    for (int colour = 0; colour < 2; colour++) {
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 1 + ((j + 1 + colour) & 1); i < nx - 1; i += 2) {
+                const int id = IDX(i, j, nx);
+                const uint8_t f = flags[id];
+                if (!(f & OF_FLUID)) continue;
+                double ap, sum;
+                ppe_cell(c, nx, i, j, f, p, &ap, &sum);
+                if (ap == 0.0) continue;
+                const double gs = (rhs[id] - sum) / ap;
+                p[id] = p[id] + om * (gs - p[id]);
+            }
+    }
+}
+
+int of_phys_poisson(int nx, int ny, const double *dx, const double *dy,
+                    const uint8_t *flags, const double *rhs,
+                    const of_params *prm, double *p, double *res_out)
+{
+    ppe_coef c;
+    ppe_coef_build(nx, ny, dx, dy, &c);
+    double *tmp = prm->ppe_solver == 0 ? calloc((size_t)nx * ny, sizeof(double)) : NULL;
+    const int every = prm->check_every > 0 ? prm->check_every : 1;
+    int sweeps = 0;
+    double res = prm->dt * residual_inf(&c, nx, ny, flags, rhs, p, NULL);
+    while (res > prm->ppe_tol && sweeps < prm->ppe_itermax) {
+        int k = every;
+        if (sweeps + k > prm->ppe_itermax) k = prm->ppe_itermax - sweeps;
+        for (int s = 0; s < k; s++) ppe_sweep(&c, nx, ny, flags, rhs, prm, p, tmp);
+        sweeps += k;
+        res = prm->dt * residual_inf(&c, nx, ny, flags, rhs, p, NULL);
+    }
+    if (res_out) *res_out = res;
+    free(tmp);
+    ppe_coef_free(&c);
+    return sweeps;
+}
+
+static int8_t *no_body_ids(int nx, int ny)
+{
+    int8_t *b = malloc((size_t)nx * ny);
+    memset(b, -1, (size_t)nx * ny);
+    return b;
+}
+
+void of_phys_init(int nx, int ny, const double *dx, const double *dy,
+                  const uint8_t *flags, int nbody, const of_body *bodies,
+                  int ng, const of_ghost *gh, const of_params *prm,
+                  double *u, double *v, double *uf, double *vf, double *p)
+{
+    /* fluid cells start at the free stream, solid cells at zero; ghost
+     * cells then take their image-point values.  Body ownership of the
+     * cells next to the fluid is recovered from the ghost records. */
+    const size_t n = (size_t)nx * ny;
+    int8_t *bid = no_body_ids(nx, ny);
+    for (int g = 0; g < ng; g++) bid[gh[g].gidx] = (int8_t)gh[g].body;
+    for (size_t id = 0; id < n; id++) {
+        u[id] = (flags[id] & OF_FLUID) ? prm->ubc : 0.0;
+        v[id] = (flags[id] & OF_FLUID) ? prm->vbc : 0.0;
+        p[id] = 0.0;
+    }
+    domain_bc_vel(nx, ny, u, prm->ubc);
+    domain_bc_vel(nx, ny, v, prm->vbc);
+    of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, u, v);
+    faces_from_cells(nx, ny, dx, dy, flags, bid, bodies, prm, u, v, uf, vf);
+    free(bid);
+}
+
+void of_phys_step(int nx, int ny, const double *dx, const double *dy,
+                  const uint8_t *flags, int nbody, const of_body *bodies,
+                  int ng, const of_ghost *gh, const of_params *prm,
+                  double *u, double *v, double *uf, double *vf, double *p,
+                  of_stats *st)
+{
+    const size_t n = (size_t)nx * ny;
+    int8_t *bid = no_body_ids(nx, ny);
+    for (int g = 0; g < ng; g++) bid[gh[g].gidx] = (int8_t)gh[g].body;
+    double *iblank = malloc(n * sizeof(double));
+    for (size_t id = 0; id < n; id++) iblank[id] = (flags[id] & OF_FLUID) ? 1.0 : 0.0;
+
+    ad_coef ac;
+    ad_coef_build(nx, ny, dx, dy, prm->dt, prm->re, &ac);
+    double *sx = calloc(n, sizeof(double)), *sy = calloc(n, sizeof(double));
+    double *un = malloc(n * sizeof(double)), *vn = malloc(n * sizeof(double));
+    double *rhs = calloc(n, sizeof(double));
+
+    /* P1: explicit convective source from the projected face velocities */
+    source_cells(nx, ny, dx, dy, prm->dt, prm->conv_factor, u, uf, vf, sx);
+    source_cells(nx, ny, dx, dy, prm->dt, prm->conv_factor, v, uf, vf, sy);
+
+    /* P2: backward-Euler viscous Jacobi sweeps (ADSolver.cu:315-370) */
+    memcpy(un, u, n * sizeof(double));
+    memcpy(vn, v, n * sizeof(double));
+    double res = 1.0 + 1.0;
+    int iter = 0;
+    double *uc = u, *vc = v, *uo = un, *vo = vn;
+    while (res > prm->ad_tol && iter < prm->ad_itermax) {
+        ad_sweep(nx, ny, &ac, iblank, sx, uc, uo);
+        ad_sweep(nx, ny, &ac, iblank, sy, vc, vo);
+        domain_bc_vel(nx, ny, uo, prm->ubc);
+        domain_bc_vel(nx, ny, vo, prm->vbc);
+        of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, uo, vo);
+        res = l1_change(nx, ny, iblank, uo, uc) + l1_change(nx, ny, iblank, vo, vc);
+        double *t = uc; uc = uo; uo = t;
+        t = vc; vc = vo; vo = t;
+        iter++;
+    }
+    if (uc != u) { memcpy(u, uc, n * sizeof(double)); memcpy(v, vc, n * sizeof(double)); }
+
+    /* P3: intermediate face velocities and the divergence RHS */
+    faces_from_cells(nx, ny, dx, dy, flags, bid, bodies, prm, u, v, uf, vf);
+    of_phys_rhs(nx, ny, dx, dy, flags, prm->dt, uf, vf, rhs);
+
+    /* P4/P5: pressure Poisson, warm-started from p^n */
+    double pres;
+    const int sweeps = of_phys_poisson(nx, ny, dx, dy, flags, rhs, prm, p, &pres);
+    domain_bc_p(nx, ny, p);
+    of_phys_ghost_pressure(nx, ny, ng, gh, nbody, bodies, p);
+
+    /* P7: projection of faces (open faces only) and cells */
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            const uint8_t f = flags[id];
+            if (f & OF_OPEN_E) {
+                const double g = (p[id + 1] - p[id]) * (2.0 / (dx[i] + dx[i + 1]));
+                uf[i + (j - 1) * (nx - 1)] = uf[i + (j - 1) * (nx - 1)] - prm->dt * g;
+            }
+            if (f & OF_OPEN_N) {
+                const double g = (p[id + nx] - p[id]) * (2.0 / (dy[j] + dy[j + 1]));
+                vf[(i - 1) + j * (nx - 2)] = vf[(i - 1) + j * (nx - 2)] - prm->dt * g;
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            const int id = IDX(i, j, nx);
+            const double dxi = dx[i], dxp = dx[i + 1], dxm = dx[i - 1];
+            const double dyj = dy[j], dyp = dy[j + 1], dym = dy[j - 1];
+            const double pe = (1 / (dxi + dxp)) * (p[id + 1] * dxi + p[id] * dxp);
+            const double pw = (1 / (dxi + dxm)) * (p[id] * dxm + p[id - 1] * dxi);
+            const double pn = (1 / (dyj + dyp)) * (p[id + nx] * dyj + p[id] * dyp);
+            const double ps = (1 / (dyj + dym)) * (p[id] * dym + p[id - nx] * dyj);
+            un[id] = iblank[id] * (u[id] - prm->dt * ((pe - pw) * (1.0 / dxi)));
+            vn[id] = iblank[id] * (v[id] - prm->dt * ((pn - ps) * (1.0 / dyj)));
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+            u[IDX(i, j, nx)] = un[IDX(i, j, nx)];
+            v[IDX(i, j, nx)] = vn[IDX(i, j, nx)];
+        }
+    domain_bc_vel(nx, ny, u, prm->ubc);
+    domain_bc_vel(nx, ny, v, prm->vbc);
+    of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, u, v);
+
+    /* P8: divergence check */
+    const double dmax = of_phys_divergence(nx, ny, dx, dy, flags, uf, vf, NULL);
+    if (st) {
+        st->ad_iters = iter;
+        st->ad_res = res;
+        st->ppe_sweeps = sweeps;
+        st->ppe_res = pres;
+        st->div_max = dmax;
+    }
+    free(bid); free(iblank); free(sx); free(sy); free(un); free(vn); free(rhs);
+    ad_coef_free(&ac);
+}
