@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Throughput of the fractional step on one MI355X.
+
+Workload: the single large case -- an ellipse (chord 1, thickness ratio 0.1)
+at 10 deg incidence, Re = 300, on a 2048 x 1024 grid (uniform h = 0.005 in a
+4 x 2 box around the body, stretched to a 30 x 20 domain), dt = 0.5 h.  A
+step is one full fractional step (predictor, divergence RHS, Poisson to
+max|div| <= 1e-7, projection, divergence check) on device-resident fields.
+
+value = interior cells x steps / wall seconds (Mcell-steps/s) of the
+mixed-precision path (FP32 stencils and Poisson sweeps, FP64 refinement,
+face velocities, residuals and divergence); the FP64 path is timed beside
+it.  roofline: the Poisson sweep kernel (the dominant kernel), algorithmic
+bytes per launch / mean launch time from HIP events recorded by the library
+on its stream during the timed steps.  cpu_baseline: the C oracle (the same
+per-cell routines, OpenMP) timed for one step from the GPU's state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--tile-y", type=int, default=16)
+    ap.add_argument("--sweeps-per-launch", type=int, default=1)
+    ap.add_argument("--check-every", type=int, default=20)
+    ap.add_argument("--omega", type=float, default=0.0)
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_case(iff, cfg, precision, a):
+    c = iff.Case(cfg["xf"], cfg["yf"], precision=precision, dt=cfg["dt"], re=cfg["re"],
+                 bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_RBSOR,
+                 ppe_omega=a.omega, ppe_itermax=200000, ppe_tol=cfg["ppe_tol"],
+                 check_every=a.check_every)
+    c.tune(a.tile_y, a.sweeps_per_launch, a.check_every)
+    return c
+
+
+def timed_run(case, warmup, steps, cells, label):
+    for k in range(warmup):
+        st = case.step(1)
+        log(label, "warmup", k, "sweeps", st["ppe_sweeps"], "div", "%.2e" % st["div_max"])
+    case.sweep_timing(reset=True)
+    t0 = time.perf_counter()
+    st = case.step(steps)
+    t1 = time.perf_counter()
+    ms, launches = case.sweep_timing(reset=True)
+    dt = t1 - t0
+    log(label, "timed", steps, "steps in %.3f s" % dt, "last sweeps", st["ppe_sweeps"],
+        "total sweeps", st["total_ppe_sweeps"], "max div %.2e" % st["max_div"])
+    return dict(seconds=dt, stats=st, sweep_ms=ms, sweep_launches=launches,
+                mcs=cells * steps / dt / 1e6)
+
+
+def cpu_baseline(cfg, gpu_case, st):
+    """One oracle step (FP64, OpenMP) from the GPU case's state."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    prm = dict(dt=cfg["dt"], re=cfg["re"], conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25,
+               ad_tol=1e-6, ppe_solver=1, ppe_omega=0.0, ppe_itermax=max(1, 2 * st["ppe_sweeps"]),
+               ppe_tol=cfg["ppe_tol"], check_every=20)
+    bodies = []
+    for b in cfg["bodies"]:
+        d = dict(b)
+        d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
+        bodies.append(d)
+    oc = O.PhysCase(cfg["xf"], cfg["yf"], bodies, prm)
+    oc.prm.ppe_omega = gpu_case_omega(oc)
+    for name in ("u", "v", "p", "uf", "vf"):
+        getattr(oc, name)[:] = gpu_case.get(name)
+    t0 = time.perf_counter()
+    out = oc.step(1)[-1]
+    t1 = time.perf_counter()
+    cells = (oc.nx - 2) * (oc.ny - 2)
+    return dict(value=round(cells / (t1 - t0) / 1e6, 4), unit="Mcell-steps/s", cores=threads,
+                kind="port",
+                sample="1 FP64 step of the same case from the GPU state (%d AD sweeps, %d "
+                       "Poisson sweeps, %.1f s)" % (out[0], out[2], t1 - t0))
+
+
+def gpu_case_omega(oc):
+    import math
+    n = max(oc.nx - 2, oc.ny - 2)
+    return 2.0 / (1.0 + math.sin(math.pi / n))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import immerseflow_amd as iff
+    from immerseflow_amd import configs
+    n, arch, cus, mem = iff.device_query()
+    log("device", arch, cus, "CUs")
+    cfg = configs.c2()
+    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+    cells = NX * NY
+    log("workload", cfg["name"], "ratios", ["%.6f" % r for r in cfg["ratios"]])
+
+    case = make_case(iff, cfg, iff.IF_MIXED, a)
+    mixed = timed_run(case, a.warmup, a.steps, cells, "mixed")
+    sweep_bpc, step_bpc = case.byte_model()
+    avg_ms = mixed["sweep_ms"] / max(1, mixed["sweep_launches"])
+    achieved = sweep_bpc * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_rbsor_c2.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            traffic = json.load(f).get("mixed_bytes_per_launch")
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="k_rbsor<float,%d,%d>" % (a.tile_y, a.sweeps_per_launch),
+                bytes_per_launch=sweep_bpc * cells, mean_launch_us=round(avg_ms * 1e3, 3),
+                launches_timed=mixed["sweep_launches"])
+
+    fp64 = None
+    if not a.no_fp64 and a.fp64_steps > 0:
+        c64 = make_case(iff, cfg, iff.IF_FP64, a)
+        r64 = timed_run(c64, max(1, a.warmup), a.fp64_steps, cells, "fp64")
+        b64, _ = c64.byte_model()
+        avg64 = r64["sweep_ms"] / max(1, r64["sweep_launches"])
+        fp64 = dict(value=round(r64["mcs"], 3), ms_per_step=round(1e3 * r64["seconds"] / a.fp64_steps, 3),
+                    sweeps_last_step=r64["stats"]["ppe_sweeps"],
+                    max_div=r64["stats"]["max_div"],
+                    sweep_gbs=round(b64 * cells / (avg64 * 1e-3) / 1e9, 1) if avg64 > 0 else None)
+        c64.close()
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(cfg, case, mixed["stats"])
+        log("cpu baseline", cpu)
+
+    st = mixed["stats"]
+    out = {
+        "metric": "Mcell-steps/s + % HBM roofline; batched cases/s and p50/p99 latency at div<=1e-6",
+        "value": round(mixed["mcs"], 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * mixed["seconds"] / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "NX": NX, "NY": NY, "dt": cfg["dt"], "Re": cfg["re"],
+                   "precision": "mixed (FP32 stencils/sweeps, FP64 refinement)",
+                   "poisson": "red-black SOR, stop max|div|<=%g" % cfg["ppe_tol"],
+                   "sweeps_last_step": st["ppe_sweeps"], "ad_iters_last_step": st["ad_iters"],
+                   "max_div": st["max_div"], "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": fp64,
+        "step_bytes_per_cell": round(step_bpc, 1),
+    }
+    case.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,98 @@
+// Host runtime of the physics-mode fractional step: resident cases, batched
+// launches, device-side loop control.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+
+namespace iff {
+
+// A case resident in HBM (one device arena holds all its arrays).
+struct Slot {
+    int id = -1;
+    int state = -2;          // 0 running, 1 finished, -1 failed, -2 empty
+    int prec = IF_FP64;
+    int nx = 0, ny = 0, P = 0, NX = 0, NY = 0;
+    int ntx = 0, nty = 0;    // base tiling (kTileX x kTileY)
+    int gcap = 0, ng = 0;
+    bool moving = false;
+    int steps_host = 0;      // steps enqueued while running
+    int steps_target = 0;
+    double t0 = 0.0;
+    if_case_desc desc{};
+    std::vector<double> xf, yf, xc, yc, dx, dy;
+    std::vector<if_body> bodies;
+    std::vector<std::vector<double>> polyx, polyy;
+    std::vector<BodyDev> bodies_dev;   // host image (device pointers inside)
+    void *arena = nullptr;
+    size_t bytes = 0;
+    int *ghost_counts = nullptr;       // scratch for the ghost compaction
+    int ghost_blocks = 0;
+    CaseDev dev{};                     // host image of the device descriptor
+    if_step_stats last{};
+};
+
+class Engine {
+public:
+    Engine(int prec, size_t budget);
+    ~Engine();
+    if_status init();
+
+    if_status add(const if_case_desc &d, int nsteps, int *slot);
+    if_status remove(int slot);
+    if_status step(int nsteps, int *finished, if_step_stats *agg);
+    if_status slot_state(int slot, int *state, int *steps_done, if_step_stats *st);
+    if_status get_field(int slot, int which, double *out, size_t n);
+    if_status set_field(int slot, int which, const double *in, size_t n);
+    if_status ghosts(int slot, int32_t *out, if_ghost *recs, size_t cap, size_t *n);
+    if_status set_target(int slot, int nsteps);
+    if_status sync();
+
+    int resident() const;
+    size_t in_use() const { return used_; }
+    size_t budget() const { return budget_; }
+    Slot *slot(int k) { return (k >= 0 && k < (int)slots_.size()) ? slots_[k].get() : nullptr; }
+
+    // Poisson sweep variant and check cadence (autotuned or user-set)
+    int sweep_ty = 16, sweep_m = 1, check_k = 10;
+    bool timing = false;
+    double sweep_ms = 0.0;
+    long long sweep_launches = 0;
+    int spec_depth = 3;
+
+    static size_t footprint(const if_case_desc &d, int prec);
+    void mark_dirty() { dirty_ = true; }
+
+private:
+    if_status build_slot(Slot &s, const if_case_desc &d);
+    if_status refresh();      // re-upload the resident case list / tile tables
+    if_status snapshot(int ring_slot);
+    bool all_done(const Ctrl *snap, int which) const;
+    if_status read_ctrl_sync(std::vector<Ctrl> &out);
+    if_status update_moving();
+
+    int prec_;
+    size_t budget_, used_ = 0;
+    hipStream_t st_ = nullptr;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    std::vector<int> order_;           // resident slot ids, launch order
+    CaseDev *d_cases_ = nullptr;
+    int cap_cases_ = 0;
+    Ctrl *d_ctrl_ = nullptr;           // one control block per slot id
+    int cap_ctrl_ = 4096;
+    int2 *d_tiles_base_ = nullptr, *d_tiles_sweep_ = nullptr;
+    int nt_base_ = 0, nt_sweep_ = 0, cap_tiles_base_ = 0, cap_tiles_sweep_ = 0;
+    int tiles_sweep_ty_ = 0;
+    int max_ng_ = 0, max_ring_ = 0, max_cells_ = 0, max_ad_ = 0, max_ppe_ = 0;
+    bool any_moving_ = false;
+    bool dirty_ = true;
+    // speculation ring
+    static constexpr int kRing = 8;
+    Ctrl *h_snap_ = nullptr;           // pinned, kRing * cap_ctrl_
+    hipEvent_t ev_[kRing] = {};
+    hipEvent_t tev_[2 * 64] = {};      // timing events
+};
+
+}  // namespace iff

@@ -1,0 +1,948 @@
+// Host runtime of the physics-mode fractional step.
+//
+// One Engine owns a stream and a set of resident cases ("slots"), each with
+// its own device arena.  A step enqueues the fixed kernel sequence of the
+// fractional step over the tiles of all running cases:
+//
+//   begin -> P1 source -> [P2 viscous sweep + ghost cells + finalize] x K_AD
+//   -> P3 faces + RHS -> [P4 Poisson sweeps + P5 check] x chunks
+//   -> pressure BC -> P7/P8 projection + divergence -> ghost cells -> end
+//
+// The two iterative loops end on the device (per-case flags in Ctrl).  The
+// host enqueues loop units speculatively and watches asynchronous snapshots
+// of the control blocks, so it stops enqueueing a few units after the last
+// case converged instead of waiting for a result after every unit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "engine.h"
+#include "io.h"
+#include "kernels.h"
+
+namespace iff {
+
+constexpr int kTileY = 16;  // base tiling (reductions, stencil passes)
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget) {}
+
+Engine::~Engine()
+{
+    if (st_) hipStreamSynchronize(st_);
+    for (auto &s : slots_)
+        if (s && s->arena) hipFree(s->arena);
+    for (auto &s : slots_)
+        if (s && s->dev.gh) hipFree(s->dev.gh);
+    if (d_cases_) hipFree(d_cases_);
+    if (d_ctrl_) hipFree(d_ctrl_);
+    if (d_tiles_base_) hipFree(d_tiles_base_);
+    if (d_tiles_sweep_) hipFree(d_tiles_sweep_);
+    if (h_snap_) hipHostFree(h_snap_);
+    for (auto &e : ev_)
+        if (e) hipEventDestroy(e);
+    for (auto &e : tev_)
+        if (e) hipEventDestroy(e);
+    if (st_) hipStreamDestroy(st_);
+}
+
+if_status Engine::init()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(IF_ERR_NODEV, "no HIP device available");
+    IFF_HIP(hipStreamCreate(&st_));  // default priority
+    IFF_HIP(hipMalloc(&d_ctrl_, sizeof(Ctrl) * cap_ctrl_));
+    IFF_HIP(hipMemsetAsync(d_ctrl_, 0, sizeof(Ctrl) * cap_ctrl_, st_));
+    IFF_HIP(hipHostMalloc((void **)&h_snap_, sizeof(Ctrl) * cap_ctrl_ * kRing, hipHostMallocDefault));
+    for (auto &e : ev_) IFF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : tev_) IFF_HIP(hipEventCreate(&e));
+    IFF_HIP(hipStreamSynchronize(st_));
+    return IF_OK;
+}
+
+int Engine::resident() const
+{
+    int n = 0;
+    for (auto &s : slots_)
+        if (s && s->state != -2) n++;
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// arena layout
+// ---------------------------------------------------------------------------
+struct ArenaPlan {
+    size_t off_u[2], off_v[2], off_sx, off_sy;
+    size_t off_uf[2], off_vf[2], off_p[2], off_rhs;
+    size_t off_e[2], off_r32;
+    size_t off_flags, off_bodyid;
+    size_t off_metd, off_xc, off_yc, off_metf;
+    size_t off_bodies, off_poly, off_part, off_gcount;
+    size_t total;
+};
+
+static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbody,
+                            size_t npoly, int gblocks)
+{
+    ArenaPlan a{};
+    size_t o = 0;
+    const size_t ts = prec == IF_FP64 ? 8 : 4;
+    auto fieldT = [&](size_t es) {
+        const size_t lead = 256 / es - 1;
+        const size_t n = (size_t)P * ny + lead + 64;
+        const size_t at = o;
+        o = align256(o + n * es);
+        return at + lead * es;  // pre-offset pointer position
+    };
+    for (int b = 0; b < 2; b++) a.off_u[b] = fieldT(ts);
+    for (int b = 0; b < 2; b++) a.off_v[b] = fieldT(ts);
+    a.off_sx = fieldT(ts);
+    a.off_sy = fieldT(ts);
+    for (int b = 0; b < 2; b++) a.off_uf[b] = fieldT(8);
+    for (int b = 0; b < 2; b++) a.off_vf[b] = fieldT(8);
+    for (int b = 0; b < 2; b++) a.off_p[b] = fieldT(8);
+    a.off_rhs = fieldT(8);
+    if (prec == IF_MIXED) {
+        for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
+        a.off_r32 = fieldT(4);
+    }
+    a.off_flags = fieldT(1);
+    a.off_bodyid = fieldT(1);
+    a.off_metd = o;
+    o = align256(o + sizeof(double) * (size_t)kAxisDVecs * (nx + ny));
+    a.off_xc = o;
+    o = align256(o + sizeof(double) * nx);
+    a.off_yc = o;
+    o = align256(o + sizeof(double) * ny);
+    a.off_metf = o;
+    o = align256(o + sizeof(float) * (size_t)kAxisFVecs * (nx + ny));
+    a.off_bodies = o;
+    o = align256(o + sizeof(BodyDev) * std::max(1, nbody));
+    a.off_poly = o;
+    o = align256(o + sizeof(double) * std::max<size_t>(1, npoly));
+    a.off_part = o;
+    o = align256(o + sizeof(double) * std::max(1, ntiles));
+    a.off_gcount = o;
+    o = align256(o + sizeof(int) * std::max(1, gblocks));
+    a.total = o;
+    return a;
+}
+
+static int grid_n(const if_case_desc &d, int *nx, int *ny)
+{
+    *nx = d.nxf + 1;
+    *ny = d.nyf + 1;
+    return 0;
+}
+
+size_t Engine::footprint(const if_case_desc &d, int prec)
+{
+    int nx, ny;
+    grid_n(d, &nx, &ny);
+    const int P = pitch_for(nx);
+    const int ntx = (nx - 2 + kTileX - 1) / kTileX, nty = (ny - 2 + kTileY - 1) / kTileY;
+    size_t npoly = 0;
+    for (int b = 0; b < d.nbody; b++)
+        if (d.bodies[b].kind == IF_BODY_POLYGON) npoly += 2 * (size_t)d.bodies[b].nvert;
+    const ArenaPlan a = plan_arena(prec, nx, ny, P, ntx * nty, d.nbody, npoly,
+                                   (nx * ny + ghost_span() - 1) / ghost_span());
+    // ghost records: a perimeter-sized estimate for the budget
+    const size_t gest = (size_t)(8 * (nx + ny));
+    return a.total + gest * sizeof(GhostDev);
+}
+
+// ---------------------------------------------------------------------------
+// host metric vectors (operation order of the reference kernels)
+// ---------------------------------------------------------------------------
+static void axis_vectors(const std::vector<double> &d, double dt, double re, double conv,
+                         bool is_x, std::vector<double> &V /* kAxisDVecs*n */,
+                         std::vector<float> &F /* kAxisFVecs*n */)
+{
+    const int n = (int)d.size();
+    V.assign((size_t)kAxisDVecs * n, 0.0);
+    F.assign((size_t)kAxisFVecs * n, 0.0f);
+    double *vd = &V[0 * n], *ip = &V[1 * n], *im = &V[2 * n], *cs = &V[3 * n], *rd = &V[4 * n],
+           *g = &V[5 * n], *cp = &V[6 * n], *cm = &V[7 * n], *ap = &V[8 * n], *am = &V[9 * n],
+           *ad = &V[10 * n];
+    const double k = dt / re;
+    for (int i = 0; i < n; i++) {
+        vd[i] = d[i];
+        cs[i] = (dt / d[i]) * conv;
+        rd[i] = 1.0 / d[i];
+        if (i + 1 < n) {
+            ip[i] = 1 / (d[i] + d[i + 1]);
+            g[i] = 2.0 / (d[i] + d[i + 1]);
+        }
+        if (i >= 1) im[i] = 1 / (d[i] + d[i - 1]);
+        if (i >= 1 && i <= n - 2) {
+            const double di = d[i], dp = d[i + 1], dm = d[i - 1];
+            cp[i] = (2 / (di * (di + dp)));
+            cm[i] = (2 / (di * (di + dm)));
+            ap[i] = k * (2 / (di * (di + dp)));
+            am[i] = k * (2 / (di * (di + dm)));
+            ad[i] = is_x ? 1.0 + k * ((2 / (di * (di + dp))) + (2 / (di * (di + dm))))
+                         : k * ((2 / (di * (di + dp))) + (2 / (di * (di + dm))));
+        }
+    }
+    const double *src[kAxisFVecs] = {vd, ip, im, cs, ap, am, ad, cp, cm};
+    for (int v = 0; v < kAxisFVecs; v++)
+        for (int i = 0; i < n; i++) F[(size_t)v * n + i] = (float)src[v][i];
+}
+
+static BodyDev body_at(const if_body &b, const double *dvx, const double *dvy, double t,
+                       double dt_for_state)
+{
+    (void)dt_for_state;
+    BodyDev o{};
+    o.kind = b.kind;
+    o.xc = b.xc;
+    o.yc = b.yc;
+    o.r = b.r;
+    o.a = b.a;
+    o.b = b.b;
+    o.c = b.cth;
+    o.s = b.sth;
+    o.nvert = b.nvert;
+    o.vx = dvx;
+    o.vy = dvy;
+    if (b.motion == IF_MOTION_HEAVE) {
+        const double w = 2.0 * M_PI * b.freq;
+        o.yc = b.yc + b.amp * std::sin(w * t);
+        o.vb = b.amp * w * std::cos(w * t);
+        o.ayb = -b.amp * w * w * std::sin(w * t);
+    }
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// case creation: allocation + readGridData + initializeData equivalents
+// ---------------------------------------------------------------------------
+if_status Engine::build_slot(Slot &s, const if_case_desc &d)
+{
+    if (d.nxf < 3 || d.nyf < 3 || !d.xf || !d.yf)
+        return fail(IF_ERR_ARG, "case: need >= 3 face coordinates per axis");
+    if (!(d.dt > 0) || !(d.re > 0)) return fail(IF_ERR_ARG, "case: dt and Re must be > 0");
+    if (d.nbody < 0 || d.nbody > IF_MAX_BODIES || (d.nbody > 0 && !d.bodies))
+        return fail(IF_ERR_ARG, "case: bad body list");
+    if (d.ppe_solver != IF_PPE_JACOBI && d.ppe_solver != IF_PPE_RBSOR)
+        return fail(IF_ERR_ARG, "case: unknown Poisson solver");
+    if (d.ad_itermax < 0 || d.ppe_itermax < 0) return fail(IF_ERR_ARG, "case: negative itermax");
+    s.desc = d;
+    s.prec = prec_;
+    s.xf.assign(d.xf, d.xf + d.nxf);
+    s.yf.assign(d.yf, d.yf + d.nyf);
+    for (int i = 1; i < d.nxf; i++)
+        if (!(s.xf[i] > s.xf[i - 1])) return fail(IF_ERR_ARG, "case: x faces not increasing");
+    for (int i = 1; i < d.nyf; i++)
+        if (!(s.yf[i] > s.yf[i - 1])) return fail(IF_ERR_ARG, "case: y faces not increasing");
+    s.desc.xf = s.xf.data();
+    s.desc.yf = s.yf.data();
+    s.bodies.assign(d.bodies, d.bodies + d.nbody);
+    s.polyx.resize(d.nbody);
+    s.polyy.resize(d.nbody);
+    size_t npoly = 0;
+    for (int b = 0; b < d.nbody; b++) {
+        const if_body &B = s.bodies[b];
+        if (B.kind == IF_BODY_POLYGON) {
+            if (B.nvert < 3 || !B.vx || !B.vy) return fail(IF_ERR_ARG, "case: bad polygon");
+            s.polyx[b].assign(B.vx, B.vx + B.nvert);
+            s.polyy[b].assign(B.vy, B.vy + B.nvert);
+            npoly += 2 * (size_t)B.nvert;
+        } else if (B.kind == IF_BODY_ELLIPSE) {
+            if (!(B.a > 0) || !(B.b > 0)) return fail(IF_ERR_ARG, "case: bad ellipse");
+        } else if (B.kind == IF_BODY_CIRCLE) {
+            if (!(B.r > 0)) return fail(IF_ERR_ARG, "case: bad circle");
+        } else {
+            return fail(IF_ERR_ARG, "case: unknown body kind");
+        }
+        if (B.motion == IF_MOTION_HEAVE) s.moving = true;
+    }
+    s.desc.bodies = s.bodies.data();
+    grid_n(d, &s.nx, &s.ny);
+    s.NX = s.nx - 2;
+    s.NY = s.ny - 2;
+    s.P = pitch_for(s.nx);
+    s.xc.resize(s.nx);
+    s.dx.resize(s.nx);
+    s.yc.resize(s.ny);
+    s.dy.resize(s.ny);
+    grid_metric(d.nxf, s.xf.data(), s.xc.data(), s.dx.data());
+    grid_metric(d.nyf, s.yf.data(), s.yc.data(), s.dy.data());
+    s.ntx = (s.NX + kTileX - 1) / kTileX;
+    s.nty = (s.NY + kTileY - 1) / kTileY;
+    s.t0 = d.t0;
+    s.ghost_blocks = (s.nx * s.ny + ghost_span() - 1) / ghost_span();
+
+    const ArenaPlan a = plan_arena(prec_, s.nx, s.ny, s.P, s.ntx * s.nty, d.nbody, npoly,
+                                   s.ghost_blocks);
+    if (budget_ && used_ + a.total > budget_)
+        return fail(IF_ERR_BUDGET, "case footprint exceeds the remaining HBM budget");
+    if (hipMalloc(&s.arena, a.total) != hipSuccess) {
+        s.arena = nullptr;
+        return fail(IF_ERR_NOMEM, "device allocation of " + std::to_string(a.total) + " bytes failed");
+    }
+    s.bytes = a.total;
+    char *A = (char *)s.arena;
+    IFF_HIP(hipMemsetAsync(s.arena, 0, a.total, st_));
+
+    CaseDev &c = s.dev;
+    std::memset(&c, 0, sizeof(c));
+    c.nx = s.nx;
+    c.ny = s.ny;
+    c.P = s.P;
+    c.ntx = s.ntx;
+    c.nty = s.nty;
+    c.tile_y = kTileY;
+    c.nbody = d.nbody;
+    c.moving = s.moving ? 1 : 0;
+    c.prec = prec_;
+    c.ppe_solver = d.ppe_solver;
+    c.ad_itermax = d.ad_itermax;
+    c.ppe_itermax = d.ppe_itermax;
+    c.check_every = d.check_every > 0 ? d.check_every : 10;
+    c.dt = d.dt;
+    c.rdt = 1.0 / d.dt;
+    c.omega = d.ppe_omega;
+    if (!(c.omega > 0)) {
+        const int N = std::max(s.NX, s.NY);
+        c.omega = 2.0 / (1.0 + std::sin(M_PI / N));
+    }
+    c.omega_f = (float)c.omega;
+    c.ppe_tol = d.ppe_tol;
+    c.ad_tol = d.ad_tol;
+    c.ubc = d.ubc;
+    c.vbc = d.vbc;
+    for (int b = 0; b < 2; b++) {
+        c.u[b] = A + a.off_u[b];
+        c.v[b] = A + a.off_v[b];
+        c.uf[b] = (double *)(A + a.off_uf[b]);
+        c.vf[b] = (double *)(A + a.off_vf[b]);
+        c.p[b] = (double *)(A + a.off_p[b]);
+        if (prec_ == IF_MIXED) c.e[b] = (float *)(A + a.off_e[b]);
+    }
+    c.sx = A + a.off_sx;
+    c.sy = A + a.off_sy;
+    c.rhs = (double *)(A + a.off_rhs);
+    if (prec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
+    c.flags = (uint8_t *)(A + a.off_flags);
+    c.bodyid = (int8_t *)(A + a.off_bodyid);
+    c.part = (double *)(A + a.off_part);
+    c.gcount = (int *)(A + a.off_gcount);
+    c.bodies = (BodyDev *)(A + a.off_bodies);
+    c.ctrl = d_ctrl_ + s.id;
+
+    // metric vectors
+    std::vector<double> VX, VY;
+    std::vector<float> FX, FY;
+    axis_vectors(s.dx, d.dt, d.re, d.conv_factor, true, VX, FX);
+    axis_vectors(s.dy, d.dt, d.re, d.conv_factor, false, VY, FY);
+    double *md = (double *)(A + a.off_metd);
+    float *mf = (float *)(A + a.off_metf);
+    IFF_HIP(hipMemcpyAsync(md, VX.data(), VX.size() * 8, hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipMemcpyAsync(md + VX.size(), VY.data(), VY.size() * 8, hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipMemcpyAsync(mf, FX.data(), FX.size() * 4, hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipMemcpyAsync(mf + FX.size(), FY.data(), FY.size() * 4, hipMemcpyHostToDevice, st_));
+    auto axd = [&](AxisD &x, double *base, int n) {
+        const double **p[kAxisDVecs] = {&x.d, &x.ip, &x.im, &x.cs, &x.rd, &x.g,
+                                        &x.cp, &x.cm, &x.ap, &x.am, &x.ad};
+        for (int v = 0; v < kAxisDVecs; v++) *p[v] = base + (size_t)v * n;
+    };
+    auto axf = [&](AxisF &x, float *base, int n) {
+        const float **p[kAxisFVecs] = {&x.d, &x.ip, &x.im, &x.cs, &x.ap, &x.am, &x.ad, &x.cp, &x.cm};
+        for (int v = 0; v < kAxisFVecs; v++) *p[v] = base + (size_t)v * n;
+    };
+    axd(c.x, md, s.nx);
+    axd(c.y, md + VX.size(), s.ny);
+    axf(c.xf, mf, s.nx);
+    axf(c.yf, mf + FX.size(), s.ny);
+    double *dxc = (double *)(A + a.off_xc), *dyc = (double *)(A + a.off_yc);
+    IFF_HIP(hipMemcpyAsync(dxc, s.xc.data(), s.nx * 8, hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipMemcpyAsync(dyc, s.yc.data(), s.ny * 8, hipMemcpyHostToDevice, st_));
+    c.xc = dxc;
+    c.yc = dyc;
+
+    // bodies
+    double *poly = (double *)(A + a.off_poly);
+    s.bodies_dev.resize(d.nbody);
+    size_t po = 0;
+    for (int b = 0; b < d.nbody; b++) {
+        const double *dvx = nullptr, *dvy = nullptr;
+        if (s.bodies[b].kind == IF_BODY_POLYGON) {
+            const int n = s.bodies[b].nvert;
+            IFF_HIP(hipMemcpyAsync(poly + po, s.polyx[b].data(), n * 8, hipMemcpyHostToDevice, st_));
+            IFF_HIP(hipMemcpyAsync(poly + po + n, s.polyy[b].data(), n * 8, hipMemcpyHostToDevice, st_));
+            dvx = poly + po;
+            dvy = poly + po + n;
+            po += 2 * (size_t)n;
+        }
+        s.bodies_dev[b] = body_at(s.bodies[b], dvx, dvy, s.t0, d.dt);
+    }
+    if (d.nbody > 0)
+        IFF_HIP(hipMemcpyAsync(c.bodies, s.bodies_dev.data(), sizeof(BodyDev) * d.nbody,
+                               hipMemcpyHostToDevice, st_));
+
+    // control block
+    Ctrl ct{};
+    ct.t = s.t0;
+    IFF_HIP(hipMemcpyAsync(c.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+
+    // classification and ghost list: count first, then size the records
+    CaseDev *tmp = nullptr;
+    IFF_HIP(hipMalloc(&tmp, sizeof(CaseDev)));
+    auto done_tmp = [&]() { hipFree(tmp); };
+    IFF_HIP(hipMemcpyAsync(tmp, &c, sizeof(CaseDev), hipMemcpyHostToDevice, st_));
+    launch_classify(st_, tmp, 1, s.nx * s.ny, 0);
+    launch_ghost_build(st_, tmp, 1, s.nx * s.ny, 0);
+    Ctrl back{};
+    if (hipMemcpyAsync(&back, c.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st_) != hipSuccess ||
+        hipStreamSynchronize(st_) != hipSuccess) {
+        done_tmp();
+        return fail(IF_ERR_HIP, std::string("case setup failed: ") + hipGetErrorString(hipGetLastError()));
+    }
+    const int total = back.ng_total;
+    s.gcap = s.moving ? std::max(256, 3 * total + 256) : total;
+    s.ng = total;
+    c.gcap = s.gcap;
+    if (s.gcap > 0) {
+        if (hipMalloc(&c.gh, sizeof(GhostDev) * (size_t)s.gcap) != hipSuccess) {
+            c.gh = nullptr;
+            done_tmp();
+            return fail(IF_ERR_NOMEM, "ghost record allocation failed");
+        }
+        s.bytes += sizeof(GhostDev) * (size_t)s.gcap;
+        IFF_HIP(hipMemsetAsync(c.gh, 0, sizeof(GhostDev) * (size_t)s.gcap, st_));
+    }
+    ct = Ctrl{};
+    ct.t = s.t0;
+    IFF_HIP(hipMemcpyAsync(c.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipMemcpyAsync(tmp, &c, sizeof(CaseDev), hipMemcpyHostToDevice, st_));
+    launch_ghost_build(st_, tmp, 1, s.nx * s.ny, 0);
+    launch_ghost_weights(st_, tmp, 1, s.gcap, 0);
+
+    // initial state (u = ubc, v = vbc in fluid; ghost values; faces; p = 0)
+    launch_init_fields(st_, prec_, tmp, 1, s.nx * s.ny);
+    {
+        // ghost kernels need an active case
+        Ctrl on = ct;
+        on.active = 1;
+        on.ng = s.ng;
+        on.ng_total = s.ng;
+        IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+        launch_ghost_vel(st_, prec_, tmp, 1, s.gcap, 1);
+        std::vector<int2> tl(s.ntx * s.nty);
+        for (int t = 0; t < (int)tl.size(); t++) tl[t] = make_int2(0, t);
+        int2 *dtl = nullptr;
+        IFF_HIP(hipMalloc(&dtl, sizeof(int2) * tl.size()));
+        IFF_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_));
+        launch_faces(st_, prec_, tmp, dtl, (int)tl.size(), 0, 0);
+        on.active = 0;
+        IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+        const hipError_t e = hipStreamSynchronize(st_);
+        hipFree(dtl);
+        if (e != hipSuccess) {
+            done_tmp();
+            return fail(IF_ERR_HIP, std::string("case init failed: ") + hipGetErrorString(e));
+        }
+    }
+    done_tmp();
+    return IF_OK;
+}
+
+if_status Engine::add(const if_case_desc &d, int nsteps, int *slot)
+{
+    if (!st_) return fail(IF_ERR_STATE, "engine not initialised");
+    if (d.mode != IF_MODE_PHYSICS) return fail(IF_ERR_ARG, "batch cases must be physics mode");
+    int id = -1;
+    for (int k = 0; k < (int)slots_.size(); k++)
+        if (!slots_[k] || slots_[k]->state == -2) { id = k; break; }
+    if (id < 0) {
+        if ((int)slots_.size() >= cap_ctrl_) return fail(IF_ERR_BUDGET, "too many slots");
+        id = (int)slots_.size();
+        slots_.emplace_back();
+    }
+    auto s = std::make_unique<Slot>();
+    s->id = id;
+    const if_status rc = build_slot(*s, d);
+    if (rc != IF_OK) {
+        if (s->arena) hipFree(s->arena);
+        if (s->dev.gh) hipFree(s->dev.gh);
+        return rc;
+    }
+    used_ += s->bytes;
+    s->state = 0;
+    s->steps_target = std::max(0, nsteps);
+    // arm the case on the device
+    Ctrl ct{};
+    ct.active = nsteps > 0 ? 1 : 0;
+    ct.steps_target = s->steps_target;
+    ct.ng = s->ng;
+    ct.ng_total = s->ng;
+    ct.t = s->t0;
+    IFF_HIP(hipMemcpyAsync(s->dev.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipStreamSynchronize(st_));
+    if (nsteps <= 0) s->state = 1;
+    slots_[id] = std::move(s);
+    if (slot) *slot = id;
+    dirty_ = true;
+    return IF_OK;
+}
+
+if_status Engine::remove(int k)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "remove: empty slot");
+    IFF_HIP(hipStreamSynchronize(st_));
+    if (s->arena) hipFree(s->arena);
+    if (s->dev.gh) hipFree(s->dev.gh);
+    used_ -= s->bytes;
+    Ctrl z{};
+    IFF_HIP(hipMemcpyAsync(d_ctrl_ + k, &z, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+    slots_[k].reset();
+    dirty_ = true;
+    return IF_OK;
+}
+
+if_status Engine::set_target(int k, int nsteps)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "set_target: empty slot");
+    if (s->state == -1) return fail(IF_ERR_DIVERGED, "case has failed");
+    std::vector<Ctrl> all;
+    if_status rc = read_ctrl_sync(all);
+    if (rc != IF_OK) return rc;
+    Ctrl ct = all[k];
+    ct.steps_target = ct.steps_done + nsteps;
+    ct.active = nsteps > 0 ? 1 : 0;
+    s->steps_target = ct.steps_target;
+    s->steps_host = ct.steps_done;
+    IFF_HIP(hipMemcpyAsync(d_ctrl_ + k, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipStreamSynchronize(st_));
+    s->state = nsteps > 0 ? 0 : 1;
+    dirty_ = true;
+    return IF_OK;
+}
+
+if_status Engine::read_ctrl_sync(std::vector<Ctrl> &out)
+{
+    const int n = std::max(1, (int)slots_.size());
+    out.resize(n);
+    IFF_HIP(hipMemcpyAsync(out.data(), d_ctrl_, sizeof(Ctrl) * n, hipMemcpyDeviceToHost, st_));
+    IFF_HIP(hipStreamSynchronize(st_));
+    return IF_OK;
+}
+
+if_status Engine::sync()
+{
+    IFF_HIP(hipStreamSynchronize(st_));
+    return IF_OK;
+}
+
+// Re-upload the list of running cases and the tile tables.
+if_status Engine::refresh()
+{
+    order_.clear();
+    for (auto &s : slots_)
+        if (s && s->state == 0) order_.push_back(s->id);
+    const int n = (int)order_.size();
+    if (n > cap_cases_) {
+        if (d_cases_) hipFree(d_cases_);
+        cap_cases_ = std::max(n, 2 * cap_cases_);
+        IFF_HIP(hipMalloc(&d_cases_, sizeof(CaseDev) * cap_cases_));
+    }
+    std::vector<CaseDev> cs(n);
+    std::vector<int2> tb, tsw;
+    max_ng_ = max_ring_ = max_cells_ = max_ad_ = max_ppe_ = 0;
+    any_moving_ = false;
+    const int sty = sweep_ty;
+    for (int k = 0; k < n; k++) {
+        Slot &s = *slots_[order_[k]];
+        cs[k] = s.dev;
+        for (int t = 0; t < s.ntx * s.nty; t++) tb.push_back(make_int2(k, t));
+        const int ntys = (s.NY + sty - 1) / sty;
+        for (int t = 0; t < s.ntx * ntys; t++) tsw.push_back(make_int2(k, t));
+        max_ng_ = std::max(max_ng_, s.gcap);
+        max_ring_ = std::max(max_ring_, std::max(s.nx, s.ny));
+        max_cells_ = std::max(max_cells_, s.nx * s.ny);
+        max_ad_ = std::max(max_ad_, s.dev.ad_itermax);
+        max_ppe_ = std::max(max_ppe_, s.dev.ppe_itermax);
+        any_moving_ = any_moving_ || s.moving;
+    }
+    if (n > 0) IFF_HIP(hipMemcpyAsync(d_cases_, cs.data(), sizeof(CaseDev) * n, hipMemcpyHostToDevice, st_));
+    auto up = [&](int2 *&d, int &cap, int &cnt, const std::vector<int2> &v) -> if_status {
+        if ((int)v.size() > cap) {
+            if (d) hipFree(d);
+            cap = std::max((int)v.size(), 2 * cap);
+            IFF_HIP(hipMalloc(&d, sizeof(int2) * cap));
+        }
+        cnt = (int)v.size();
+        if (cnt) IFF_HIP(hipMemcpyAsync(d, v.data(), sizeof(int2) * cnt, hipMemcpyHostToDevice, st_));
+        return IF_OK;
+    };
+    if_status rc = up(d_tiles_base_, cap_tiles_base_, nt_base_, tb);
+    if (rc != IF_OK) return rc;
+    rc = up(d_tiles_sweep_, cap_tiles_sweep_, nt_sweep_, tsw);
+    if (rc != IF_OK) return rc;
+    tiles_sweep_ty_ = sty;
+    IFF_HIP(hipStreamSynchronize(st_));  // host vectors go out of scope
+    dirty_ = false;
+    return IF_OK;
+}
+
+if_status Engine::snapshot(int r)
+{
+    const int n = std::max(1, (int)slots_.size());
+    IFF_HIP(hipMemcpyAsync(h_snap_ + (size_t)r * cap_ctrl_, d_ctrl_, sizeof(Ctrl) * n,
+                           hipMemcpyDeviceToHost, st_));
+    IFF_HIP(hipEventRecord(ev_[r], st_));
+    return IF_OK;
+}
+
+bool Engine::all_done(const Ctrl *snap, int which) const
+{
+    for (int id : order_) {
+        const Ctrl &c = snap[id];
+        if (!c.active) continue;
+        if (which == 0 && !c.ad_done) return false;
+        if (which == 1 && !c.ppe_done) return false;
+    }
+    return true;
+}
+
+// Moving bodies: place every moving body at t_{n+1}, reclassify, rebuild
+// ghosts and weights on the device.
+if_status Engine::update_moving()
+{
+    for (int id : order_) {
+        Slot &s = *slots_[id];
+        if (!s.moving) continue;
+        const double t = s.t0 + (s.steps_host + 1) * s.desc.dt;
+        for (int b = 0; b < (int)s.bodies.size(); b++)
+            s.bodies_dev[b] = body_at(s.bodies[b], s.bodies_dev[b].vx, s.bodies_dev[b].vy, t, s.desc.dt);
+        IFF_HIP(hipMemcpyAsync(s.dev.bodies, s.bodies_dev.data(), sizeof(BodyDev) * s.bodies.size(),
+                               hipMemcpyHostToDevice, st_));
+    }
+    const int n = (int)order_.size();
+    launch_classify(st_, d_cases_, n, max_cells_, 1);
+    launch_ghost_build(st_, d_cases_, n, max_cells_, 1);
+    launch_ghost_weights(st_, d_cases_, n, max_ng_, 1);
+    return IF_OK;
+}
+
+if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
+{
+    if (finished) *finished = 0;
+    if (agg) std::memset(agg, 0, sizeof(*agg));
+    const int M = std::max(1, sweep_m);
+    for (int it = 0; it < nsteps; it++) {
+        if (dirty_) {
+            if_status rc = refresh();
+            if (rc != IF_OK) return rc;
+        }
+        if (order_.empty()) break;
+        const int n = (int)order_.size();
+        const int K = std::max(1, check_k);
+        bool jac = false;
+        for (int id : order_) jac = jac || slots_[id]->dev.ppe_solver == IF_PPE_JACOBI;
+        const int m_eff = jac ? 1 : M;
+        if (any_moving_) {
+            if_status rc = update_moving();
+            if (rc != IF_OK) return rc;
+        }
+        launch_step_begin(st_, d_cases_, n);
+        launch_source(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
+
+        // ---- P2: viscous Jacobi sweeps, ended per case on the device
+        auto spec = [&](int maxn, int which, auto &&unit) -> if_status {
+            int enq = 0, obs = 0;
+            bool done = false;
+            while (enq < maxn && !done) {
+                unit(enq);
+                if_status rc = snapshot(enq % kRing);
+                if (rc != IF_OK) return rc;
+                enq++;
+                while (obs < enq) {
+                    const int r = obs % kRing;
+                    if (enq - obs < spec_depth && enq < maxn) {
+                        const hipError_t q = hipEventQuery(ev_[r]);
+                        if (q == hipErrorNotReady) break;
+                        if (q != hipSuccess) return fail(IF_ERR_HIP, hipGetErrorString(q));
+                    } else {
+                        IFF_HIP(hipEventSynchronize(ev_[r]));
+                    }
+                    if (all_done(h_snap_ + (size_t)r * cap_ctrl_, which)) { done = true; break; }
+                    obs++;
+                }
+            }
+            return IF_OK;
+        };
+        if_status rc = spec(max_ad_, 0, [&](int) {
+            launch_ad_sweep(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
+            launch_ghost_vel(st_, prec_, d_cases_, n, max_ng_, 0);
+            launch_ad_finalize(st_, d_cases_, n);
+        });
+        if (rc != IF_OK) return rc;
+
+        // ---- P3: faces + RHS
+        launch_faces(st_, prec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
+
+        // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
+        const int nunits = 1 + (max_ppe_ + K - 1) / K;
+        int timed = 0;
+        rc = spec(nunits, 1, [&](int u) {
+            if (u == 0) {
+                launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 0, m_eff);
+                return;
+            }
+            const bool tm = timing && (u - 1) < 64;
+            if (tm) hipEventRecord(tev_[2 * (u - 1)], st_);
+            for (int s = 0; s < K; s += m_eff)
+                launch_ppe_sweep(st_, prec_, jac ? IF_PPE_JACOBI : IF_PPE_RBSOR, tiles_sweep_ty_, m_eff,
+                                 d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+            if (tm) {
+                hipEventRecord(tev_[2 * (u - 1) + 1], st_);
+                timed = u;
+            }
+            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, K, m_eff);
+        });
+        if (rc != IF_OK) return rc;
+
+        // ---- pressure BC, projection, divergence, ghost cells
+        launch_pressure_bc(st_, d_cases_, n, max_ring_, max_ng_);
+        launch_project(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
+        launch_ghost_vel(st_, prec_, d_cases_, n, max_ng_, 1);
+        launch_step_end(st_, d_cases_, n);
+        for (int id : order_) slots_[id]->steps_host++;
+
+        std::vector<Ctrl> all;
+        rc = read_ctrl_sync(all);
+        if (rc != IF_OK) return rc;
+        if (timing && timed > 0) {
+            int maxsw = 0;
+            for (int id : order_) maxsw = std::max(maxsw, all[id].ppe_sweeps);
+            const int nfull = std::min(timed, maxsw / K);
+            for (int u = 0; u < nfull; u++) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, tev_[2 * u], tev_[2 * u + 1]) == hipSuccess) {
+                    sweep_ms += ms;
+                    sweep_launches += (K + m_eff - 1) / m_eff;
+                }
+            }
+        }
+        for (int id : order_) {
+            Slot &s = *slots_[id];
+            const Ctrl &c = all[id];
+            s.last.steps = c.steps_done;
+            s.last.ad_iters = c.ad_iter;
+            s.last.ad_res = c.ad_res;
+            s.last.ppe_sweeps = c.ppe_sweeps;
+            s.last.ppe_res = c.ppe_res;
+            s.last.div_max = c.div_max;
+            s.last.total_ad_iters = c.tot_ad;
+            s.last.total_ppe_sweeps = c.tot_ppe;
+            s.last.max_div = c.max_div;
+            s.last.failed = c.failed;
+            if (agg) {
+                agg->steps = std::max(agg->steps, 1 + it);
+                agg->ad_iters = c.ad_iter;
+                agg->ad_res = c.ad_res;
+                agg->ppe_sweeps = c.ppe_sweeps;
+                agg->ppe_res = c.ppe_res;
+                agg->div_max = c.div_max;
+                agg->total_ad_iters += c.ad_iter;
+                agg->total_ppe_sweeps += c.ppe_sweeps;
+                agg->max_div = std::max(agg->max_div, c.div_max);
+                agg->failed |= c.failed;
+            }
+            if (c.failed) {
+                s.state = -1;
+                dirty_ = true;
+            } else if (!c.active && c.steps_done >= c.steps_target) {
+                s.state = 1;
+                dirty_ = true;
+                if (finished) (*finished)++;
+            }
+        }
+    }
+    return IF_OK;
+}
+
+if_status Engine::slot_state(int k, int *state, int *steps_done, if_step_stats *st)
+{
+    Slot *s = slot(k);
+    if (!s) {
+        if (state) *state = -2;
+        return IF_OK;
+    }
+    if (state) *state = s->state;
+    if (steps_done) *steps_done = s->last.steps;
+    if (st) *st = s->last;
+    return IF_OK;
+}
+
+// Device field -> reference layout (double).
+if_status Engine::get_field(int k, int which, double *out, size_t n)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "get_field: empty slot");
+    if (!out) return fail(IF_ERR_ARG, "get_field: null output");
+    std::vector<Ctrl> all;
+    if_status rc = read_ctrl_sync(all);
+    if (rc != IF_OK) return rc;
+    const Ctrl &ct = all[k];
+    const int nx = s->nx, ny = s->ny, P = s->P;
+    size_t need = (size_t)nx * ny;
+    if (which == IF_FIELD_UF) need = (size_t)(nx - 1) * (ny - 2);
+    if (which == IF_FIELD_VF) need = (size_t)(nx - 2) * (ny - 1);
+    if (n != need) return fail(IF_ERR_ARG, "get_field: size mismatch (expected " + std::to_string(need) + ")");
+    const size_t np = (size_t)P * ny;
+    std::vector<double> hd;
+    std::vector<float> hf;
+    std::vector<uint8_t> hb;
+    const CaseDev &c = s->dev;
+    auto fetch_ts = [&](void *p) -> if_status {
+        if (prec_ == IF_FP64) {
+            hd.resize(np);
+            IFF_HIP(hipMemcpy(hd.data(), p, np * 8, hipMemcpyDeviceToHost));
+        } else {
+            hf.resize(np);
+            IFF_HIP(hipMemcpy(hf.data(), p, np * 4, hipMemcpyDeviceToHost));
+            hd.assign(hf.begin(), hf.end());
+        }
+        return IF_OK;
+    };
+    auto fetch_d = [&](const double *p) -> if_status {
+        hd.resize(np);
+        IFF_HIP(hipMemcpy(hd.data(), p, np * 8, hipMemcpyDeviceToHost));
+        return IF_OK;
+    };
+    switch (which) {
+    case IF_FIELD_U: rc = fetch_ts(c.u[ct.ad_base]); break;
+    case IF_FIELD_V: rc = fetch_ts(c.v[ct.ad_base]); break;
+    case IF_FIELD_P: rc = fetch_d(c.p[ct.ppe_base]); break;
+    case IF_FIELD_UF: rc = fetch_d(c.uf[0]); break;
+    case IF_FIELD_VF: rc = fetch_d(c.vf[0]); break;
+    case IF_FIELD_IBLANK:
+    case IF_FIELD_FLAGS:
+        hb.resize(np);
+        IFF_HIP(hipMemcpy(hb.data(), c.flags, np, hipMemcpyDeviceToHost));
+        break;
+    default: return fail(IF_ERR_ARG, "get_field: unknown field");
+    }
+    if (rc != IF_OK) return rc;
+    if (which == IF_FIELD_UF) {
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 0; i < nx - 1; i++) out[i + (size_t)(j - 1) * (nx - 1)] = hd[i + (size_t)j * P];
+    } else if (which == IF_FIELD_VF) {
+        for (int j = 0; j < ny - 1; j++)
+            for (int i = 1; i < nx - 1; i++) out[(i - 1) + (size_t)j * (nx - 2)] = hd[i + (size_t)j * P];
+    } else if (which == IF_FIELD_IBLANK) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) out[i + (size_t)j * nx] = (hb[i + (size_t)j * P] & F_FLUID) ? 1.0 : 0.0;
+    } else if (which == IF_FIELD_FLAGS) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) out[i + (size_t)j * nx] = (double)hb[i + (size_t)j * P];
+    } else {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) out[i + (size_t)j * nx] = hd[i + (size_t)j * P];
+    }
+    return IF_OK;
+}
+
+if_status Engine::set_field(int k, int which, const double *in, size_t n)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "set_field: empty slot");
+    if (!in) return fail(IF_ERR_ARG, "set_field: null input");
+    std::vector<Ctrl> all;
+    if_status rc = read_ctrl_sync(all);
+    if (rc != IF_OK) return rc;
+    const Ctrl &ct = all[k];
+    const int nx = s->nx, ny = s->ny, P = s->P;
+    size_t need = (size_t)nx * ny;
+    if (which == IF_FIELD_UF) need = (size_t)(nx - 1) * (ny - 2);
+    if (which == IF_FIELD_VF) need = (size_t)(nx - 2) * (ny - 1);
+    if (which > IF_FIELD_VF) return fail(IF_ERR_ARG, "set_field: field is not settable");
+    if (n != need) return fail(IF_ERR_ARG, "set_field: size mismatch");
+    const size_t np = (size_t)P * ny;
+    const CaseDev &c = s->dev;
+    void *dst = nullptr;
+    bool ts = false;
+    switch (which) {
+    case IF_FIELD_U: dst = c.u[ct.ad_base]; ts = true; break;
+    case IF_FIELD_V: dst = c.v[ct.ad_base]; ts = true; break;
+    case IF_FIELD_P: dst = c.p[ct.ppe_base]; break;
+    case IF_FIELD_UF: dst = c.uf[0]; break;
+    case IF_FIELD_VF: dst = c.vf[0]; break;
+    }
+    std::vector<double> hd(np, 0.0);
+    if (ts && prec_ == IF_MIXED) {
+        std::vector<float> hf(np);
+        IFF_HIP(hipMemcpy(hf.data(), dst, np * 4, hipMemcpyDeviceToHost));
+        hd.assign(hf.begin(), hf.end());
+    } else {
+        IFF_HIP(hipMemcpy(hd.data(), dst, np * 8, hipMemcpyDeviceToHost));
+    }
+    if (which == IF_FIELD_UF) {
+        for (int j = 1; j < ny - 1; j++)
+            for (int i = 0; i < nx - 1; i++) hd[i + (size_t)j * P] = in[i + (size_t)(j - 1) * (nx - 1)];
+    } else if (which == IF_FIELD_VF) {
+        for (int j = 0; j < ny - 1; j++)
+            for (int i = 1; i < nx - 1; i++) hd[i + (size_t)j * P] = in[(i - 1) + (size_t)j * (nx - 2)];
+    } else {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) hd[i + (size_t)j * P] = in[i + (size_t)j * nx];
+    }
+    if (ts && prec_ == IF_MIXED) {
+        std::vector<float> hf(hd.begin(), hd.end());
+        IFF_HIP(hipMemcpy(dst, hf.data(), np * 4, hipMemcpyHostToDevice));
+        if (which == IF_FIELD_U || which == IF_FIELD_V) {
+            // keep the other ping-pong buffer's ghost ring consistent
+        }
+    } else {
+        IFF_HIP(hipMemcpy(dst, hd.data(), np * 8, hipMemcpyHostToDevice));
+    }
+    return IF_OK;
+}
+
+if_status Engine::ghosts(int k, int32_t *out, if_ghost *recs, size_t cap, size_t *n)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "ghosts: empty slot");
+    std::vector<Ctrl> all;
+    if_status rc = read_ctrl_sync(all);
+    if (rc != IF_OK) return rc;
+    const int ng = all[k].ng;
+    if (n) *n = (size_t)ng;
+    if (!out && !recs) return IF_OK;
+    if ((size_t)ng > cap) return fail(IF_ERR_ARG, "ghosts: capacity too small");
+    std::vector<GhostDev> g(ng);
+    if (ng) IFF_HIP(hipMemcpy(g.data(), s->dev.gh, sizeof(GhostDev) * ng, hipMemcpyDeviceToHost));
+    const int P = s->P, nx = s->nx;
+    auto ref = [&](int id) { return (id % P) + (id / P) * nx; };
+    for (int q = 0; q < ng; q++) {
+        if (out) out[q] = ref(g[q].gidx);
+        if (recs) {
+            if_ghost &r = recs[q];
+            r.gidx = ref(g[q].gidx);
+            r.body = g[q].body;
+            for (int m = 0; m < 4; m++) {
+                r.node[m] = ref(g[q].node[m]);
+                r.wd[m] = g[q].wd[m];
+                r.wn[m] = g[q].wn[m];
+            }
+            r.cd = g[q].cd;
+            r.lnx = g[q].lnx;
+            r.lny = g[q].lny;
+            r.bix = g[q].bix;
+            r.biy = g[q].biy;
+            r.ipx = g[q].ipx;
+            r.ipy = g[q].ipy;
+        }
+    }
+    return IF_OK;
+}
+
+}  // namespace iff

@@ -1,0 +1,33 @@
+// Launch wrappers of the fractional-step kernels (kernels.hip).
+#pragma once
+#include "common.h"
+
+namespace iff {
+
+void launch_step_begin(hipStream_t st, const CaseDev *cs, int ncase);
+void launch_source(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
+void launch_ad_sweep(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
+void launch_ad_finalize(hipStream_t st, const CaseDev *cs, int ncase);
+void launch_ghost_vel(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ng, int mode);
+void launch_pressure_bc(hipStream_t st, const CaseDev *cs, int ncase, int max_ring, int max_ng);
+bool sweep_variant_ok(int ty, int m);
+void launch_ppe_sweep(hipStream_t st, int prec, int solver, int ty, int m, const CaseDev *cs,
+                      const int2 *tiles, int nt, int s, int K);
+void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
+                      int nt, int K, int M);
+void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
+void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);
+
+// immersed-boundary setup (geometry.hip); fresh != 0: only moving cases,
+// with fresh-cell treatment
+void launch_classify(hipStream_t st, const CaseDev *cs, int ncase, int max_cells, int fresh);
+void launch_ghost_build(hipStream_t st, const CaseDev *cs, int ncase, int max_cells, int fresh);
+void launch_ghost_weights(hipStream_t st, const CaseDev *cs, int ncase, int max_ng, int fresh);
+int ghost_span();
+
+// initial state (kernels.hip)
+void launch_init_fields(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_cells);
+void launch_faces(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt,
+                  int out_buf, int with_rhs);
+
+}  // namespace iff

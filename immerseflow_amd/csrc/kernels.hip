@@ -1,0 +1,928 @@
+// gfx950 kernels of the fractional step.
+//
+// Every kernel is batched: a launch covers the interior tiles of all
+// resident cases, listed in a tile table (case id, tile id).  Block b is
+// remapped so that consecutive tiles land on the same XCD (the dispatcher
+// deals blocks round-robin over the 8 XCDs), which keeps halo re-reads of
+// neighbouring tiles in one L2.
+//
+// Each case carries a device control block (Ctrl).  Loop kernels read the
+// case's "done" flags and exit at once when the case has converged, so the
+// host can enqueue sweeps speculatively and a case's sweep count is decided
+// on the device exactly as the reference's host loop decides it
+// (ADSolver.cu:315, PPESolver.cu:172) -- without a round trip per sweep.
+//
+// Expressions are evaluated in the operation order of the reference kernels
+// (and of the FP64 oracle), compiled without FP contraction.
+#include "common.h"
+#include "kernels.h"
+
+namespace iff {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tile_of_block(int ntiles_pad)
+{
+    const int b = blockIdx.x;
+    const int per = ntiles_pad >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block (256 threads) reductions in a fixed order.
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double s[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = (s[0] + s[1]) + (s[2] + s[3]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v)
+{
+    __shared__ double s[4];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = fmax(fmax(s[0], s[1]), fmax(s[2], s[3]));
+    __syncthreads();
+    return r;
+}
+
+// |r| for a max-reduction that lets a non-finite value surface as +inf.
+__device__ __forceinline__ double absinf(double r)
+{
+    const double a = fabs(r);
+    return (a <= 1.0e300) ? a : HUGE_VAL;
+}
+
+template <typename T> __device__ __forceinline__ T *fld(void *p) { return (T *)p; }
+
+struct TileCtx {
+    int cid, t, tx, ty;
+};
+
+__device__ __forceinline__ bool tile_setup(const int2 *tiles, int ntiles,
+                                           int ntiles_pad, const CaseDev *cs,
+                                           TileCtx &tc)
+{
+    const int t = tile_of_block(ntiles_pad);
+    if (t >= ntiles) return false;
+    const int2 tl = tiles[t];
+    tc.cid = tl.x;
+    tc.t = tl.y;
+    const int ntx = cs[tl.x].ntx;
+    tc.tx = tl.y % ntx;
+    tc.ty = tl.y / ntx;
+    return true;
+}
+
+// Velocity of the body owning the first solid side of a blanked face.
+__device__ __forceinline__ double blank_vel(const CaseDev &c, int ida, int idb,
+                                            int comp)
+{
+    int b = c.bodyid[ida];
+    if (b < 0) b = c.bodyid[idb];
+    if (b < 0) return 0.0;
+    return comp == 0 ? c.bodies[b].ub : c.bodies[b].vb;
+}
+
+// ---------------------------------------------------------------------------
+// step bracketing
+// ---------------------------------------------------------------------------
+__global__ void k_step_begin(const CaseDev *cs, int ncase)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncase) return;
+    Ctrl *c = cs[k].ctrl;
+    if (c->active && c->steps_done >= c->steps_target) c->active = 0;
+    c->ad_done = (cs[k].ad_itermax <= 0) ? 1 : 0;
+    c->ad_iter = 0;
+    c->ad_res = 0.0;
+    c->ppe_done = 0;
+    c->ppe_sweeps = 0;
+    c->ppe_res = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// P1: explicit convective source (ADSource, ADSolver.cu:46-79)
+// ---------------------------------------------------------------------------
+template <typename TS, typename AX>
+__device__ __forceinline__ TS conv_source(const TS *q, int id, int P, const AX &ax,
+                                          const AX &ay, int i, int j, TS ufe,
+                                          TS ufw, TS vfn, TS vfs)
+{
+    const TS dx_i = ax.d[i], dx_ip1 = ax.d[i + 1], dx_im1 = ax.d[i - 1];
+    const TS dy_j = ay.d[j], dy_jp1 = ay.d[j + 1], dy_jm1 = ay.d[j - 1];
+    const TS qc = q[id];
+    return (qc - ax.cs[i] * ((ax.ip[i] * (q[id + 1] * dx_i + qc * dx_ip1) * ufe)
+                             - (ax.im[i] * (qc * dx_im1 + q[id - 1] * dx_i) * ufw)))
+           - ay.cs[j] * ((ay.ip[j] * (q[id + P] * dy_j + qc * dy_jp1) * vfn)
+                         - (ay.im[j] * (qc * dy_jm1 + q[id - P] * dy_j) * vfs));
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_source(const CaseDev *cs, const int2 *tiles,
+                                                int ntiles, int ntiles_pad)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    if (!c.ctrl->active) return;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    if (i > c.nx - 2) return;
+    const int base = c.ctrl->ad_base;
+    const TS *u = fld<TS>(c.u[base]);
+    const TS *v = fld<TS>(c.v[base]);
+    TS *sx = fld<TS>(c.sx), *sy = fld<TS>(c.sy);
+    const double *uf = c.uf[0], *vf = c.vf[0];
+    const int P = c.P;
+    for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+        const int j = 1 + tc.ty * c.tile_y + r;
+        if (j > c.ny - 2) break;
+        const int id = i + j * P;
+        const TS ufe = (TS)uf[id], ufw = (TS)uf[id - 1];
+        const TS vfn = (TS)vf[id], vfs = (TS)vf[id - P];
+        if constexpr (sizeof(TS) == 8) {
+            sx[id] = conv_source<TS>(u, id, P, c.x, c.y, i, j, ufe, ufw, vfn, vfs);
+            sy[id] = conv_source<TS>(v, id, P, c.x, c.y, i, j, ufe, ufw, vfn, vfs);
+        } else {
+            sx[id] = conv_source<TS>(u, id, P, c.xf, c.yf, i, j, ufe, ufw, vfn, vfs);
+            sy[id] = conv_source<TS>(v, id, P, c.xf, c.yf, i, j, ufe, ufw, vfn, vfs);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2: fused u+v backward-Euler viscous Jacobi sweep (ADusolver_kernel /
+// ADvsolver_kernel, ADSolver.cu:81-120) with the L1 change of
+// Compute_{u,v}Residual_AD (:122-160) reduced per tile, and the domain ghost
+// rule of set_velocity_BC (:199-217) applied to the new iterate.
+// ---------------------------------------------------------------------------
+template <typename TS, typename AX>
+__device__ __forceinline__ TS jacobi_cell(const TS *q, const TS *s, int id, int P,
+                                          const AX &ax, const AX &ay, int i, int j,
+                                          TS blank)
+{
+    const TS t = s[id] + ax.ap[i] * q[id + 1] + ax.am[i] * q[id - 1]
+                 + ay.ap[j] * q[id + P] + ay.am[j] * q[id - P];
+    return blank * t / (ax.ad[i] + ay.ad[j]);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_ad_sweep(const CaseDev *cs, const int2 *tiles,
+                                                  int ntiles, int ntiles_pad)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ad_done) return;
+    const int base = ct->ad_base;
+    const TS *u = fld<TS>(c.u[base]), *v = fld<TS>(c.v[base]);
+    TS *un = fld<TS>(c.u[1 - base]), *vn = fld<TS>(c.v[1 - base]);
+    const TS *sx = fld<TS>(c.sx), *sy = fld<TS>(c.sy);
+    const int P = c.P, nx = c.nx, ny = c.ny;
+    const TS ub2 = (TS)c.ubc * (TS)2.0, vb2 = (TS)c.vbc * (TS)2.0;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    double acc = 0.0;
+    if (i <= nx - 2) {
+        for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+            const int j = 1 + tc.ty * c.tile_y + r;
+            if (j > ny - 2) break;
+            const int id = i + j * P;
+            const bool fl = c.flags[id] & F_FLUID;
+            const TS blank = fl ? (TS)1.0 : (TS)0.0;
+            TS qu, qv;
+            if constexpr (sizeof(TS) == 8) {
+                qu = jacobi_cell<TS>(u, sx, id, P, c.x, c.y, i, j, blank);
+                qv = jacobi_cell<TS>(v, sy, id, P, c.x, c.y, i, j, blank);
+            } else {
+                qu = jacobi_cell<TS>(u, sx, id, P, c.xf, c.yf, i, j, blank);
+                qv = jacobi_cell<TS>(v, sy, id, P, c.xf, c.yf, i, j, blank);
+            }
+            un[id] = qu;
+            vn[id] = qv;
+            if (fl) acc += (double)fabs(qu - u[id]) + (double)fabs(qv - v[id]);
+            if (i == 1) { un[id - 1] = -qu + ub2; vn[id - 1] = -qv + vb2; }
+            if (i == nx - 2) { un[id + 1] = -qu + ub2; vn[id + 1] = -qv + vb2; }
+            if (j == 1) { un[id - P] = -qu + ub2; vn[id - P] = -qv + vb2; }
+            if (j == ny - 2) { un[id + P] = -qu + ub2; vn[id + P] = -qv + vb2; }
+        }
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) c.part[tc.t] = acc;
+}
+
+// One block per case: the stopping rule of ADSolver.cu:315 on the device.
+__global__ __launch_bounds__(256) void k_ad_finalize(const CaseDev *cs, int ncase)
+{
+    const CaseDev &c = cs[blockIdx.x];
+    Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ad_done) return;
+    const int nt = c.ntx * c.nty;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nt; k += 256) s += c.part[k];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        ct->ad_res = s;
+        ct->ad_iter += 1;
+        ct->ad_base ^= 1;
+        ct->tot_ad += 1;
+        if (!(s <= 1.0e300)) ct->failed = 1;
+        if (!(s > c.ad_tol) || ct->ad_iter >= c.ad_itermax) ct->ad_done = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ghost-cell (image point) updates.  mode 0: on the new AD iterate (skipped
+// once the case's AD loop is done); mode 1: on the current velocity.
+// Stencil nodes are fluid cells (or the ghost itself with weight 0), which
+// these kernels never write: no ordering between ghost cells is needed.
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ __launch_bounds__(256) void k_ghost_vel(const CaseDev *cs, int mode)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    const Ctrl *ct = c.ctrl;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ct->ng || !ct->active) return;
+    int buf = ct->ad_base;
+    if (mode == 0) {
+        if (ct->ad_done) return;
+        buf = 1 - buf;
+    }
+    TS *u = fld<TS>(c.u[buf]), *v = fld<TS>(c.v[buf]);
+    const GhostDev &o = c.gh[g];
+    const BodyDev &b = c.bodies[o.body];
+    double su = o.cd * b.ub, sv = o.cd * b.vb;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        su = su + o.wd[k] * (double)u[o.node[k]];
+        sv = sv + o.wd[k] * (double)v[o.node[k]];
+    }
+    u[o.gidx] = (TS)su;
+    v[o.gidx] = (TS)sv;
+}
+
+// Pressure: homogeneous Neumann on the domain ring (edges), then ghost
+// cells p_GC = p_IP (+ moving-body term).
+__global__ __launch_bounds__(256) void k_p_ring(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    if (!c.ctrl->active) return;
+    double *p = c.p[c.ctrl->ppe_base];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int P = c.P, nx = c.nx, ny = c.ny;
+    if (k < ny - 2) {
+        const int j = 1 + k;
+        p[0 + j * P] = p[1 + j * P];
+        p[nx - 1 + j * P] = p[nx - 2 + j * P];
+    }
+    if (k < nx - 2) {
+        const int i = 1 + k;
+        p[i] = p[i + P];
+        p[i + (ny - 1) * P] = p[i + (ny - 2) * P];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ghost_p(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.ctrl->ng || !c.ctrl->active) return;
+    double *p = c.p[c.ctrl->ppe_base];
+    const GhostDev &o = c.gh[g];
+    const BodyDev &b = c.bodies[o.body];
+    double sp = o.wn[0] * p[o.node[0]];
+#pragma unroll
+    for (int k = 1; k < 4; k++) sp = sp + o.wn[k] * p[o.node[k]];
+    p[o.gidx] = sp + (o.lnx * b.axb + o.lny * b.ayb);
+}
+
+// ---------------------------------------------------------------------------
+// P3: intermediate face velocities (Compute_velf interpolation,
+// ADSolver.cu:173-174,184-185, on open faces; body velocity on blanked
+// faces; the BC on domain faces) and the divergence RHS, in one pass.
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ __launch_bounds__(256) void k_faces_rhs(const CaseDev *cs, const int2 *tiles,
+                                                   int ntiles, int ntiles_pad, int out_buf,
+                                                   int with_rhs)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    if (!c.ctrl->active) return;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    if (i > nx - 2) return;
+    const int base = c.ctrl->ad_base;
+    const TS *u = fld<TS>(c.u[base]), *v = fld<TS>(c.v[base]);
+    double *uf = c.uf[out_buf], *vf = c.vf[out_buf];
+    const AxisD &ax = c.x, &ay = c.y;
+    const double dxi = ax.d[i], dxp = ax.d[i + 1], dxm = ax.d[i - 1];
+    for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+        const int j = 1 + tc.ty * c.tile_y + r;
+        if (j > ny - 2) break;
+        const int id = i + j * P;
+        const uint8_t f = c.flags[id];
+        const double uc = (double)u[id], vc = (double)v[id];
+        double ufe, ufw, vfn, vfs;
+        if (i == nx - 2) ufe = c.ubc;
+        else if (f & F_OPEN_E) ufe = ax.ip[i] * ((double)u[id + 1] * dxi + uc * dxp);
+        else ufe = blank_vel(c, id, id + 1, 0);
+        if (i == 1) ufw = c.ubc;
+        else if (f & F_OPEN_W) ufw = ax.ip[i - 1] * (uc * dxm + (double)u[id - 1] * dxi);
+        else ufw = blank_vel(c, id - 1, id, 0);
+        const double dyj = ay.d[j], dyp = ay.d[j + 1], dym = ay.d[j - 1];
+        if (j == ny - 2) vfn = c.vbc;
+        else if (f & F_OPEN_N) vfn = ay.ip[j] * ((double)v[id + P] * dyj + vc * dyp);
+        else vfn = blank_vel(c, id, id + P, 1);
+        if (j == 1) vfs = c.vbc;
+        else if (f & F_OPEN_S) vfs = ay.ip[j - 1] * (vc * dym + (double)v[id - P] * dyj);
+        else vfs = blank_vel(c, id - P, id, 1);
+        uf[id] = ufe;
+        vf[id] = vfn;
+        if (i == 1) uf[id - 1] = ufw;
+        if (j == 1) vf[id - P] = vfs;
+        if (with_rhs)
+            c.rhs[id] = (f & F_FLUID) ? ((ufe - ufw) * ax.rd[i] + (vfn - vfs) * ay.rd[j]) * c.rdt
+                                      : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P4: pressure Poisson.  Coefficients of calculatePPECoefficients
+// (PPESolver.cu:93-99) on open faces, zero across blanked and domain faces.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PpeAxis {
+    const T *cp, *cm;
+};
+
+// Red-black SOR, M full sweeps per launch on an LDS tile with a 2M halo
+// (temporal blocking).  Reads the unknown from buffer src, writes dst: the
+// ping-pong keeps tiles from reading halos that other tiles are writing,
+// and the result equals M global red-black sweeps bit for bit.
+// T = double: unknown p, right side rhs.  T = float (mixed precision):
+// unknown e (error), right side r32; the first launch of a chunk starts
+// from e = 0.
+template <typename T, int TY, int M>
+__global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *tiles,
+                                               int ntiles, int ntiles_pad, int s, int K)
+{
+    constexpr int H = 2 * M;
+    constexpr int W = kTileX + 2 * H, HT = TY + 2 * H;
+    constexpr int W1 = kTileX + 2 * (H - 1), HT1 = TY + 2 * (H - 1);
+    __shared__ T sp[HT][W];
+    __shared__ T sr[HT1][W1];
+    __shared__ uint8_t sf[HT1][W1];
+
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    if (s >= kexec) return;
+    int meff = kexec - s;
+    if (meff > M) meff = M;
+    const int L = s / M;  // launch index inside the chunk
+
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const T *src;
+    T *dst;
+    const T *rhs;
+    bool zero_src = false;
+    PpeAxis<T> ax, ay;
+    T om;
+    if constexpr (sizeof(T) == 8) {
+        src = c.p[(ct->ppe_base + L) & 1];
+        dst = c.p[(ct->ppe_base + L + 1) & 1];
+        rhs = c.rhs;
+        ax = {c.x.cp, c.x.cm};
+        ay = {c.y.cp, c.y.cm};
+        om = c.omega;
+    } else {
+        src = c.e[L & 1];
+        dst = c.e[(L + 1) & 1];
+        zero_src = (L == 0);
+        rhs = c.r32;
+        ax = {c.xf.cp, c.xf.cm};
+        ay = {c.yf.cp, c.yf.cm};
+        om = c.omega_f;
+    }
+
+    const int i0 = 1 + tc.tx * kTileX, j0 = 1 + tc.ty * TY;
+    // load phase: p with halo H, rhs/flags with halo H-1
+    for (int k = threadIdx.x; k < W * HT; k += 256) {
+        const int lx = k % W, ly = k / W;
+        const int gi = i0 - H + lx, gj = j0 - H + ly;
+        T val = (T)0;
+        if (!zero_src && gi >= 0 && gi < nx && gj >= 0 && gj < ny) val = src[gi + gj * P];
+        sp[ly][lx] = val;
+    }
+    for (int k = threadIdx.x; k < W1 * HT1; k += 256) {
+        const int lx = k % W1, ly = k / W1;
+        const int gi = i0 - (H - 1) + lx, gj = j0 - (H - 1) + ly;
+        T rv = (T)0;
+        uint8_t fv = 0;
+        if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= ny - 2) {
+            rv = rhs[gi + gj * P];
+            fv = c.flags[gi + gj * P];
+        }
+        sr[ly][lx] = rv;
+        sf[ly][lx] = fv;
+    }
+    __syncthreads();
+
+    // 2*meff half-sweeps on a shrinking region (ring H-1-h around the tile)
+    for (int h = 0; h < 2 * meff; h++) {
+        const int colour = h & 1;
+        const int R = H - 1 - h;
+        const int rows = TY + 2 * R;
+        const int wid = kTileX + 2 * R;
+        const int pairs = (wid + 1) >> 1;
+        for (int k = threadIdx.x; k < rows * pairs; k += 256) {
+            const int row = k / pairs, pr = k - row * pairs;
+            const int ly = H - R + row;
+            const int gj = j0 - H + ly;
+            int lx = H - R + 2 * pr;
+            const int gi0 = i0 - H + lx;
+            lx += ((gi0 + gj) ^ colour) & 1;
+            if (lx >= H + kTileX + R) continue;
+            const int gi = i0 - H + lx;
+            const uint8_t f = sf[ly - 1][lx - 1];
+            if (!(f & F_FLUID)) continue;
+            const T aE = (f & F_OPEN_E) ? ax.cp[gi] : (T)0;
+            const T aW = (f & F_OPEN_W) ? ax.cm[gi] : (T)0;
+            const T aN = (f & F_OPEN_N) ? ay.cp[gj] : (T)0;
+            const T aS = (f & F_OPEN_S) ? ay.cm[gj] : (T)0;
+            const T ap = -((aE + aW) + (aN + aS));
+            if (ap == (T)0) continue;
+            const T sum = (aE * sp[ly][lx + 1] + aW * sp[ly][lx - 1])
+                          + (aN * sp[ly + 1][lx] + aS * sp[ly - 1][lx]);
+            const T pc = sp[ly][lx];
+            const T gs = (sr[ly - 1][lx - 1] - sum) / ap;
+            sp[ly][lx] = pc + om * (gs - pc);
+        }
+        __syncthreads();
+    }
+
+    // write the tile (all interior cells: non-fluid values pass through)
+    for (int k = threadIdx.x; k < kTileX * TY; k += 256) {
+        const int lx = k & (kTileX - 1), ly = k / kTileX;
+        const int gi = i0 + lx, gj = j0 + ly;
+        if (gi <= nx - 2 && gj <= ny - 2) dst[gi + gj * P] = sp[H + ly][H + lx];
+    }
+}
+
+// Weighted point Jacobi (jacobiIteration, PPESolver.cu:13-31, with the RHS
+// term and the relaxation weight; weight 1 is the reference's iteration).
+template <typename T>
+__global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *tiles,
+                                                int ntiles, int ntiles_pad, int s, int K)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    if (s >= kexec) return;
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    if (i > nx - 2) return;
+    const T *src;
+    T *dst;
+    const T *rhs;
+    PpeAxis<T> ax, ay;
+    if constexpr (sizeof(T) == 8) {
+        src = c.p[(ct->ppe_base + s) & 1];
+        dst = c.p[(ct->ppe_base + s + 1) & 1];
+        rhs = c.rhs;
+        ax = {c.x.cp, c.x.cm};
+        ay = {c.y.cp, c.y.cm};
+    } else {
+        src = c.e[s & 1];
+        dst = c.e[(s + 1) & 1];
+        rhs = c.r32;
+        ax = {c.xf.cp, c.xf.cm};
+        ay = {c.yf.cp, c.yf.cm};
+    }
+    const bool zero_src = (sizeof(T) == 4) && s == 0;
+    const T om = (sizeof(T) == 8) ? (T)c.omega : (T)c.omega_f;
+    for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+        const int j = 1 + tc.ty * c.tile_y + r;
+        if (j > ny - 2) break;
+        const int id = i + j * P;
+        const uint8_t f = c.flags[id];
+        T val = zero_src ? (T)0 : src[id];
+        if (f & F_FLUID) {
+            const T aE = (f & F_OPEN_E) ? ax.cp[i] : (T)0;
+            const T aW = (f & F_OPEN_W) ? ax.cm[i] : (T)0;
+            const T aN = (f & F_OPEN_N) ? ay.cp[j] : (T)0;
+            const T aS = (f & F_OPEN_S) ? ay.cm[j] : (T)0;
+            const T ap = -((aE + aW) + (aN + aS));
+            if (ap != (T)0) {
+                T sum = (T)0;
+                if (!zero_src)
+                    sum = (aE * src[id + 1] + aW * src[id - 1]) + (aN * src[id + P] + aS * src[id - P]);
+                const T gs = (rhs[id] - sum) / ap;
+                val = val + om * (gs - val);
+            }
+        }
+        dst[id] = val;
+    }
+}
+
+// FP64 residual r = rhs - A p (max norm per tile) after a chunk of K sweeps
+// done in launches of M sweeps (K = 0: the check before any sweep).
+__global__ __launch_bounds__(256) void k_ppe_resid(const CaseDev *cs, const int2 *tiles,
+                                                   int ntiles, int ntiles_pad, int K, int M)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    const int nl = (kexec + M - 1) / M;
+    const double *p = c.p[(ct->ppe_base + nl) & 1];
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    double m = 0.0;
+    if (i <= nx - 2) {
+        const double cpx = c.x.cp[i], cmx = c.x.cm[i];
+        for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+            const int j = 1 + tc.ty * c.tile_y + r;
+            if (j > ny - 2) break;
+            const int id = i + j * P;
+            const uint8_t f = c.flags[id];
+            if (!(f & F_FLUID)) continue;
+            const double aE = (f & F_OPEN_E) ? cpx : 0.0;
+            const double aW = (f & F_OPEN_W) ? cmx : 0.0;
+            const double aN = (f & F_OPEN_N) ? c.y.cp[j] : 0.0;
+            const double aS = (f & F_OPEN_S) ? c.y.cm[j] : 0.0;
+            const double ap = -((aE + aW) + (aN + aS));
+            if (ap == 0.0) continue;
+            const double sum = (aE * p[id + 1] + aW * p[id - 1]) + (aN * p[id + P] + aS * p[id - P]);
+            const double rr = c.rhs[id] - (ap * p[id] + sum);
+            m = fmax(m, absinf(rr));
+        }
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) c.part[tc.t] = m;
+}
+
+// Mixed precision: p_new = p + e (FP64) after a chunk of FP32 error sweeps,
+// then the FP64 residual of p_new, stored as FP32 for the next chunk
+// (defect correction / iterative refinement).  K = 0: residual only.
+__global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, const int2 *tiles,
+                                                          int ntiles, int ntiles_pad, int K,
+                                                          int M)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    const int nl = (kexec + M - 1) / M;
+    const bool upd = kexec > 0;
+    const double *p = c.p[ct->ppe_base];
+    double *pn = c.p[1 - ct->ppe_base];
+    const float *e = c.e[nl & 1];
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    double m = 0.0;
+    if (i <= nx - 2) {
+        const double cpx = c.x.cp[i], cmx = c.x.cm[i];
+        for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+            const int j = 1 + tc.ty * c.tile_y + r;
+            if (j > ny - 2) break;
+            const int id = i + j * P;
+            const uint8_t f = c.flags[id];
+            double pc = p[id], pE = p[id + 1], pW = p[id - 1], pN = p[id + P], pS = p[id - P];
+            if (upd) {
+                pc += (double)e[id];
+                pE += (double)e[id + 1];
+                pW += (double)e[id - 1];
+                pN += (double)e[id + P];
+                pS += (double)e[id - P];
+                pn[id] = pc;
+            }
+            float r32 = 0.0f;
+            if (f & F_FLUID) {
+                const double aE = (f & F_OPEN_E) ? cpx : 0.0;
+                const double aW = (f & F_OPEN_W) ? cmx : 0.0;
+                const double aN = (f & F_OPEN_N) ? c.y.cp[j] : 0.0;
+                const double aS = (f & F_OPEN_S) ? c.y.cm[j] : 0.0;
+                const double ap = -((aE + aW) + (aN + aS));
+                if (ap != 0.0) {
+                    const double sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
+                    const double rr = c.rhs[id] - (ap * pc + sum);
+                    r32 = (float)rr;
+                    m = fmax(m, absinf(rr));
+                }
+            }
+            c.r32[id] = r32;
+        }
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) c.part[tc.t] = m;
+}
+
+// One block per case: the Poisson stopping rule on the device.
+// nl_rule: 0 = FP64 ping-pong (base advances by the launch count),
+//          1 = mixed (base flips once when an update happened).
+__global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int ncase, int K,
+                                                      int M, int mixed)
+{
+    const CaseDev &c = cs[blockIdx.x];
+    Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    const int nt = c.ntx * c.nty;
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nt; k += 256) m = fmax(m, c.part[k]);
+    m = block_max(m);
+    if (threadIdx.x == 0) {
+        int kexec = c.ppe_itermax - ct->ppe_sweeps;
+        if (kexec > K) kexec = K;
+        const int nl = (kexec + M - 1) / M;
+        ct->ppe_sweeps += kexec;
+        ct->tot_ppe += kexec;
+        if (mixed) {
+            if (kexec > 0) ct->ppe_base ^= 1;
+        } else {
+            ct->ppe_base = (ct->ppe_base + nl) & 1;
+        }
+        const double res = c.dt * m;
+        ct->ppe_res = res;
+        if (!(res <= 1.0e300)) ct->failed = 1;
+        if (!(res > c.ppe_tol) || ct->ppe_sweeps >= c.ppe_itermax || ct->failed) ct->ppe_done = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P7/P8: projection of faces and cells, domain ghosts, divergence max.
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ __launch_bounds__(256) void k_project(const CaseDev *cs, const int2 *tiles,
+                                                 int ntiles, int ntiles_pad)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active) return;
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    const double *p = c.p[ct->ppe_base];
+    const double *ufs = c.uf[1], *vfs_ = c.vf[1];
+    double *ufn = c.uf[0], *vfn_ = c.vf[0];
+    TS *u = fld<TS>(c.u[ct->ad_base]), *v = fld<TS>(c.v[ct->ad_base]);
+    const double dt = c.dt;
+    const TS ub2 = (TS)c.ubc * (TS)2.0, vb2 = (TS)c.vbc * (TS)2.0;
+    const AxisD &ax = c.x, &ay = c.y;
+    double m = 0.0;
+    if (i <= nx - 2) {
+        const double dxi = ax.d[i], dxp = ax.d[i + 1], dxm = ax.d[i - 1];
+        for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+            const int j = 1 + tc.ty * c.tile_y + r;
+            if (j > ny - 2) break;
+            const int id = i + j * P;
+            const uint8_t f = c.flags[id];
+            const double pc = p[id], pE = p[id + 1], pW = p[id - 1], pN = p[id + P], pS = p[id - P];
+            // faces owned by this cell (east, north) and its west/south
+            // neighbours' (recomputed for the divergence, same expression)
+            double ue = ufs[id], uw = ufs[id - 1], vn = vfs_[id], vs = vfs_[id - P];
+            if (f & F_OPEN_E) ue = ue - dt * ((pE - pc) * ax.g[i]);
+            if (f & F_OPEN_W) uw = uw - dt * ((pc - pW) * ax.g[i - 1]);
+            if (f & F_OPEN_N) vn = vn - dt * ((pN - pc) * ay.g[j]);
+            if (f & F_OPEN_S) vs = vs - dt * ((pc - pS) * ay.g[j - 1]);
+            ufn[id] = ue;
+            vfn_[id] = vn;
+            if (i == 1) ufn[id - 1] = uw;
+            if (j == 1) vfn_[id - P] = vs;
+            // cell velocities
+            const double dyj = ay.d[j], dyp = ay.d[j + 1], dym = ay.d[j - 1];
+            const double pe = ax.ip[i] * (pE * dxi + pc * dxp);
+            const double pw = ax.im[i] * (pc * dxm + pW * dxi);
+            const double pn = ay.ip[j] * (pN * dyj + pc * dyp);
+            const double ps = ay.im[j] * (pc * dym + pS * dyj);
+            const double blank = (f & F_FLUID) ? 1.0 : 0.0;
+            const TS qu = (TS)(blank * ((double)u[id] - dt * ((pe - pw) * ax.rd[i])));
+            const TS qv = (TS)(blank * ((double)v[id] - dt * ((pn - ps) * ay.rd[j])));
+            u[id] = qu;
+            v[id] = qv;
+            if (i == 1) { u[id - 1] = -qu + ub2; v[id - 1] = -qv + vb2; }
+            if (i == nx - 2) { u[id + 1] = -qu + ub2; v[id + 1] = -qv + vb2; }
+            if (j == 1) { u[id - P] = -qu + ub2; v[id - P] = -qv + vb2; }
+            if (j == ny - 2) { u[id + P] = -qu + ub2; v[id + P] = -qv + vb2; }
+            if (f & F_FLUID) m = fmax(m, absinf((ue - uw) * ax.rd[i] + (vn - vs) * ay.rd[j]));
+        }
+    }
+    m = block_max(m);
+    if (threadIdx.x == 0) c.part[tc.t] = m;
+}
+
+__global__ __launch_bounds__(256) void k_step_end(const CaseDev *cs, int ncase)
+{
+    const CaseDev &c = cs[blockIdx.x];
+    Ctrl *ct = c.ctrl;
+    if (!ct->active) return;
+    const int nt = c.ntx * c.nty;
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nt; k += 256) m = fmax(m, c.part[k]);
+    m = block_max(m);
+    if (threadIdx.x == 0) {
+        ct->div_max = m;
+        if (m > ct->max_div || !(m <= 1.0e300)) ct->max_div = m;
+        if (!(m <= 1.0e300)) ct->failed = 1;
+        ct->steps_done += 1;
+        ct->t += c.dt;
+        if (ct->failed || ct->steps_done >= ct->steps_target) ct->active = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch wrappers
+// ---------------------------------------------------------------------------
+static inline dim3 tile_grid(int ntiles) { return dim3((unsigned)(((ntiles + 7) / 8) * 8)); }
+static inline int tile_pad(int ntiles) { return ((ntiles + 7) / 8) * 8; }
+
+void launch_step_begin(hipStream_t st, const CaseDev *cs, int ncase)
+{
+    hipLaunchKernelGGL(k_step_begin, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase);
+}
+
+void launch_source(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt)
+{
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_source<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+    else
+        hipLaunchKernelGGL(k_source<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+}
+
+void launch_ad_sweep(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt)
+{
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_ad_sweep<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+    else
+        hipLaunchKernelGGL(k_ad_sweep<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+}
+
+void launch_ad_finalize(hipStream_t st, const CaseDev *cs, int ncase)
+{
+    hipLaunchKernelGGL(k_ad_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase);
+}
+
+void launch_ghost_vel(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ng, int mode)
+{
+    if (max_ng <= 0) return;
+    dim3 g((max_ng + 255) / 256, ncase);
+    if (prec == IF_FP64) hipLaunchKernelGGL(k_ghost_vel<double>, g, dim3(256), 0, st, cs, mode);
+    else hipLaunchKernelGGL(k_ghost_vel<float>, g, dim3(256), 0, st, cs, mode);
+}
+
+void launch_pressure_bc(hipStream_t st, const CaseDev *cs, int ncase, int max_ring, int max_ng)
+{
+    hipLaunchKernelGGL(k_p_ring, dim3((max_ring + 255) / 256, ncase), dim3(256), 0, st, cs);
+    if (max_ng > 0)
+        hipLaunchKernelGGL(k_ghost_p, dim3((max_ng + 255) / 256, ncase), dim3(256), 0, st, cs);
+}
+
+void launch_faces(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt,
+                  int out_buf, int with_rhs)
+{
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_faces_rhs<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                           tile_pad(nt), out_buf, with_rhs);
+    else
+        hipLaunchKernelGGL(k_faces_rhs<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                           tile_pad(nt), out_buf, with_rhs);
+}
+
+// Initial state of a physics case: u = ubc, v = vbc on fluid cells (the
+// ghost ring included), zero on solid cells, in both ping-pong buffers
+// (their corners are never rewritten); the domain-ghost rule on the edges.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_init_fields(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    const int L = blockIdx.x * 256 + threadIdx.x;
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    if (L >= nx * ny) return;
+    const int i = L % nx, j = L / nx;
+    const int id = i + j * P;
+    const bool fl = c.flags[id] & F_FLUID;
+    TS uu = fl ? (TS)c.ubc : (TS)0, vv = fl ? (TS)c.vbc : (TS)0;
+    const bool ei = (i == 0 || i == nx - 1), ej = (j == 0 || j == ny - 1);
+    if (ei != ej) {  // edge ghost (not a corner): -q_in + 2 q_bc
+        const int ii = (i == 0) ? 1 : (i == nx - 1 ? nx - 2 : i);
+        const int jj = (j == 0) ? 1 : (j == ny - 1 ? ny - 2 : j);
+        const bool fin = c.flags[ii + jj * P] & F_FLUID;
+        const TS uin = fin ? (TS)c.ubc : (TS)0, vin = fin ? (TS)c.vbc : (TS)0;
+        uu = -uin + (TS)c.ubc * (TS)2.0;
+        vv = -vin + (TS)c.vbc * (TS)2.0;
+    }
+    for (int b = 0; b < 2; b++) {
+        fld<TS>(c.u[b])[id] = uu;
+        fld<TS>(c.v[b])[id] = vv;
+    }
+}
+
+void launch_init_fields(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_cells)
+{
+    dim3 g((max_cells + 255) / 256, ncase);
+    if (prec == IF_FP64) hipLaunchKernelGGL(k_init_fields<double>, g, dim3(256), 0, st, cs);
+    else hipLaunchKernelGGL(k_init_fields<float>, g, dim3(256), 0, st, cs);
+}
+
+template <typename T, int TY, int M>
+static void rbsor_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K)
+{
+    hipLaunchKernelGGL((k_rbsor<T, TY, M>), tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                       tile_pad(nt), s, K);
+}
+
+bool sweep_variant_ok(int ty, int m)
+{
+    return (ty == 8 || ty == 16 || ty == 32) && (m == 1 || m == 2 || m == 4);
+}
+
+void launch_ppe_sweep(hipStream_t st, int prec, int solver, int ty, int m, const CaseDev *cs,
+                      const int2 *tiles, int nt, int s, int K)
+{
+    if (solver == IF_PPE_JACOBI) {
+        if (prec == IF_FP64)
+            hipLaunchKernelGGL(k_jacobi<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                               tile_pad(nt), s, K);
+        else
+            hipLaunchKernelGGL(k_jacobi<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                               tile_pad(nt), s, K);
+        return;
+    }
+#define IFF_RB(T, TY_, M_) \
+    if (ty == TY_ && m == M_) { rbsor_one<T, TY_, M_>(st, cs, tiles, nt, s, K); return; }
+    if (prec == IF_FP64) {
+        IFF_RB(double, 8, 1) IFF_RB(double, 16, 1) IFF_RB(double, 32, 1)
+        IFF_RB(double, 8, 2) IFF_RB(double, 16, 2) IFF_RB(double, 32, 2)
+        IFF_RB(double, 8, 4) IFF_RB(double, 16, 4) IFF_RB(double, 32, 4)
+    } else {
+        IFF_RB(float, 8, 1) IFF_RB(float, 16, 1) IFF_RB(float, 32, 1)
+        IFF_RB(float, 8, 2) IFF_RB(float, 16, 2) IFF_RB(float, 32, 2)
+        IFF_RB(float, 8, 4) IFF_RB(float, 16, 4) IFF_RB(float, 32, 4)
+    }
+#undef IFF_RB
+}
+
+void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
+                      int nt, int K, int M)
+{
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_ppe_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt), K, M);
+    else
+        hipLaunchKernelGGL(k_ppe_update_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
+                           tile_pad(nt), K, M);
+    hipLaunchKernelGGL(k_ppe_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase, K, M,
+                       prec == IF_FP64 ? 0 : 1);
+}
+
+void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt)
+{
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_project<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+    else
+        hipLaunchKernelGGL(k_project<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+}
+
+void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase)
+{
+    hipLaunchKernelGGL(k_step_end, dim3(ncase), dim3(256), 0, st, cs, ncase);
+}
+
+}  // namespace iff

@@ -1,0 +1,237 @@
+/*
+ * ImmerseFlow-AMD C ABI -- the drop-in boundary for ImmerseFlow++'s
+ * fractional-step time step on MI355X (gfx950).
+ *
+ * The reference (sushrutkr/ImmerseFlow) exposes its solver as the member
+ * functions of `struct ImmerseFlow` (src/header/globalVariables.cuh:70-88)
+ * called in a fixed order by main() (src/main.cu:75-102), plus its files:
+ * inputs.txt, xgrid/ygrid face files and Tecplot .dat outputs.  Each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - plain C types, caller-owned host buffers, library-owned device memory;
+ *  - every call returns an if_status (0 = success); the library never calls
+ *    exit() (the reference does, globalVariables.cuh:91-107); the message of
+ *    the last failure on the calling thread is if_last_error();
+ *  - a handle is not re-entrant: calls on one handle must not overlap; other
+ *    handles may be driven from other threads;
+ *  - cell arrays use the reference layout: nx*ny values with nx, ny counting
+ *    the ghost ring and idx = i + j*nx (src/include/preSim.cu:10); face
+ *    arrays: uf (nx-1)*(ny-2), vf (nx-2)*(ny-1) (ADSolver.cu:66-74).
+ */
+#ifndef IMMERSEFLOW_IF_ABI_H
+#define IMMERSEFLOW_IF_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int if_status;
+
+enum {
+    IF_OK = 0,
+    IF_ERR_ARG = -1,      /* bad argument or descriptor */
+    IF_ERR_IO = -2,       /* file missing / unreadable / unwritable */
+    IF_ERR_PARSE = -3,    /* malformed deck or grid file */
+    IF_ERR_HIP = -4,      /* HIP runtime error */
+    IF_ERR_NOMEM = -5,    /* host or device allocation failed */
+    IF_ERR_STATE = -6,    /* call not valid for this handle's mode/state */
+    IF_ERR_NODEV = -7,    /* no usable gfx950 device */
+    IF_ERR_BUDGET = -8,   /* batch admission refused: HBM budget */
+    IF_ERR_DIVERGED = -9  /* non-finite residual: the case is marked failed */
+};
+
+enum { IF_MODE_PHYSICS = 0, IF_MODE_COMPAT = 1 };
+enum { IF_FP64 = 0, IF_MIXED = 1 };
+enum { IF_PPE_JACOBI = 0, IF_PPE_RBSOR = 1 };
+enum { IF_BODY_CIRCLE = 0, IF_BODY_ELLIPSE = 1, IF_BODY_POLYGON = 2 };
+enum { IF_MOTION_NONE = 0, IF_MOTION_HEAVE = 1 };
+
+enum {
+    IF_FIELD_U = 0, IF_FIELD_V = 1, IF_FIELD_P = 2, IF_FIELD_UF = 3,
+    IF_FIELD_VF = 4, IF_FIELD_IBLANK = 5, IF_FIELD_FLAGS = 6
+};
+
+#define IF_MAX_BODIES 16
+
+/* One immersed body.  Geometry is plain data: orientation enters as
+ * cos/sin computed by the caller, polygons as vertex arrays. */
+typedef struct {
+    int kind;               /* IF_BODY_* */
+    double xc, yc;          /* centre / polygon reference point */
+    double r;               /* circle radius; polygon bounding radius */
+    double a, b;            /* ellipse semi-axes */
+    double cth, sth;        /* ellipse rotation cos/sin */
+    int nvert;              /* polygon vertices */
+    const double *vx, *vy;  /* polygon vertices relative to (xc, yc) */
+    int motion;             /* IF_MOTION_* */
+    double amp, freq;       /* heave y = yc + amp*sin(2 pi freq t) */
+} if_body;
+
+/* The deck of src/main.cu:10-59 (CFDInput, globalVariables.cuh:15-33), nx,
+ * ny including the ghost ring after main.cu:55-58, plus the physics-mode
+ * extension read from keyword lines the reference parser ignores. */
+typedef struct {
+    int restart, restart_time;
+    int nx, ny, nxf, nyf;
+    double lx, ly;
+    int w_ad, w_ppe, ad_itermax, ppe_itermax, ad_solver, ppe_solver;
+    double error_max, tmax, dt, re, mu;
+    int write_interval;
+    /* extension ("Fractional step" block) */
+    int mode;               /* IF_MODE_*; default compat */
+    int precision;          /* IF_FP64 / IF_MIXED */
+    double conv_factor;     /* default 0.5 (ADSolver.cu:66-74) */
+    double ppe_omega;       /* SOR relaxation; <= 0: from the grid */
+    double ppe_tol;         /* dt*||b - A p||_inf stop level */
+    int check_every;        /* sweeps per Poisson check */
+    int nbody;
+    if_body bodies[IF_MAX_BODIES];
+    double poly_store[IF_MAX_BODIES][2][512]; /* storage for parsed polygons */
+} if_inputs;
+
+/* A case: grid + parameters + bodies (ImmerseFlow's allocation +
+ * readGridData + initializeData, preSim.cu:147-240,242-372). */
+typedef struct {
+    int mode;               /* IF_MODE_* */
+    int precision;          /* IF_FP64 / IF_MIXED (physics mode) */
+    int nxf, nyf;           /* face counts (NX+1, NY+1) */
+    const double *xf, *yf;  /* face coordinates (grid-file contents) */
+    double dt, re;
+    double conv_factor;     /* ADSource factor, 0.5 in the reference */
+    double ubc, vbc;        /* Dirichlet velocity on the four sides */
+    int ad_itermax;
+    double ad_tol;          /* L1 sum of |dq| (ADSolver.cu:315), 1e-6 */
+    int ppe_solver;         /* IF_PPE_JACOBI / IF_PPE_RBSOR */
+    double ppe_omega;
+    int ppe_itermax;
+    double ppe_tol;
+    int check_every;
+    double t0;              /* start time, for moving bodies */
+    int nbody;
+    const if_body *bodies;
+} if_case_desc;
+
+typedef struct {
+    int steps;              /* steps taken by this call */
+    int ad_iters;           /* last step: Jacobi sweeps executed */
+    double ad_res;          /* last step: final uRes+vRes */
+    int ppe_sweeps;         /* last step: Poisson sweeps executed */
+    double ppe_res;         /* last step: dt*||b - A p||_inf */
+    double div_max;         /* last step: max |D uf| over fluid cells */
+    long long total_ad_iters, total_ppe_sweeps; /* over the call */
+    double max_div;         /* max over the call's steps */
+    int failed;             /* non-finite state detected */
+} if_step_stats;
+
+typedef struct {
+    int32_t gidx;           /* ghost cell, reference linear index */
+    int32_t body;
+    int32_t node[4];        /* bilinear stencil cells, reference indices */
+    double wd[4], cd;       /* Dirichlet weights, body-velocity coefficient */
+    double wn[4];           /* Neumann weights */
+    double lnx, lny;        /* IP - GC (moving-body Neumann term) */
+    double bix, biy, ipx, ipy;
+} if_ghost;
+
+typedef struct if_case if_case;
+typedef struct if_batch if_batch;
+
+/* ---- device -------------------------------------------------------------
+ * Replaces ImmerseFlow::CUDAQuery (preSim.cu:187-199). */
+if_status if_device_query(int *ndev, char *name, size_t name_len,
+                          int *cus, size_t *hbm_bytes);
+
+/* ---- files ---------------------------------------------------------------
+ * if_parse_inputs replaces readInputFile (main.cu:10-59), same grammar. */
+if_status if_parse_inputs(const char *path, if_inputs *out);
+/* if_read_grid replaces the file part of readGridData (preSim.cu:267-291):
+ * nf "<id> <x>" lines; xf must hold nf values. */
+if_status if_read_grid(const char *path, int nf, double *xf);
+/* if_grid_metric replaces the metric part of readGridData
+ * (preSim.cu:293-355): n = nf+1 centres (ghost mirrors) and widths. */
+if_status if_grid_metric(int nf, const double *xf, double *xc, double *dx);
+/* if_write_tecplot replaces write_results_to_file (postSim.cu:41-66). */
+if_status if_write_tecplot(const char *path, int nx, int ny, const double *x,
+                           const double *y, const double *f);
+
+/* ---- single case ---------------------------------------------------------
+ * if_case_create replaces allocation + readGridData + initializeData. */
+if_status if_case_create(const if_case_desc *desc, if_case **out);
+/* if_case_destroy replaces freeAllocation (preSim.cu:164-179). */
+void if_case_destroy(if_case *c);
+/* Compat mode: one ImmerseFlow::ADsolver call (ADSolver.cu:268-395). */
+if_status if_case_ad_step(if_case *c, if_step_stats *st);
+/* Compat mode: ImmerseFlow::PPESolver (PPESolver.cu:137-205). */
+if_status if_case_ppe_solve(if_case *c, if_step_stats *st);
+/* Physics mode: nsteps full fractional steps (predictor, divergence RHS,
+ * Poisson, projection, divergence check). */
+if_status if_case_step(if_case *c, int nsteps, if_step_stats *st);
+/* Copy a field to the host (replaces saveDataToFile's D2H copy,
+ * postSim.cu:10-39); n must equal the field's reference-layout size. */
+if_status if_case_get_field(if_case *c, int which, double *out, size_t n);
+/* Load a field (restart); same layouts as if_case_get_field. */
+if_status if_case_set_field(if_case *c, int which, const double *in, size_t n);
+/* Grid vectors of the case (xc, yc, dx, dy: nx and ny values). */
+if_status if_case_grid(if_case *c, double *xc, double *yc, double *dx,
+                       double *dy);
+if_status if_case_dims(if_case *c, int *nx, int *ny);
+/* Ghost-cell list (sorted reference linear indices) and image-point
+ * records.  *n gets the count; out/recs may be NULL to query it. */
+if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
+                         size_t cap, size_t *n);
+/* Poisson sweep launch shape: tile height (8, 16 or 32 cells), red-black
+ * sweeps per launch (1, 2 or 4: temporal blocking in LDS) and sweeps per
+ * convergence check.  Results do not depend on the first two. */
+if_status if_case_tune(if_case *c, int tile_y, int sweeps_per_launch,
+                       int check_every);
+/* Accumulated event timing of the Poisson sweep kernel (milliseconds,
+ * launches) over the steps since the last reset. */
+if_status if_case_sweep_timing(if_case *c, int reset, double *ms,
+                               long long *launches);
+/* Algorithmic HBM bytes per interior cell of one Poisson sweep launch and
+ * of one step at the given sweep counts (the roofline byte model). */
+if_status if_case_byte_model(if_case *c, double *sweep_bytes_per_cell,
+                             double *step_bytes_per_cell);
+
+/* ---- batches and serving -------------------------------------------------
+ * Many independent cases advanced together: every kernel launch covers the
+ * tiles of all resident cases; per-case device flags end each case's
+ * iterations independently; per-case reductions run in a fixed order, so a
+ * case's result does not depend on what else is resident. */
+if_status if_batch_create(int precision, size_t hbm_budget, if_batch **out);
+void if_batch_destroy(if_batch *b);
+/* Admit a case that should run nsteps steps; IF_ERR_BUDGET when its
+ * footprint does not fit the remaining budget. *slot gets its id. */
+if_status if_batch_add(if_batch *b, const if_case_desc *desc, int nsteps,
+                       int *slot);
+/* Advance every resident unfinished case by up to nsteps steps (one batched
+ * fractional step per iteration).  *finished gets how many cases completed
+ * their step count during the call. */
+if_status if_batch_step(if_batch *b, int nsteps, int *finished);
+/* State of a slot: 0 running, 1 finished, -1 failed, -2 empty. */
+if_status if_batch_slot_state(if_batch *b, int slot, int *state,
+                              int *steps_done, if_step_stats *st);
+if_status if_batch_get_field(if_batch *b, int slot, int which, double *out,
+                             size_t n);
+/* Release a slot (frees its device memory back to the budget). */
+if_status if_batch_remove(if_batch *b, int slot);
+if_status if_batch_info(if_batch *b, int *resident, size_t *bytes_in_use,
+                        double *sweep_ms, long long *sweep_launches);
+/* As if_case_tune, for every case of the batch. */
+if_status if_batch_tune(if_batch *b, int tile_y, int sweeps_per_launch,
+                        int check_every);
+/* Footprint in device bytes that if_batch_add would charge for desc. */
+if_status if_case_footprint(const if_case_desc *desc, int precision,
+                            size_t *bytes);
+
+/* ---- errors ---------------------------------------------------------------*/
+const char *if_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -635,8 +635,8 @@ static void domain_bc_p(int nx, int ny, double *p)
     }
 }
 
-/* Velocity on a blanked (fluid/solid) interior face: the body velocity of
- * the solid side. */
+/* Velocity on a blanked interior face (at least one side solid): the
+ * velocity of the body owning the first solid side (classification ids). */
 static double blank_vel(const int8_t *body_id, int ida, int idb,
                         const of_body *bodies, int comp)
 {
@@ -716,7 +716,8 @@ double of_phys_divergence(int nx, int ny, const double *dx, const double *dy,
             double d = 0.0;
             if (flags[id] & OF_FLUID) d = cell_div(nx, i, j, dx, dy, uf, vf);
             if (div) div[id] = d;
-            if (fabs(d) > m) m = fabs(d);
+            if (!(fabs(d) <= 1.0e300)) m = INFINITY;
+            else if (fabs(d) > m) m = fabs(d);
         }
         row[j] = m;
     }
@@ -774,10 +775,11 @@ static double residual_inf(const ppe_coef *c, int nx, int ny,
             if (f & OF_FLUID) {
                 double ap, sum;
                 ppe_cell(c, nx, i, j, f, p, &ap, &sum);
-                rr = rhs[id] - (ap * p[id] + sum);
+                if (ap != 0.0) rr = rhs[id] - (ap * p[id] + sum);
             }
             if (r) r[id] = rr;
-            if (fabs(rr) > m) m = fabs(rr);
+            if (!(fabs(rr) <= 1.0e300)) m = INFINITY;
+            else if (fabs(rr) > m) m = fabs(rr);
         }
         row[j] = m;
     }
@@ -800,8 +802,10 @@ double of_phys_residual(int nx, int ny, const double *dx, const double *dy,
 
 /* Red-black SOR sweep (colour (i+j)&1 == 0 first), the parallel form of the
  * deck's PPE_Solver 1 "Point GS" with relaxation w-PPE (inputs.txt:14-15);
- * Jacobi (jacobiIteration, PPESolver.cu:13-31, with the RHS) when
- * ppe_solver == 0. */
+ * weighted Jacobi (jacobiIteration, PPESolver.cu:13-31, with the RHS and
+ * the relaxation weight) when ppe_solver == 0.  On the all-Neumann problem
+ * plain Jacobi (weight 1) does not converge: the checkerboard mode has
+ * iteration eigenvalue -1. */
 static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
                       const double *rhs, const of_params *prm, double *p,
                       double *tmp)
@@ -817,7 +821,10 @@ static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
                 if (f & OF_FLUID) {
                     double ap, sum;
                     ppe_cell(c, nx, i, j, f, p, &ap, &sum);
-                    if (ap != 0.0) val = (rhs[id] - sum) / ap;
+                    if (ap != 0.0) {
+                        const double gs = (rhs[id] - sum) / ap;
+                        val = p[id] + om * (gs - p[id]);
+                    }
                 }
                 tmp[id] = val;
             }
@@ -865,24 +872,15 @@ int of_phys_poisson(int nx, int ny, const double *dx, const double *dy,
     return sweeps;
 }
 
-static int8_t *no_body_ids(int nx, int ny)
-{
-    int8_t *b = malloc((size_t)nx * ny);
-    memset(b, -1, (size_t)nx * ny);
-    return b;
-}
-
 void of_phys_init(int nx, int ny, const double *dx, const double *dy,
-                  const uint8_t *flags, int nbody, const of_body *bodies,
-                  int ng, const of_ghost *gh, const of_params *prm,
-                  double *u, double *v, double *uf, double *vf, double *p)
+                  const uint8_t *flags, const int8_t *bid, int nbody,
+                  const of_body *bodies, int ng, const of_ghost *gh,
+                  const of_params *prm, double *u, double *v, double *uf,
+                  double *vf, double *p)
 {
     /* fluid cells start at the free stream, solid cells at zero; ghost
-     * cells then take their image-point values.  Body ownership of the
-     * cells next to the fluid is recovered from the ghost records. */
+     * cells then take their image-point values. */
     const size_t n = (size_t)nx * ny;
-    int8_t *bid = no_body_ids(nx, ny);
-    for (int g = 0; g < ng; g++) bid[gh[g].gidx] = (int8_t)gh[g].body;
     for (size_t id = 0; id < n; id++) {
         u[id] = (flags[id] & OF_FLUID) ? prm->ubc : 0.0;
         v[id] = (flags[id] & OF_FLUID) ? prm->vbc : 0.0;
@@ -892,18 +890,15 @@ void of_phys_init(int nx, int ny, const double *dx, const double *dy,
     domain_bc_vel(nx, ny, v, prm->vbc);
     of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, u, v);
     faces_from_cells(nx, ny, dx, dy, flags, bid, bodies, prm, u, v, uf, vf);
-    free(bid);
 }
 
 void of_phys_step(int nx, int ny, const double *dx, const double *dy,
-                  const uint8_t *flags, int nbody, const of_body *bodies,
-                  int ng, const of_ghost *gh, const of_params *prm,
-                  double *u, double *v, double *uf, double *vf, double *p,
-                  of_stats *st)
+                  const uint8_t *flags, const int8_t *bid, int nbody,
+                  const of_body *bodies, int ng, const of_ghost *gh,
+                  const of_params *prm, double *u, double *v, double *uf,
+                  double *vf, double *p, of_stats *st)
 {
     const size_t n = (size_t)nx * ny;
-    int8_t *bid = no_body_ids(nx, ny);
-    for (int g = 0; g < ng; g++) bid[gh[g].gidx] = (int8_t)gh[g].body;
     double *iblank = malloc(n * sizeof(double));
     for (size_t id = 0; id < n; id++) iblank[id] = (flags[id] & OF_FLUID) ? 1.0 : 0.0;
 
@@ -993,6 +988,6 @@ void of_phys_step(int nx, int ny, const double *dx, const double *dy,
         st->ppe_res = pres;
         st->div_max = dmax;
     }
-    free(bid); free(iblank); free(sx); free(sy); free(un); free(vn); free(rhs);
+    free(iblank); free(sx); free(sy); free(un); free(vn); free(rhs);
     ad_coef_free(&ac);
 }

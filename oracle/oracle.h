@@ -152,16 +152,17 @@ typedef struct {
  * velocities interpolated (blanked faces = body velocity, domain faces =
  * BC), p = 0. */
 void of_phys_init(int nx, int ny, const double *dx, const double *dy,
-                  const uint8_t *flags, int nbody, const of_body *bodies,
-                  int ng, const of_ghost *gh, const of_params *prm,
-                  double *u, double *v, double *uf, double *vf, double *p);
+                  const uint8_t *flags, const int8_t *body_id, int nbody,
+                  const of_body *bodies, int ng, const of_ghost *gh,
+                  const of_params *prm, double *u, double *v, double *uf,
+                  double *vf, double *p);
 
 /* One fractional step. */
 void of_phys_step(int nx, int ny, const double *dx, const double *dy,
-                  const uint8_t *flags, int nbody, const of_body *bodies,
-                  int ng, const of_ghost *gh, const of_params *prm,
-                  double *u, double *v, double *uf, double *vf, double *p,
-                  of_stats *st);
+                  const uint8_t *flags, const int8_t *body_id, int nbody,
+                  const of_body *bodies, int ng, const of_ghost *gh,
+                  const of_params *prm, double *u, double *v, double *uf,
+                  double *vf, double *p, of_stats *st);
 
 /* Pieces of the step, exposed for closed-form tests. */
 void of_phys_rhs(int nx, int ny, const double *dx, const double *dy,

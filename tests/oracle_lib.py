@@ -85,7 +85,7 @@ def lib():
                                    C.POINTER(Ghost)]
     L.of_closest_point.argtypes = [C.POINTER(Body), C.c_double, C.c_double,
                                    _dp, _dp]
-    phys_sig = [C.c_int, C.c_int, _dp, _dp, _u8p, C.c_int, C.POINTER(Body),
+    phys_sig = [C.c_int, C.c_int, _dp, _dp, _u8p, _i8p, C.c_int, C.POINTER(Body),
                 C.c_int, C.POINTER(Ghost), C.POINTER(Params), _dp, _dp, _dp,
                 _dp, _dp]
     L.of_phys_init.argtypes = phys_sig
@@ -219,8 +219,8 @@ class PhysCase:
         self.uf = np.zeros((nx - 1) * (ny - 2))
         self.vf = np.zeros((nx - 2) * (ny - 1))
         lib().of_phys_init(nx, ny, dptr(self.dx), dptr(self.dy),
-                           u8ptr(self.flags), self.bodies.n, self.bodies.arr,
-                           self.ng, self.gh, C.byref(self.prm), dptr(self.u),
+                           u8ptr(self.flags), i8ptr(self.bid), self.bodies.n,
+                           self.bodies.arr, self.ng, self.gh, C.byref(self.prm), dptr(self.u),
                            dptr(self.v), dptr(self.uf), dptr(self.vf),
                            dptr(self.p))
 
@@ -229,8 +229,8 @@ class PhysCase:
         out = []
         for _ in range(n):
             lib().of_phys_step(self.nx, self.ny, dptr(self.dx), dptr(self.dy),
-                               u8ptr(self.flags), self.bodies.n,
-                               self.bodies.arr, self.ng, self.gh,
+                               u8ptr(self.flags), i8ptr(self.bid),
+                               self.bodies.n, self.bodies.arr, self.ng, self.gh,
                                C.byref(self.prm), dptr(self.u), dptr(self.v),
                                dptr(self.uf), dptr(self.vf), dptr(self.p),
                                C.byref(st))

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden files.
+
+Tolerances: integer and index outputs (flags, ghost lists, sweep counts)
+must be identical; FP64 fields agree to 1e-10 relative (the kernels
+evaluate the oracle's expressions in the same order without contraction, so
+they are normally bit-identical; only reduction order differs); the mixed
+path (FP32 stencils and Poisson sweeps with FP64 refinement) agrees with the
+FP64 oracle to 1e-4 relative at the same Poisson stop level.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN
+
+import immerseflow_amd as iff
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def rel(a, b):
+    s = max(1.0, float(np.max(np.abs(b))))
+    return float(np.max(np.abs(a - b))) / s
+
+
+def stretched_faces(n, L, ratio):
+    w = ratio ** np.arange(n)
+    w = w / w.sum() * L
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
+# ---------------------------------------------------------------------------
+# compat mode: the reference's own outputs
+# ---------------------------------------------------------------------------
+def test_compat_adsolver_golden(require_gpu, tmp_path):
+    _say("compat ADsolver 52x52 x 20 steps")
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat2"))
+    c = iff.Case(xf, xf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ad_itermax=25)
+    iters = [c.ad_step()["ad_iters"] for _ in range(20)]
+    assert iters == [5] * 20
+    iff.write_tecplot(str(tmp_path / "uc.dat"), c.xc, c.yc, c.get("u"))
+    iff.write_tecplot(str(tmp_path / "vc.dat"), c.xc, c.yc, c.get("v"))
+    for name in ("uc.dat", "vc.dat"):
+        assert (tmp_path / name).read_bytes() == open(os.path.join(GOLDEN, name), "rb").read(), name
+    # against the oracle's fields (the IC uses the device's exp/pow)
+    xc, dx = O.metric(xf)
+    n = len(xc)
+    u, v, p = np.zeros(n * n), np.zeros(n * n), np.zeros(n * n)
+    uf, vf = np.zeros((n - 1) * (n - 2)), np.zeros((n - 2) * (n - 1))
+    L = O.lib()
+    L.of_compat_init(n, n, O.dptr(xc), O.dptr(xc), O.dptr(u), O.dptr(v), O.dptr(uf), O.dptr(vf), O.dptr(p))
+    ib = np.ones(n * n)
+    for _ in range(20):
+        r = C.c_double()
+        L.of_compat_ad_step(n, n, O.dptr(dx), O.dptr(dx), 0.001, 150.0, 25, O.dptr(ib), O.dptr(u),
+                            O.dptr(v), O.dptr(uf), O.dptr(vf), C.byref(r))
+    assert rel(c.get("u"), u) < 1e-13 and rel(c.get("v"), v) < 1e-13
+    assert np.array_equal(c.get("iblank"), np.ones(n * n))
+    c.close()
+
+
+def test_compat_ppe_golden(require_gpu, tmp_path):
+    _say("compat PPESolver 52x52")
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat2"))
+    c = iff.Case(xf, xf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ppe_itermax=100000)
+    st = c.ppe_solve()
+    assert 14000 < st["ppe_sweeps"] < 15500
+    iff.write_tecplot(str(tmp_path / "p.dat"), c.xc, c.yc, c.get("p"))
+    assert (tmp_path / "p.dat").read_bytes() == open(os.path.join(GOLDEN, "p.dat"), "rb").read()
+    c.close()
+
+
+# ---------------------------------------------------------------------------
+# physics mode: FP64 against the oracle
+# ---------------------------------------------------------------------------
+def _oracle_bodies(bodies):
+    out = []
+    for b in bodies:
+        d = dict(b)
+        d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
+        out.append(d)
+    return out
+
+
+def _pair(xf, yf, bodies, prec=iff.IF_FP64, **kw):
+    prm = dict(dt=kw.get("dt", 0.01), re=kw.get("re", 100.0), conv_factor=0.5, ubc=1.0,
+               vbc=0.0, ad_itermax=kw.get("ad_itermax", 25), ad_tol=1e-6,
+               ppe_solver=kw.get("ppe_solver", 1), ppe_omega=kw.get("ppe_omega", 1.8),
+               ppe_itermax=kw.get("ppe_itermax", 200000), ppe_tol=kw.get("ppe_tol", 1e-9),
+               check_every=kw.get("check_every", 10))
+    oc = O.PhysCase(xf, yf, _oracle_bodies(bodies), prm)
+    gc = iff.Case(xf, yf, precision=prec, bodies=bodies, **prm)
+    return oc, gc
+
+
+def _compare_geometry(oc, gc):
+    flags = gc.get("flags").astype(np.uint8)
+    assert np.array_equal(flags, oc.flags)
+    lst, recs, ng = gc.ghosts()
+    assert ng == oc.ng and np.array_equal(lst, oc.glist)
+    for g in range(ng):
+        a, b = recs[g], oc.gh[g]
+        assert list(a.node) == list(b.node) and a.body == b.body
+        for k in range(4):
+            assert abs(a.wd[k] - b.wd[k]) <= 1e-14 and abs(a.wn[k] - b.wn[k]) <= 1e-14
+        assert abs(a.cd - b.cd) <= 1e-14
+
+
+def _compare_fields(oc, gc, tol):
+    for name, ref in (("u", oc.u), ("v", oc.v), ("p", oc.p), ("uf", oc.uf), ("vf", oc.vf)):
+        e = rel(gc.get(name), ref)
+        assert e <= tol, (name, e)
+
+
+CASES = {
+    "circle-stretched": (stretched_faces(72, 4.0, 1.01), stretched_faces(50, 3.0, 0.99),
+                         [iff.circle(1.6, 1.45, 0.45)]),
+    "ellipse": (np.linspace(0, 4.0, 97), np.linspace(0, 2.0, 49),
+                [iff.ellipse(1.5, 1.0, 0.5, 0.12, np.radians(10.0))]),
+    "stars": (np.linspace(0, 4.0, 129), np.linspace(0, 4.0, 129),
+              [iff.star(1.3, 1.2, 0.45, [0.05, 0.03, 0.02, 0.06, 0.01], [0.1, 1.2, 2.3, 0.4, 5.0]),
+               iff.star(2.8, 2.7, 0.4, [0.02, 0.07, 0.04, 0.01, 0.03], [3.0, 0.5, 1.5, 2.5, 4.0])]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_physics_fp64_matches_oracle(require_gpu, name):
+    xf, yf, bodies = CASES[name]
+    _say("physics fp64", name, len(xf) - 1, "x", len(yf) - 1)
+    oc, gc = _pair(xf, yf, bodies, dt=0.01)
+    _compare_geometry(oc, gc)
+    _compare_fields(oc, gc, 1e-14)   # initial state
+    for s in range(4):
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["ad_iters"] == o[0], (s, g, o)
+        assert g["ppe_sweeps"] == o[2], (s, g, o)
+        assert abs(g["div_max"] - o[4]) <= 1e-10 * max(o[4], 1e-12) + 1e-18
+        _compare_fields(oc, gc, 1e-10)
+    gc.close()
+
+
+@pytest.mark.parametrize("ty,m", [(8, 1), (32, 1), (16, 2), (32, 4), (8, 4)])
+def test_sweep_variants_are_identical(require_gpu, ty, m):
+    """Tile height and temporal blocking change the launch shape, never the
+    result: equal to the oracle's red-black SOR sweep for sweep."""
+    xf, yf, bodies = CASES["circle-stretched"]
+    _say("sweep variant", ty, m)
+    oc, gc = _pair(xf, yf, bodies, dt=0.01, check_every=12)
+    gc.tune(ty, m, 12)
+    for _ in range(3):
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["ppe_sweeps"] == o[2]
+    _compare_fields(oc, gc, 1e-10)
+    gc.close()
+
+
+def test_weighted_jacobi_matches_oracle(require_gpu):
+    xf, yf, bodies = CASES["circle-stretched"]
+    _say("weighted jacobi")
+    oc, gc = _pair(xf, yf, bodies, dt=0.01, ppe_solver=0, ppe_omega=0.8, ppe_tol=1e-6,
+                   check_every=25)
+    for _ in range(2):
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["ppe_sweeps"] == o[2]
+    _compare_fields(oc, gc, 1e-10)
+    gc.close()
+
+
+def test_mixed_precision_against_fp64_oracle(require_gpu):
+    xf, yf, bodies = CASES["circle-stretched"]
+    _say("mixed precision")
+    oc, gc = _pair(xf, yf, bodies, prec=iff.IF_MIXED, dt=0.01, ppe_tol=1e-9, check_every=10)
+    for _ in range(4):
+        oc.step(1)
+        g = gc.step(1)
+        assert g["div_max"] <= 2e-9
+    for name, ref in (("u", oc.u), ("v", oc.v), ("uf", oc.uf), ("vf", oc.vf)):
+        assert rel(gc.get(name), ref) <= 1e-4, name
+    # pressure up to its additive constant (volume-weighted mean over fluid)
+    fl = (oc.flags & 1).astype(bool)
+    W = np.outer(oc.dy, oc.dx).ravel()
+    pg, po = gc.get("p"), oc.p
+    inner = np.zeros_like(fl)
+    F2 = fl.reshape(oc.ny, oc.nx)
+    F2[[0, -1], :] = False
+    F2[:, [0, -1]] = False
+    m = F2.ravel()
+    pg = pg - np.sum(W[m] * pg[m]) / np.sum(W[m])
+    po = po - np.sum(W[m] * po[m]) / np.sum(W[m])
+    assert rel(pg[m], po[m]) <= 1e-4
+    gc.close()
+
+
+# ---------------------------------------------------------------------------
+# batching: a case's result does not depend on what else is resident
+# ---------------------------------------------------------------------------
+def test_batch_alone_equals_in_mix(require_gpu):
+    specs = [
+        (np.linspace(0, 3, 61), np.linspace(0, 2, 41), [iff.circle(1.0, 1.0, 0.3)], 0.01, 120.0),
+        (stretched_faces(90, 4, 1.01), stretched_faces(70, 3, 1.0), [iff.ellipse(2, 1.5, 0.4, 0.1, 0.3)], 0.008, 300.0),
+        (np.linspace(0, 2, 33), np.linspace(0, 2, 33), [iff.circle(0.8, 1.0, 0.25)], 0.02, 50.0),
+    ]
+    steps = [3, 5, 4]
+    _say("batch alone vs mix")
+    kw = dict(ppe_tol=1e-8, check_every=10, ppe_omega=1.85)
+    mix = iff.Batch(iff.IF_FP64)
+    slots = [mix.add(xf, yf, n, dt=dt, re=re, bodies=b, **kw)
+             for (xf, yf, b, dt, re), n in zip(specs, steps)]
+    while any(mix.state(s)[0] == 0 for s in slots):
+        mix.step(1)
+    for k, ((xf, yf, b, dt, re), n) in enumerate(zip(specs, steps)):
+        st, done, _ = mix.state(slots[k])
+        assert st == 1 and done == n
+        alone = iff.Batch(iff.IF_FP64)
+        s = alone.add(xf, yf, n, dt=dt, re=re, bodies=b, **kw)
+        alone.step(n)
+        for f in ("u", "v", "p", "uf", "vf"):
+            assert np.array_equal(alone.get(s, f), mix.get(slots[k], f)), (k, f)
+        alone.close()
+    mix.close()
+
+
+def test_batch_budget_admission(require_gpu):
+    xf = np.linspace(0, 1, 65)
+    need = iff.Batch.footprint(xf, xf, dt=0.01, re=100.0)
+    b = iff.Batch(iff.IF_FP64, hbm_budget=int(need * 1.5))
+    assert b.add(xf, xf, 1, dt=0.01, re=100.0) is not None
+    assert b.add(xf, xf, 1, dt=0.01, re=100.0) is None   # refused: over budget
+    b.close()

@@ -1,0 +1,148 @@
+"""Host-side library checks (CPU): exported symbols, the deck grammar, grid
+files, the Tecplot writer, error paths.  No GPU compute is called."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN, ROOT, gpu_present
+
+import immerseflow_amd as iff
+from immerseflow_amd import _abi
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "immerseflow", "if_abi.h")).read()
+    decl = r"^(?:if_status|void|const char\s*\*)\s*\*?\s*(if_[a-z_]+)\s*\("
+    return sorted(set(re.findall(decl, txt, re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = _abi.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(n for n, _, _ in _abi.SIGNATURES) == declared
+
+
+def test_parse_reference_deck():
+    d = iff.read_input_file(os.path.join(GOLDEN, "inputs.txt"))
+    assert (d["restart"], d["restart_time"]) == (0, 9)
+    assert (d["nx"], d["ny"], d["nxf"], d["nyf"]) == (52, 52, 51, 51)
+    assert (d["lx"], d["ly"]) == (10.0, 5.0)
+    assert (d["w_ad"], d["w_ppe"], d["ad_itermax"], d["ppe_itermax"]) == (1, 1, 25, 100000)
+    assert (d["ad_solver"], d["ppe_solver"]) == (1, 1)
+    assert (d["error_max"], d["tmax"], d["dt"], d["re"], d["mu"]) == (1e-6, 100.0, 0.001, 150.0, 0.01)
+    assert d["write_interval"] == 1000
+    assert d["mode"] == iff.IF_MODE_COMPAT and d["bodies"] == []
+
+
+def test_parse_extension_blocks(tmp_path):
+    deck = open(os.path.join(GOLDEN, "inputs.txt")).read() + """
+___________________________| Fractional Step |_______________________________
+Mode   Precision   Conv-factor   Omega-PPE   Tol-PPE   Check-every
+physics  mixed   0.5   1.9   1e-7   20
+
+___________________________| Bodies |_______________________________
+Body (circle xc yc r)
+circle 5 8 0.5
+Body (ellipse xc yc a b angle-deg)
+ellipse 10 10 0.5 0.05 10
+Body (star xc yc R a2..a6 phi2..phi6 markers)
+star 3 3 0.4 0.01 0.02 0.03 0.04 0.05 0 1 2 3 4 400
+Body (heave xc yc r amp freq)
+heave 5 8 0.5 0.5 0.2
+"""
+    p = tmp_path / "deck.txt"
+    p.write_text(deck)
+    d = iff.read_input_file(str(p))
+    assert d["mode"] == iff.IF_MODE_PHYSICS and d["precision"] == iff.IF_MIXED
+    assert (d["conv_factor"], d["ppe_omega"], d["ppe_tol"], d["check_every"]) == (0.5, 1.9, 1e-7, 20)
+    kinds = [b["kind"] for b in d["bodies"]]
+    assert kinds == [iff.IF_BODY_CIRCLE, iff.IF_BODY_ELLIPSE, iff.IF_BODY_POLYGON, iff.IF_BODY_CIRCLE]
+    e = d["bodies"][1]
+    assert abs(e["cth"] - np.cos(np.radians(10))) < 1e-15 and abs(e["sth"] - np.sin(np.radians(10))) < 1e-15
+    s = d["bodies"][2]
+    assert len(s["vx"]) == 400 and s["r"] >= np.max(np.hypot(s["vx"] - 3, s["vy"] - 3))
+    assert d["bodies"][3]["motion"] == iff.IF_MOTION_HEAVE
+    # the reference-era keys are unchanged by the extension
+    assert (d["nx"], d["ny"], d["ad_itermax"], d["dt"]) == (52, 52, 25, 0.001)
+
+
+def test_parse_errors(tmp_path):
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_input_file(str(tmp_path / "missing.txt"))
+    assert e.value.code == _abi.IF_ERR_IO
+    p = tmp_path / "bad.txt"
+    p.write_text("nothing here\n")
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_input_file(str(p))
+    assert e.value.code == _abi.IF_ERR_PARSE
+
+
+def test_grid_reader_and_metric_match_the_oracle():
+    for name, n in (("xgrid.dat2", 51), ("xgrid.dat", 181), ("ygrid.dat", 129)):
+        xf = iff.read_grid(os.path.join(GOLDEN, name), n)
+        assert np.array_equal(xf, O.read_grid_file(os.path.join(GOLDEN, name)))
+        xc, dx = iff.grid_metric(xf)
+        oxc, odx = O.metric(xf)
+        assert np.array_equal(xc, oxc) and np.array_equal(dx, odx)
+
+
+def test_short_grid_file_is_an_error(tmp_path):
+    p = tmp_path / "short.dat"
+    p.write_text("1 0.0\n2 0.5\n")
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_grid(str(p), 51)
+    assert e.value.code == _abi.IF_ERR_PARSE
+
+
+def test_writer_reproduces_golden_bytes(tmp_path):
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat2"))
+    xc, _ = O.metric(xf)
+    out = tmp_path / "final_results.dat"
+    iff.write_tecplot(str(out), xc, xc, np.ones(52 * 52))
+    assert out.read_bytes() == open(os.path.join(GOLDEN, "final_results.dat"), "rb").read()
+
+
+def test_writer_row_order_matches_reference_kat(tmp_path):
+    """test/output.txt (the reference's 3x3 writer known answer): j outer,
+    i inner, one row per cell."""
+    kat = open(os.path.join(GOLDEN, "writer_kat_output.txt")).read().splitlines()
+    out = tmp_path / "k.dat"
+    iff.write_tecplot(str(out), np.arange(3.0), np.arange(3.0), np.arange(1.0, 10.0))
+    got = out.read_text().splitlines()
+    assert got[2] == kat[2]
+    for a, b in zip(got[3:], kat[3:]):
+        ga, gb = a.split(","), b.split(",")
+        assert float(ga[0]) == float(gb[0]) and float(ga[1]) == float(gb[1]) and ga[2] == gb[2]
+
+
+def test_writer_unwritable_path():
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.write_tecplot("/nonexistent_dir/x.dat", np.zeros(2), np.zeros(2), np.zeros(4))
+    assert e.value.code == _abi.IF_ERR_IO
+
+
+@pytest.mark.skipif(gpu_present(), reason="checks the no-device error path")
+def test_no_device_fails_loudly():
+    xf = np.linspace(0, 1, 17)
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.Case(xf, xf, dt=0.01, re=100.0)
+    assert e.value.code == _abi.IF_ERR_NODEV
+    with pytest.raises(iff.ImmerseFlowError):
+        iff.Batch()
+
+
+def test_case_descriptor_validation_without_device():
+    # bad descriptors are refused before any device work
+    lib = _abi.load()
+    d, keep = iff.make_desc(np.array([0.0, 1.0]), np.array([0.0, 1.0]), dt=0.1, re=1.0)
+    h = C.c_void_p()
+    rc = lib.if_case_create(C.byref(d), C.byref(h))
+    assert rc in (_abi.IF_ERR_ARG, _abi.IF_ERR_NODEV)
+    assert lib.if_case_step(None, 1, None) == _abi.IF_ERR_ARG

@@ -1,0 +1,390 @@
+"""Pins of the oracle's physics-mode functions (CPU).
+
+The reference has no code for the divergence RHS, the Poisson operator as
+intended, the projection or the immersed boundary (PPESolver.cu does not
+compile, AD_PPE_Correction.cu is empty, iBlankComputeKernel is a stub), so
+these functions are held here by closed forms, invariants, independent
+brute force and direct solves -- each chosen so that a plausible mistake
+(a wrong coefficient, sign or index, a dropped term) fails.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN
+
+
+def uniform_faces(n, L):
+    return np.linspace(0.0, L, n + 1)
+
+
+def stretched_faces(n, L, ratio=1.03):
+    """Geometric widths, normalised to length L."""
+    w = ratio ** np.arange(n)
+    w = w / w.sum() * L
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
+def prm(**kw):
+    base = dict(dt=0.01, re=100.0, conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25,
+                ad_tol=1e-6, ppe_solver=1, ppe_omega=1.8, ppe_itermax=200000,
+                ppe_tol=1e-10, check_every=5)
+    base.update(kw)
+    return base
+
+
+def grid2(a, nx, ny):
+    return a.reshape(ny, nx)
+
+
+# ---------------------------------------------------------------------------
+# Poisson operator and solvers
+# ---------------------------------------------------------------------------
+def _flags_nobody(nx, ny, xc, yc):
+    bs = O.BodySet([])
+    flags, bid = O.classify(xc, yc, bs)
+    return flags, bid
+
+
+def test_poisson_operator_is_div_of_face_gradient():
+    """A p == D(G p) with G the face gradient (p_E - p_P) * 2/(dx_P + dx_E) on
+    open faces and zero on closed ones, D the cell divergence."""
+    xf = stretched_faces(30, 3.0, 1.05)
+    yf = stretched_faces(22, 2.0, 0.96)
+    xc, dx = O.metric(xf)
+    yc, dy = O.metric(yf)
+    nx, ny = len(xc), len(yc)
+    bs = O.BodySet([dict(kind=O.CIRCLE, xc=1.4, yc=1.1, r=0.35)])
+    flags, _ = O.classify(xc, yc, bs)
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(nx * ny)
+    r = np.zeros(nx * ny)
+    O.lib().of_phys_residual(nx, ny, O.dptr(dx), O.dptr(dy), O.u8ptr(flags),
+                             O.dptr(np.zeros(nx * ny)), O.dptr(p), O.dptr(r))
+    F = grid2(flags, nx, ny)
+    P = grid2(p, nx, ny)
+    Ap = np.zeros((ny, nx))
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            if not F[j, i] & O.OF_FLUID:
+                continue
+            def gx(a, b):  # gradient on the face between cells a=(i) and b=(i+1)
+                return (P[j, b] - P[j, a]) * 2.0 / (dx[a] + dx[b])
+            def gy(a, b):
+                return (P[b, i] - P[a, i]) * 2.0 / (dy[a] + dy[b])
+            fe = gx(i, i + 1) if F[j, i] & O.OF_OPEN_E else 0.0
+            fw = gx(i - 1, i) if F[j, i] & O.OF_OPEN_W else 0.0
+            fn = gy(j, j + 1) if F[j, i] & O.OF_OPEN_N else 0.0
+            fs = gy(j - 1, j) if F[j, i] & O.OF_OPEN_S else 0.0
+            Ap[j, i] = (fe - fw) / dx[i] + (fn - fs) / dy[j]
+    assert np.max(np.abs(-grid2(r, nx, ny) - Ap)) <= 1e-10 * np.max(np.abs(Ap))
+
+
+def _neumann_solve_direct(flags, dx, dy, rhs):
+    """Direct sparse solve of the Neumann system on fluid cells, mean-zero."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    nx, ny = len(dx), len(dy)
+    F = grid2(flags, nx, ny)
+    ids = -np.ones((ny, nx), dtype=np.int64)
+    cells = [(j, i) for j in range(1, ny - 1) for i in range(1, nx - 1) if F[j, i] & O.OF_FLUID]
+    for k, (j, i) in enumerate(cells):
+        ids[j, i] = k
+    n = len(cells)
+    A = sp.lil_matrix((n + 1, n + 1))
+    b = np.zeros(n + 1)
+    R = grid2(rhs, nx, ny)
+    for k, (j, i) in enumerate(cells):
+        diag = 0.0
+        for bit, di, dj, c in ((O.OF_OPEN_E, 1, 0, 2 / (dx[i] * (dx[i] + dx[i + 1]))),
+                               (O.OF_OPEN_W, -1, 0, 2 / (dx[i] * (dx[i] + dx[i - 1]))),
+                               (O.OF_OPEN_N, 0, 1, 2 / (dy[j] * (dy[j] + dy[j + 1]))),
+                               (O.OF_OPEN_S, 0, -1, 2 / (dy[j] * (dy[j] + dy[j - 1])))):
+            if F[j, i] & bit:
+                A[k, ids[j + dj, i + di]] = c
+                diag -= c
+        A[k, k] = diag
+        A[k, n] = dx[i] * dy[j]    # Lagrange multiplier for the gauge
+        A[n, k] = dx[i] * dy[j]
+        b[k] = R[j, i]
+    x = spl.spsolve(A.tocsc(), b)
+    out = np.zeros(nx * ny)
+    for k, (j, i) in enumerate(cells):
+        out[i + j * nx] = x[k]
+    return out, cells
+
+
+def _gauge(p, cells, dx, dy, nx):
+    w = np.array([dx[i] * dy[j] for j, i in cells])
+    v = np.array([p[i + j * nx] for j, i in cells])
+    return v - np.sum(w * v) / np.sum(w)
+
+
+@pytest.mark.parametrize("solver,omega", [(1, 1.7), (0, 0.8)])
+def test_poisson_iteration_matches_direct_solve(solver, omega):
+    xf = stretched_faces(20, 2.0, 1.04)
+    yf = stretched_faces(16, 1.6, 1.02)
+    xc, dx = O.metric(xf)
+    yc, dy = O.metric(yf)
+    nx, ny = len(xc), len(yc)
+    bs = O.BodySet([dict(kind=O.CIRCLE, xc=0.9, yc=0.8, r=0.3)])
+    flags, _ = O.classify(xc, yc, bs)
+    rng = np.random.default_rng(7)
+    # a compatible right side: the divergence of random face velocities
+    uf = rng.standard_normal((nx - 1) * (ny - 2))
+    vf = rng.standard_normal((nx - 2) * (ny - 1))
+    F = grid2(flags, nx, ny)
+    for j in range(1, ny - 1):
+        for i in range(0, nx - 1):
+            if i == 0 or i == nx - 2 or not (F[j, i] & O.OF_OPEN_E):
+                uf[i + (j - 1) * (nx - 1)] = 0.0
+    for j in range(0, ny - 1):
+        for i in range(1, nx - 1):
+            if j == 0 or j == ny - 2 or not (F[j, i] & O.OF_OPEN_N):
+                vf[(i - 1) + j * (nx - 2)] = 0.0
+    rhs = np.zeros(nx * ny)
+    O.lib().of_phys_rhs(nx, ny, O.dptr(dx), O.dptr(dy), O.u8ptr(flags), 1.0, O.dptr(uf),
+                        O.dptr(vf), O.dptr(rhs))
+    direct, cells = _neumann_solve_direct(flags, dx, dy, rhs)
+    p = np.zeros(nx * ny)
+    pr = O.Params(**prm(dt=1.0, ppe_solver=solver, ppe_omega=omega, ppe_tol=1e-11,
+                        ppe_itermax=400000, check_every=10))
+    res = C.c_double()
+    O.lib().of_phys_poisson(nx, ny, O.dptr(dx), O.dptr(dy), O.u8ptr(flags), O.dptr(rhs),
+                            C.byref(pr), O.dptr(p), C.byref(res))
+    assert res.value <= 1e-11
+    a = _gauge(p, cells, dx, dy, nx)
+    b = _gauge(direct, cells, dx, dy, nx)
+    assert np.max(np.abs(a - b)) <= 1e-8 * max(1.0, np.max(np.abs(b)))
+
+
+def test_poisson_second_order_on_manufactured_solution():
+    """p = cos(pi x/L) cos(pi y/L) meets the Neumann condition; the discrete
+    solution converges at second order."""
+    L = 1.0
+    errs = []
+    for n in (16, 32, 64):
+        xf = uniform_faces(n, L)
+        xc, dx = O.metric(xf)
+        nx = len(xc)
+        flags, _ = _flags_nobody(nx, nx, xc, xc)
+        X, Y = np.meshgrid(xc, xc)
+        pe = np.cos(np.pi * X / L) * np.cos(np.pi * Y / L)
+        rhs = (-2.0 * (np.pi / L) ** 2 * pe).ravel().copy()
+        F = flags.reshape(nx, nx)
+        rhs[(F.ravel() & O.OF_FLUID) == 0] = 0.0
+        rhs[[k for k in range(nx * nx) if (k % nx in (0, nx - 1)) or (k // nx in (0, nx - 1))]] = 0.0
+        p = np.zeros(nx * nx)
+        pr = O.Params(**prm(dt=1.0, ppe_omega=2 / (1 + math.sin(math.pi / n)), ppe_tol=1e-11,
+                            ppe_itermax=500000, check_every=20))
+        res = C.c_double()
+        O.lib().of_phys_poisson(nx, nx, O.dptr(dx), O.dptr(dx), O.u8ptr(flags), O.dptr(rhs),
+                                C.byref(pr), O.dptr(p), C.byref(res))
+        P = p.reshape(nx, nx)[1:-1, 1:-1]
+        E = pe[1:-1, 1:-1]
+        P = P - P.mean() + E.mean()
+        errs.append(np.max(np.abs(P - E)))
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
+
+
+# ---------------------------------------------------------------------------
+# full step: invariants
+# ---------------------------------------------------------------------------
+def test_free_stream_is_preserved_on_stretched_grid():
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat"))
+    yf = O.read_grid_file(os.path.join(GOLDEN, "ygrid.dat"))
+    c = O.PhysCase(xf, yf, [], prm(dt=0.005, ppe_tol=1e-12, check_every=10))
+    out = c.step(3)
+    assert np.max(np.abs(c.u - 1.0)) < 1e-12
+    assert np.max(np.abs(c.v)) < 1e-12
+    assert out[-1][4] < 1e-12
+
+
+def test_projection_leaves_divergence_equal_to_poisson_residual():
+    xf = stretched_faces(40, 4.0, 1.02)
+    yf = stretched_faces(30, 3.0, 1.01)
+    c = O.PhysCase(xf, yf, [dict(kind=O.CIRCLE, xc=1.5, yc=1.5, r=0.4)],
+                   prm(dt=0.02, ppe_tol=1e-7, check_every=3))
+    for ad_it, ad_res, sw, pres, div in c.step(4):
+        assert sw > 0
+        assert abs(div - pres) <= 1e-9 + 1e-6 * pres
+        assert div <= 1e-7
+
+
+def test_ad_stopping_rule_and_sweep_counts():
+    xf = uniform_faces(32, 2.0)
+    c = O.PhysCase(xf, xf, [dict(kind=O.CIRCLE, xc=1.0, yc=1.0, r=0.3)],
+                   prm(dt=0.02, ad_itermax=3))
+    out = c.step(2)
+    assert all(o[0] == 3 for o in out)
+
+
+# ---------------------------------------------------------------------------
+# immersed boundary
+# ---------------------------------------------------------------------------
+def _c1_grid(n=256):
+    xf = uniform_faces(n, 16.0)
+    xc, dx = O.metric(xf)
+    return xf, xc, dx
+
+
+def test_circle_classification_and_ghost_list_brute_force():
+    xf, xc, dx = _c1_grid(256)
+    nx = len(xc)
+    bs = O.BodySet([dict(kind=O.CIRCLE, xc=5.0, yc=8.0, r=0.5)])
+    flags, bid = O.classify(xc, xc, bs)
+    X, Y = np.meshgrid(xc, xc)
+    solid = (X - 5.0) ** 2 + (Y - 8.0) ** 2 <= 0.25
+    F = grid2(flags, nx, nx)
+    assert np.array_equal((F & O.OF_FLUID) == 0, solid)
+    fl = ~solid
+    ghost = np.zeros_like(solid)
+    ghost[1:-1, 1:-1] = solid[1:-1, 1:-1] & (fl[1:-1, 2:] | fl[1:-1, :-2] | fl[2:, 1:-1] | fl[:-2, 1:-1])
+    assert np.array_equal((F & O.OF_GHOST) != 0, ghost)
+    lst, gh, ng = O.ghost_records(xc, xc, flags, bid, bs)
+    assert np.array_equal(lst, np.flatnonzero(ghost.ravel()))
+    assert ng == ghost.sum() > 0
+    # open faces: both sides fluid and not on the domain boundary
+    oe = np.zeros_like(solid)
+    oe[1:-1, 1:-2] = fl[1:-1, 1:-2] & fl[1:-1, 2:-1]
+    assert np.array_equal((F & O.OF_OPEN_E) != 0, oe)
+
+
+def test_circle_image_points_and_no_slip_closure():
+    xf = stretched_faces(48, 4.0, 1.01)
+    xc, dx = O.metric(xf)
+    yc = xc
+    nx = ny = len(xc)
+    body = dict(kind=O.CIRCLE, xc=2.05, yc=1.93, r=0.61)
+    bs = O.BodySet([body])
+    flags, bid = O.classify(xc, yc, bs)
+    lst, gh, ng = O.ghost_records(xc, yc, flags, bid, bs)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(nx * ny)
+    v = rng.standard_normal(nx * ny)
+    O.lib().of_phys_ghost_velocity(nx, ny, ng, gh, 1, bs.arr, O.dptr(u), O.dptr(v))
+    F = flags
+    for g in range(ng):
+        o = gh[g]
+        gi, gj = o.gidx % nx, o.gidx // nx
+        gx, gy = xc[gi], yc[gj]
+        # boundary intercept on the circle, along the radius
+        assert abs(math.hypot(o.bix - 2.05, o.biy - 1.93) - 0.61) < 1e-13
+        cross = (o.bix - 2.05) * (gy - 1.93) - (o.biy - 1.93) * (gx - 2.05)
+        assert abs(cross) < 1e-13
+        assert abs(o.ipx - (2 * o.bix - gx)) < 1e-15 and abs(o.ipy - (2 * o.biy - gy)) < 1e-15
+        # bilinear interpolation at IP with the ghost's own new value and the
+        # body value (0) on other solid nodes: (u_GC + u_IP)/2 = 0
+        i0 = np.searchsorted(xc, o.ipx, side="right") - 1
+        j0 = np.searchsorted(yc, o.ipy, side="right") - 1
+        fx = (o.ipx - xc[i0]) / (xc[i0 + 1] - xc[i0])
+        fy = (o.ipy - yc[j0]) / (yc[j0 + 1] - yc[j0])
+        uip = 0.0
+        for (ii, jj, w) in ((i0, j0, (1 - fx) * (1 - fy)), (i0 + 1, j0, fx * (1 - fy)),
+                            (i0, j0 + 1, (1 - fx) * fy), (i0 + 1, j0 + 1, fx * fy)):
+            k = ii + jj * nx
+            if k == o.gidx or (F[k] & O.OF_FLUID):
+                uip += w * u[k]
+        assert abs(0.5 * (u[o.gidx] + uip)) < 1e-12
+        assert abs(sum(o.wn) - 1.0) < 1e-14 or sum(o.wn) == 0.0
+
+
+def test_pressure_neumann_closure():
+    """p_GC equals the bilinear interpolation at the image point over its
+    fluid nodes, renormalised (dp/dn = 0 across the wall), computed here
+    independently from the image point's coordinates."""
+    xf = stretched_faces(40, 4.0, 1.015)
+    xc, dx = O.metric(xf)
+    nx = len(xc)
+    bs = O.BodySet([dict(kind=O.CIRCLE, xc=2.0, yc=1.9, r=0.7)])
+    flags, bid = O.classify(xc, xc, bs)
+    lst, gh, ng = O.ghost_records(xc, xc, flags, bid, bs)
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(nx * nx)
+    p0 = p.copy()
+    O.lib().of_phys_ghost_pressure(nx, nx, ng, gh, 1, bs.arr, O.dptr(p))
+    for g in range(ng):
+        o = gh[g]
+        i0 = np.searchsorted(xc, o.ipx, side="right") - 1
+        j0 = np.searchsorted(xc, o.ipy, side="right") - 1
+        fx = (o.ipx - xc[i0]) / (xc[i0 + 1] - xc[i0])
+        fy = (o.ipy - xc[j0]) / (xc[j0 + 1] - xc[j0])
+        num = den = 0.0
+        for (ii, jj, w) in ((i0, j0, (1 - fx) * (1 - fy)), (i0 + 1, j0, fx * (1 - fy)),
+                            (i0, j0 + 1, (1 - fx) * fy), (i0 + 1, j0 + 1, fx * fy)):
+            k = ii + jj * nx
+            if k != o.gidx and (flags[k] & O.OF_FLUID):
+                num += w * p0[k]
+                den += w
+        assert den > 0
+        assert abs(p[o.gidx] - num / den) < 1e-12
+    # a constant field is reproduced exactly (weights sum to one)
+    q = np.full(nx * nx, 2.5)
+    O.lib().of_phys_ghost_pressure(nx, nx, ng, gh, 1, bs.arr, O.dptr(q))
+    assert np.max(np.abs(q - 2.5)) < 1e-14
+
+
+def test_ellipse_closest_point_brute_force():
+    th = math.radians(10.0)
+    body = dict(kind=O.ELLIPSE, xc=0.3, yc=-0.2, a=0.5, b=0.05, c=math.cos(th), s=math.sin(th))
+    bs = O.BodySet([body])
+    t = np.linspace(0, 2 * np.pi, 400001)
+    ex = 0.5 * np.cos(t)
+    ey = 0.05 * np.sin(t)
+    bxs = 0.3 + ex * math.cos(th) - ey * math.sin(th)
+    bys = -0.2 + ex * math.sin(th) + ey * math.cos(th)
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        px, py = 0.3 + rng.uniform(-0.6, 0.6), -0.2 + rng.uniform(-0.2, 0.2)
+        bx, by = C.c_double(), C.c_double()
+        O.lib().of_closest_point(C.byref(bs.arr[0]), px, py, C.byref(bx), C.byref(by))
+        d = math.hypot(px - bx.value, py - by.value)
+        dmin = np.min(np.hypot(px - bxs, py - bys))
+        assert d <= dmin + 1e-9
+        assert d >= dmin - 2e-6   # sampling resolution of the brute force
+        # the point lies on the ellipse
+        X = (bx.value - 0.3) * math.cos(th) + (by.value + 0.2) * math.sin(th)
+        Y = (by.value + 0.2) * math.cos(th) - (bx.value - 0.3) * math.sin(th)
+        assert abs((X / 0.5) ** 2 + (Y / 0.05) ** 2 - 1.0) < 1e-9
+
+
+def _star(xc, yc, R, a, ph, n=400):
+    t = 2 * np.pi * np.arange(n) / n
+    r = R * (1 + sum(a[k] * np.cos((k + 2) * t + ph[k]) for k in range(5)))
+    return xc + r * np.cos(t), yc + r * np.sin(t), float(r.max()) * 1.0001
+
+
+def test_star_polygon_inside_and_closest_point():
+    rng = np.random.default_rng(5)
+    vx, vy, rb = _star(1.0, 1.0, 0.5, rng.uniform(0, 0.08, 5), rng.uniform(0, 2 * np.pi, 5))
+    body = dict(kind=O.POLYGON, xc=1.0, yc=1.0, r=rb, vx=vx, vy=vy)
+    bs = O.BodySet([body])
+    xs = np.linspace(0.2, 1.8, 81)
+    flags, _ = O.classify(xs, xs, bs)
+    # winding number by angle summation (independent of the crossing test)
+    X, Y = np.meshgrid(xs, xs)
+    ang = np.zeros_like(X)
+    for k in range(len(vx)):
+        k1 = (k + 1) % len(vx)
+        a1 = np.arctan2(vy[k] - Y, vx[k] - X)
+        a2 = np.arctan2(vy[k1] - Y, vx[k1] - X)
+        d = a2 - a1
+        d = (d + np.pi) % (2 * np.pi) - np.pi
+        ang += d
+    inside = np.abs(ang) > np.pi
+    assert np.array_equal((grid2(flags, 81, 81) & O.OF_FLUID) == 0, inside)
+    # closest point = brute force over dense edge samples
+    s = np.linspace(0, 1, 201)[:-1]
+    px = np.concatenate([vx[k] + s * (vx[(k + 1) % 400] - vx[k]) for k in range(400)])
+    py = np.concatenate([vy[k] + s * (vy[(k + 1) % 400] - vy[k]) for k in range(400)])
+    for _ in range(20):
+        qx, qy = rng.uniform(0.3, 1.7), rng.uniform(0.3, 1.7)
+        bx, by = C.c_double(), C.c_double()
+        O.lib().of_closest_point(C.byref(bs.arr[0]), qx, qy, C.byref(bx), C.byref(by))
+        d = math.hypot(qx - bx.value, qy - by.value)
+        dmin = np.min(np.hypot(qx - px, qy - py))
+        assert dmin - 1e-4 <= d <= dmin + 1e-12
