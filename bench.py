@@ -5,7 +5,7 @@ Workload: the single large case -- an ellipse (chord 1, thickness ratio 0.1)
 at 10 deg incidence, Re = 300, on a 2048 x 1024 grid (uniform h = 0.005 in a
 4 x 2 box around the body, stretched to a 30 x 20 domain), dt = 0.5 h.  A
 step is one full fractional step (predictor, divergence RHS, Poisson to
-max|div| <= 1e-7, projection, divergence check) on device-resident fields.
+max|div| <= 1e-6, projection, divergence check) on device-resident fields.
 
 value = interior cells x steps / wall seconds (Mcell-steps/s) of the
 mixed-precision path (FP32 stencils and Poisson sweeps, FP64 refinement,
@@ -35,10 +35,12 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernel", type=int, default=0, help="0 LDS-tiled, 1 register-streaming")
     ap.add_argument("--tile-y", type=int, default=16)
-    ap.add_argument("--sweeps-per-launch", type=int, default=1)
-    ap.add_argument("--check-every", type=int, default=20)
-    ap.add_argument("--omega", type=float, default=0.0)
+    ap.add_argument("--sweeps-per-launch", type=int, default=2)
+    ap.add_argument("--check-every", type=int, default=40)
+    ap.add_argument("--fp64-tile-y", type=int, default=8)
+    ap.add_argument("--omega", type=float, default=1.998)
     return ap.parse_args()
 
 
@@ -46,12 +48,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_case(iff, cfg, precision, a):
+def make_case(iff, cfg, precision, a, tile_y=None):
     c = iff.Case(cfg["xf"], cfg["yf"], precision=precision, dt=cfg["dt"], re=cfg["re"],
                  bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_RBSOR,
                  ppe_omega=a.omega, ppe_itermax=200000, ppe_tol=cfg["ppe_tol"],
                  check_every=a.check_every)
-    c.tune(a.tile_y, a.sweeps_per_launch, a.check_every)
+    c.tune(a.kernel, tile_y or a.tile_y, a.sweeps_per_launch, a.check_every)
     return c
 
 
@@ -71,7 +73,7 @@ def timed_run(case, warmup, steps, cells, label):
                 mcs=cells * steps / dt / 1e6)
 
 
-def cpu_baseline(cfg, gpu_case, st):
+def cpu_baseline(cfg, gpu_case, st, omega):
     """One oracle step (FP64, OpenMP) from the GPU case's state."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -86,7 +88,7 @@ def cpu_baseline(cfg, gpu_case, st):
         d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
         bodies.append(d)
     oc = O.PhysCase(cfg["xf"], cfg["yf"], bodies, prm)
-    oc.prm.ppe_omega = gpu_case_omega(oc)
+    oc.prm.ppe_omega = omega
     for name in ("u", "v", "p", "uf", "vf"):
         getattr(oc, name)[:] = gpu_case.get(name)
     t0 = time.perf_counter()
@@ -99,9 +101,10 @@ def cpu_baseline(cfg, gpu_case, st):
                        "Poisson sweeps, %.1f s)" % (out[0], out[2], t1 - t0))
 
 
-def gpu_case_omega(oc):
+def gpu_omega(cfg):
+    """The library's default relaxation: 2/(1 + sin(pi/N)), N the larger cell count."""
     import math
-    n = max(oc.nx - 2, oc.ny - 2)
+    n = max(len(cfg["xf"]) - 1, len(cfg["yf"]) - 1)
     return 2.0 / (1.0 + math.sin(math.pi / n))
 
 
@@ -132,13 +135,14 @@ def main():
             traffic = json.load(f).get("mixed_bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_rbsor<float,%d,%d>" % (a.tile_y, a.sweeps_per_launch),
+                kernel=("k_rbsor<float,%d,%d>" if a.kernel == 0 else "k_rbsor_stream<float,%d,%d>")
+                % ((a.tile_y, a.sweeps_per_launch) if a.kernel == 0 else (a.sweeps_per_launch, a.tile_y)),
                 bytes_per_launch=sweep_bpc * cells, mean_launch_us=round(avg_ms * 1e3, 3),
                 launches_timed=mixed["sweep_launches"])
 
     fp64 = None
     if not a.no_fp64 and a.fp64_steps > 0:
-        c64 = make_case(iff, cfg, iff.IF_FP64, a)
+        c64 = make_case(iff, cfg, iff.IF_FP64, a, a.fp64_tile_y)
         r64 = timed_run(c64, max(1, a.warmup), a.fp64_steps, cells, "fp64")
         b64, _ = c64.byte_model()
         avg64 = r64["sweep_ms"] / max(1, r64["sweep_launches"])
@@ -150,7 +154,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(cfg, case, mixed["stats"])
+        cpu = cpu_baseline(cfg, case, mixed["stats"], a.omega if a.omega > 0 else gpu_omega(cfg))
         log("cpu baseline", cpu)
 
     st = mixed["stats"]

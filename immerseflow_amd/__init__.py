@@ -245,8 +245,10 @@ class Case:
         _check(_lib().if_case_ghosts(self._h, lst, recs, k, C.byref(n)))
         return np.array(lst[:k], dtype=np.int32), recs, k
 
-    def tune(self, tile_y=16, sweeps_per_launch=1, check_every=10):
-        _check(_lib().if_case_tune(self._h, tile_y, sweeps_per_launch, check_every), "tune")
+    def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
+        """Poisson sweep kernel (0 LDS-tiled, 1 register-streaming), tile or
+        strip height, red-black sweeps per launch, sweeps per check."""
+        _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def sweep_timing(self, reset=False):
         ms, n = C.c_double(), C.c_longlong()
@@ -320,8 +322,8 @@ class Batch:
         _check(_lib().if_batch_remove(self._h, slot), "remove")
         self._keep.pop(slot, None)
 
-    def tune(self, tile_y=16, sweeps_per_launch=1, check_every=10):
-        _check(_lib().if_batch_tune(self._h, tile_y, sweeps_per_launch, check_every), "tune")
+    def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
+        _check(_lib().if_batch_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def info(self):
         r, b, ms, n = C.c_int(), C.c_size_t(), C.c_double(), C.c_longlong()

@@ -85,8 +85,8 @@ SIGNATURES = [
     ("if_case_dims", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("if_case_ghosts", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(IfGhost),
                                  C.c_size_t, C.POINTER(C.c_size_t)]),
-    ("if_case_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
-    ("if_batch_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("if_case_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("if_batch_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("if_case_sweep_timing", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     ("if_case_byte_model", C.c_int, [C.c_void_p, _dp, _dp]),
     ("if_batch_create", C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),

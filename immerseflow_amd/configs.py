@@ -74,7 +74,7 @@ def c2(n=(2048, 1024)):
     yf, ry0, ry1 = stretched_box_faces(n[1], h, 9.0, 11.0, 0.0, 20.0)
     body = ellipse(15.0, 10.0, 0.5, 0.05, math.radians(10.0))
     return dict(xf=xf, yf=yf, bodies=[body], dt=0.5 * h, re=300.0, steps=2000,
-                ppe_tol=1e-7, ratios=(rx0, rx1, ry0, ry1),
+                ppe_tol=1e-6, ratios=(rx0, rx1, ry0, ry1),
                 name="C2 ellipse 2048x1024 stretched")
 
 

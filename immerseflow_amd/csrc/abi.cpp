@@ -186,10 +186,15 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs, size_t cap, s
     IFF_GUARD(c->eng->ghosts(c->slot, out, recs, cap, n));
 }
 
-static if_status tune(Engine &e, int ty, int m, int k)
+static if_status tune(Engine &e, int kind, int ty, int m, int k)
 {
-    if (!sweep_variant_ok(ty, m)) return fail(IF_ERR_ARG, "tune: tile_y must be 8/16/32, sweeps 1/2/4");
+    if (kind == 0 && !sweep_variant_ok(ty, m))
+        return fail(IF_ERR_ARG, "tune: LDS sweep needs tile_y 8/16/32 and 1/2/4 sweeps per launch");
+    if (kind == 1 && !stream_variant_ok(ty, m))
+        return fail(IF_ERR_ARG, "tune: streaming sweep needs strip height 16/32/64/128 and 1/2/4 sweeps");
+    if (kind != 0 && kind != 1) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
     if (k < 1) return fail(IF_ERR_ARG, "tune: check_every must be >= 1");
+    e.sweep_kind = kind;
     e.sweep_ty = ty;
     e.sweep_m = m;
     e.check_k = k;
@@ -197,16 +202,16 @@ static if_status tune(Engine &e, int ty, int m, int k)
     return IF_OK;
 }
 
-if_status if_case_tune(if_case *c, int tile_y, int m, int check_every)
+if_status if_case_tune(if_case *c, int kernel, int tile_y, int m, int check_every)
 {
     if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "tune: physics mode only");
-    return tune(*c->eng, tile_y, m, check_every);
+    return tune(*c->eng, kernel, tile_y, m, check_every);
 }
 
-if_status if_batch_tune(if_batch *b, int tile_y, int m, int check_every)
+if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int m, int check_every)
 {
     if (!b) return fail(IF_ERR_ARG, "null batch");
-    return tune(*b->eng, tile_y, m, check_every);
+    return tune(*b->eng, kernel, tile_y, m, check_every);
 }
 
 if_status if_case_sweep_timing(if_case *c, int reset, double *ms, long long *launches)

@@ -13,6 +13,11 @@ void launch_pressure_bc(hipStream_t st, const CaseDev *cs, int ncase, int max_ri
 bool sweep_variant_ok(int ty, int m);
 void launch_ppe_sweep(hipStream_t st, int prec, int solver, int ty, int m, const CaseDev *cs,
                       const int2 *tiles, int nt, int s, int K);
+// register-streaming red-black SOR (sweep_stream.hip)
+bool stream_variant_ok(int ty, int m);
+int stream_out_cols(int m);
+void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const CaseDev *cs,
+                             const int2 *tiles, int nt, int s, int K);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
                       int nt, int K, int M);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);

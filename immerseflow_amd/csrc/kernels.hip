@@ -378,6 +378,75 @@ struct PpeAxis {
     const T *cp, *cm;
 };
 
+// Global-memory accessors: pointers read from the case descriptor are
+// generic; casting them to the global address space makes the compiler
+// emit global_load/store instead of flat accesses (which also count against
+// the LDS wait counter).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *G(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *GW(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// One red-black half-sweep (colour h&1) on the LDS tile, over the region
+// that is still valid after h half-sweeps: the tile plus a ring of
+// R = 2M-1-h cells.  All sizes are compile-time constants.
+template <typename T, int TY, int M, int h>
+__device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M - 2],
+                                              const uint8_t (*sf)[kTileX + 4 * M - 2],
+                                              const T *mxp, const T *mxm, const T *myp, const T *mym,
+                                              int gpar, T om)
+{
+    constexpr int H = 2 * M;
+    constexpr int R = H - 1 - h;
+    constexpr int colour = h & 1;
+    constexpr int rows = TY + 2 * R;
+    constexpr int wid = kTileX + 2 * R;
+    constexpr int pairs = (wid + 1) / 2;
+    constexpr int items = rows * pairs;
+#pragma unroll 2
+    for (int k = threadIdx.x; k < items; k += 256) {
+        const int row = k / pairs, pr = k - row * pairs;
+        const int ly = H - R + row;
+        int lx = H - R + 2 * pr;
+        // global parity of (lx, ly) is (gpar + lx + ly) & 1
+        lx += ((gpar + lx + ly) ^ colour) & 1;
+        if (lx >= H + kTileX + R) continue;
+        const uint8_t f = sf[ly - 1][lx - 1];
+        if (!(f & F_FLUID)) continue;
+        const T aE = (f & F_OPEN_E) ? mxp[lx - 1] : (T)0;
+        const T aW = (f & F_OPEN_W) ? mxm[lx - 1] : (T)0;
+        const T aN = (f & F_OPEN_N) ? myp[ly - 1] : (T)0;
+        const T aS = (f & F_OPEN_S) ? mym[ly - 1] : (T)0;
+        const T ap = -((aE + aW) + (aN + aS));
+        if (ap == (T)0) continue;
+        const T sum = (aE * sp[ly][lx + 1] + aW * sp[ly][lx - 1]) + (aN * sp[ly + 1][lx] + aS * sp[ly - 1][lx]);
+        const T pc = sp[ly][lx];
+        const T gs = (sr[ly - 1][lx - 1] - sum) / ap;
+        sp[ly][lx] = pc + om * (gs - pc);
+    }
+}
+
+template <typename T, int TY, int M, int h>
+__device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M - 2],
+                                               const uint8_t (*sf)[kTileX + 4 * M - 2], const T *mxp,
+                                               const T *mxm, const T *myp, const T *mym, int gpar, T om,
+                                               int nst)
+{
+    if constexpr (h < 2 * M) {
+        if (h < nst) {
+            rb_half_sweep<T, TY, M, h>(sp, sr, sf, mxp, mxm, myp, mym, gpar, om);
+            __syncthreads();
+        }
+        rb_half_sweeps<T, TY, M, h + 1>(sp, sr, sf, mxp, mxm, myp, mym, gpar, om, nst);
+    }
+}
+
 // Red-black SOR, M full sweeps per launch on an LDS tile with a 2M halo
 // (temporal blocking).  Reads the unknown from buffer src, writes dst: the
 // ping-pong keeps tiles from reading halos that other tiles are writing,
@@ -395,6 +464,7 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     __shared__ T sp[HT][W];
     __shared__ T sr[HT1][W1];
     __shared__ uint8_t sf[HT1][W1];
+    __shared__ T mxp[W1], mxm[W1], myp[HT1], mym[HT1];
 
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
@@ -412,87 +482,71 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     const T *src;
     T *dst;
     const T *rhs;
+    const T *cxp, *cxm, *cyp, *cym;
     bool zero_src = false;
-    PpeAxis<T> ax, ay;
     T om;
     if constexpr (sizeof(T) == 8) {
         src = c.p[(ct->ppe_base + L) & 1];
         dst = c.p[(ct->ppe_base + L + 1) & 1];
         rhs = c.rhs;
-        ax = {c.x.cp, c.x.cm};
-        ay = {c.y.cp, c.y.cm};
+        cxp = c.x.cp; cxm = c.x.cm; cyp = c.y.cp; cym = c.y.cm;
         om = c.omega;
     } else {
         src = c.e[L & 1];
         dst = c.e[(L + 1) & 1];
         zero_src = (L == 0);
         rhs = c.r32;
-        ax = {c.xf.cp, c.xf.cm};
-        ay = {c.yf.cp, c.yf.cm};
+        cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
         om = c.omega_f;
     }
 
     const int i0 = 1 + tc.tx * kTileX, j0 = 1 + tc.ty * TY;
-    // load phase: p with halo H, rhs/flags with halo H-1
+    // load phase: unknown with halo H; rhs, flags and metric with halo H-1
+#pragma unroll 4
     for (int k = threadIdx.x; k < W * HT; k += 256) {
         const int lx = k % W, ly = k / W;
         const int gi = i0 - H + lx, gj = j0 - H + ly;
         T val = (T)0;
-        if (!zero_src && gi >= 0 && gi < nx && gj >= 0 && gj < ny) val = src[gi + gj * P];
+        if (!zero_src && gi >= 0 && gi < nx && gj >= 0 && gj < ny) val = G(src)[gi + gj * P];
         sp[ly][lx] = val;
     }
+#pragma unroll 4
     for (int k = threadIdx.x; k < W1 * HT1; k += 256) {
         const int lx = k % W1, ly = k / W1;
         const int gi = i0 - (H - 1) + lx, gj = j0 - (H - 1) + ly;
         T rv = (T)0;
         uint8_t fv = 0;
         if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= ny - 2) {
-            rv = rhs[gi + gj * P];
-            fv = c.flags[gi + gj * P];
+            rv = G(rhs)[gi + gj * P];
+            fv = G(c.flags)[gi + gj * P];
         }
         sr[ly][lx] = rv;
         sf[ly][lx] = fv;
     }
+    if (threadIdx.x < W1) {
+        const int gi = i0 - (H - 1) + threadIdx.x;
+        const bool in = gi >= 1 && gi <= nx - 2;
+        mxp[threadIdx.x] = in ? G(cxp)[gi] : (T)0;
+        mxm[threadIdx.x] = in ? G(cxm)[gi] : (T)0;
+    } else if (threadIdx.x >= 128 && threadIdx.x < 128 + HT1) {
+        const int q = threadIdx.x - 128;
+        const int gj = j0 - (H - 1) + q;
+        const bool in = gj >= 1 && gj <= ny - 2;
+        myp[q] = in ? G(cyp)[gj] : (T)0;
+        mym[q] = in ? G(cym)[gj] : (T)0;
+    }
     __syncthreads();
 
-    // 2*meff half-sweeps on a shrinking region (ring H-1-h around the tile)
-    for (int h = 0; h < 2 * meff; h++) {
-        const int colour = h & 1;
-        const int R = H - 1 - h;
-        const int rows = TY + 2 * R;
-        const int wid = kTileX + 2 * R;
-        const int pairs = (wid + 1) >> 1;
-        for (int k = threadIdx.x; k < rows * pairs; k += 256) {
-            const int row = k / pairs, pr = k - row * pairs;
-            const int ly = H - R + row;
-            const int gj = j0 - H + ly;
-            int lx = H - R + 2 * pr;
-            const int gi0 = i0 - H + lx;
-            lx += ((gi0 + gj) ^ colour) & 1;
-            if (lx >= H + kTileX + R) continue;
-            const int gi = i0 - H + lx;
-            const uint8_t f = sf[ly - 1][lx - 1];
-            if (!(f & F_FLUID)) continue;
-            const T aE = (f & F_OPEN_E) ? ax.cp[gi] : (T)0;
-            const T aW = (f & F_OPEN_W) ? ax.cm[gi] : (T)0;
-            const T aN = (f & F_OPEN_N) ? ay.cp[gj] : (T)0;
-            const T aS = (f & F_OPEN_S) ? ay.cm[gj] : (T)0;
-            const T ap = -((aE + aW) + (aN + aS));
-            if (ap == (T)0) continue;
-            const T sum = (aE * sp[ly][lx + 1] + aW * sp[ly][lx - 1])
-                          + (aN * sp[ly + 1][lx] + aS * sp[ly - 1][lx]);
-            const T pc = sp[ly][lx];
-            const T gs = (sr[ly - 1][lx - 1] - sum) / ap;
-            sp[ly][lx] = pc + om * (gs - pc);
-        }
-        __syncthreads();
-    }
+    // 2*meff half-sweeps on shrinking regions
+    const int gpar = (i0 - H) + (j0 - H);
+    rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, om, 2 * meff);
 
     // write the tile (all interior cells: non-fluid values pass through)
+#pragma unroll
     for (int k = threadIdx.x; k < kTileX * TY; k += 256) {
         const int lx = k & (kTileX - 1), ly = k / kTileX;
         const int gi = i0 + lx, gj = j0 + ly;
-        if (gi <= nx - 2 && gj <= ny - 2) dst[gi + gj * P] = sp[H + ly][H + lx];
+        if (gi <= nx - 2 && gj <= ny - 2) GW(dst)[gi + gj * P] = sp[H + ly][H + lx];
     }
 }
 

@@ -183,10 +183,13 @@ if_status if_case_dims(if_case *c, int *nx, int *ny);
  * records.  *n gets the count; out/recs may be NULL to query it. */
 if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
                          size_t cap, size_t *n);
-/* Poisson sweep launch shape: tile height (8, 16 or 32 cells), red-black
- * sweeps per launch (1, 2 or 4: temporal blocking in LDS) and sweeps per
- * convergence check.  Results do not depend on the first two. */
-if_status if_case_tune(if_case *c, int tile_y, int sweeps_per_launch,
+/* Poisson sweep launch shape.  kernel 0: LDS-tiled red-black SOR, tile
+ * height 8/16/32 cells; kernel 1: register-streaming red-black SOR (one
+ * wave64 per column strip), strip height 16/32/64/128 rows.  Either does
+ * 1, 2 or 4 red-black sweeps per launch (temporal blocking); check_every is
+ * the number of sweeps between convergence checks.  Results do not depend
+ * on the kernel, the tile height or the sweeps per launch. */
+if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Accumulated event timing of the Poisson sweep kernel (milliseconds,
  * launches) over the steps since the last reset. */
@@ -222,7 +225,7 @@ if_status if_batch_remove(if_batch *b, int slot);
 if_status if_batch_info(if_batch *b, int *resident, size_t *bytes_in_use,
                         double *sweep_ms, long long *sweep_launches);
 /* As if_case_tune, for every case of the batch. */
-if_status if_batch_tune(if_batch *b, int tile_y, int sweeps_per_launch,
+if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int sweeps_per_launch,
                         int check_every);
 /* Footprint in device bytes that if_batch_add would charge for desc. */
 if_status if_case_footprint(const if_case_desc *desc, int precision,

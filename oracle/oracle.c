@@ -396,6 +396,28 @@ void of_classify(int nx, int ny, const double *xc, const double *yc,
         }
 }
 
+int of_phys_move_bodies(int nx, int ny, const double *xc, const double *yc,
+                        int nbody, const of_body *bodies, uint8_t *flags,
+                        int8_t *body_id, double *u, double *v)
+{
+    const size_t n = (size_t)nx * ny;
+    uint8_t *of = malloc(n);
+    int8_t *ob = malloc(n);
+    memcpy(of, flags, n);
+    memcpy(ob, body_id, n);
+    of_classify(nx, ny, xc, yc, nbody, bodies, flags, body_id);
+    int fresh = 0;
+    for (size_t id = 0; id < n; id++)
+        if (!(of[id] & OF_FLUID) && (flags[id] & OF_FLUID) && ob[id] >= 0) {
+            u[id] = bodies[ob[id]].ub;
+            v[id] = bodies[ob[id]].vb;
+            fresh++;
+        }
+    free(of);
+    free(ob);
+    return fresh;
+}
+
 int of_ghost_list(int nx, int ny, const uint8_t *flags, int32_t *list, int cap)
 {
     int n = 0;

@@ -164,6 +164,15 @@ void of_phys_step(int nx, int ny, const double *dx, const double *dy,
                   const of_params *prm, double *u, double *v, double *uf,
                   double *vf, double *p, of_stats *st);
 
+/* Moving bodies: before a step the caller places every body at t_{n+1}
+ * (position, velocity, acceleration as data).  This reclassifies all cells
+ * (flags, body ids) and gives each fresh cell -- solid before, fluid now --
+ * the velocity of the body that left it.  The caller then rebuilds the
+ * ghost list and image-point records.  Returns the number of fresh cells. */
+int of_phys_move_bodies(int nx, int ny, const double *xc, const double *yc,
+                        int nbody, const of_body *bodies, uint8_t *flags,
+                        int8_t *body_id, double *u, double *v);
+
 /* Pieces of the step, exposed for closed-form tests. */
 void of_phys_rhs(int nx, int ny, const double *dx, const double *dy,
                  const uint8_t *flags, double dt, const double *uf,

@@ -5,6 +5,7 @@ against, never part of the product.  This module builds it on first use when
 the shared library is missing (gcc + OpenMP, a few seconds).
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -88,6 +89,9 @@ def lib():
     phys_sig = [C.c_int, C.c_int, _dp, _dp, _u8p, _i8p, C.c_int, C.POINTER(Body),
                 C.c_int, C.POINTER(Ghost), C.POINTER(Params), _dp, _dp, _dp,
                 _dp, _dp]
+    L.of_phys_move_bodies.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(Body),
+                                      _u8p, _i8p, _dp, _dp]
+    L.of_phys_move_bodies.restype = C.c_int
     L.of_phys_init.argtypes = phys_sig
     L.of_phys_step.argtypes = phys_sig + [C.POINTER(Stats)]
     L.of_phys_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _u8p, C.c_double,
@@ -223,6 +227,22 @@ class PhysCase:
                            self.bodies.arr, self.ng, self.gh, C.byref(self.prm), dptr(self.u),
                            dptr(self.v), dptr(self.uf), dptr(self.vf),
                            dptr(self.p))
+
+    def heave(self, specs, t):
+        """Place heaving bodies at time t (y = y0 + A sin(2 pi f t)), reclassify
+        and rebuild the ghost records.  specs: {body index: (y0, A, f)}."""
+        for k, (y0, A, f) in specs.items():
+            w = 2.0 * math.pi * f
+            b = self.bodies.arr[k]
+            b.yc = y0 + A * math.sin(w * t)
+            b.vb = A * w * math.cos(w * t)
+            b.ayb = -A * w * w * math.sin(w * t)
+        fresh = lib().of_phys_move_bodies(self.nx, self.ny, dptr(self.xc), dptr(self.yc),
+                                          self.bodies.n, self.bodies.arr, u8ptr(self.flags),
+                                          i8ptr(self.bid), dptr(self.u), dptr(self.v))
+        self.glist, self.gh, self.ng = ghost_records(self.xc, self.yc, self.flags, self.bid,
+                                                     self.bodies)
+        return fresh
 
     def step(self, n=1):
         st = Stats()

@@ -148,20 +148,39 @@ def test_physics_fp64_matches_oracle(require_gpu, name):
     gc.close()
 
 
-@pytest.mark.parametrize("ty,m", [(8, 1), (32, 1), (16, 2), (32, 4), (8, 4)])
-def test_sweep_variants_are_identical(require_gpu, ty, m):
-    """Tile height and temporal blocking change the launch shape, never the
-    result: equal to the oracle's red-black SOR sweep for sweep."""
+@pytest.mark.parametrize("kind,ty,m", [(0, 8, 1), (0, 32, 1), (0, 16, 2), (0, 32, 4), (0, 8, 4),
+                                         (1, 16, 1), (1, 32, 2), (1, 64, 2), (1, 128, 4),
+                                         (1, 16, 4), (1, 32, 1)])
+def test_sweep_variants_are_identical(require_gpu, kind, ty, m):
+    """Kernel, tile height and temporal blocking change the launch shape,
+    never the result: equal to the oracle's red-black SOR sweep for sweep
+    (check_every=13 also exercises partial launches at chunk ends)."""
     xf, yf, bodies = CASES["circle-stretched"]
-    _say("sweep variant", ty, m)
-    oc, gc = _pair(xf, yf, bodies, dt=0.01, check_every=12)
-    gc.tune(ty, m, 12)
-    for _ in range(3):
-        o = oc.step(1)[-1]
-        g = gc.step(1)
-        assert g["ppe_sweeps"] == o[2]
-    _compare_fields(oc, gc, 1e-10)
-    gc.close()
+    _say("sweep variant", kind, ty, m)
+    for prec in (iff.IF_FP64, iff.IF_MIXED):
+        oc, gc = _pair(xf, yf, bodies, prec=prec, dt=0.01, check_every=13)
+        ref = None
+        if prec == iff.IF_MIXED:
+            ref = iff.Case(xf, yf, precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8,
+                           ppe_itermax=200000, ppe_tol=1e-9, check_every=13)
+            ref.tune(0, 16, 1, 13)
+        gc.tune(kind, ty, m, 13)
+        for _ in range(3):
+            o = oc.step(1)[-1]
+            g = gc.step(1)
+            if prec == iff.IF_FP64:
+                assert g["ppe_sweeps"] == o[2]
+            else:
+                r = ref.step(1)
+                assert g["ppe_sweeps"] == r["ppe_sweeps"]
+        if prec == iff.IF_FP64:
+            _compare_fields(oc, gc, 1e-10)
+        else:
+            # FP32 sweeps: the kernels agree with each other bit for bit
+            for f in ("u", "v", "p", "uf", "vf"):
+                assert np.array_equal(gc.get(f), ref.get(f)), f
+            ref.close()
+        gc.close()
 
 
 def test_weighted_jacobi_matches_oracle(require_gpu):
@@ -238,3 +257,51 @@ def test_batch_budget_admission(require_gpu):
     assert b.add(xf, xf, 1, dt=0.01, re=100.0) is not None
     assert b.add(xf, xf, 1, dt=0.01, re=100.0) is None   # refused: over budget
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# moving bodies: reclassification, fresh cells and ghost rebuild per step
+# ---------------------------------------------------------------------------
+def test_heaving_cylinder_matches_oracle(require_gpu):
+    import math
+    xf = np.linspace(0, 4, 81)
+    y0, A, f = 2.0, 0.5, 0.2
+    w = 2 * math.pi * f
+    dt = 0.02
+    _say("heaving cylinder")
+    prm = dict(dt=dt, re=100.0, conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6,
+               ppe_solver=1, ppe_omega=1.8, ppe_itermax=100000, ppe_tol=1e-8, check_every=10)
+    # the body's state at t0 = 0
+    oc = O.PhysCase(xf, xf, [dict(kind=O.CIRCLE, xc=1.5, yc=y0, r=0.4, vb=A * w)], prm)
+    gc = iff.Case(xf, xf, bodies=[iff.heaving_circle(1.5, y0, 0.4, A, f)], **prm)
+    _compare_geometry(oc, gc)
+    _compare_fields(oc, gc, 1e-14)
+    for n in range(6):
+        oc.heave({0: (y0, A, f)}, (n + 1) * dt)
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        _compare_geometry(oc, gc)
+        assert g["ppe_sweeps"] == o[2] and g["ad_iters"] == o[0], (n, g, o)
+        _compare_fields(oc, gc, 1e-10)
+    gc.close()
+
+
+def test_batch_moving_and_static_alone_equals_mix(require_gpu):
+    xs = np.linspace(0, 4, 65)
+    kw = dict(dt=0.02, re=150.0, ppe_tol=1e-8, check_every=10, ppe_omega=1.8)
+    cases = [([iff.heaving_circle(1.5, 2.0, 0.4, 0.5, 0.2)], 5),
+             ([iff.circle(2.0, 2.0, 0.5)], 4),
+             ([iff.heaving_circle(2.5, 1.8, 0.3, 0.3, 0.5)], 6)]
+    _say("batch moving + static")
+    mix = iff.Batch(iff.IF_FP64)
+    slots = [mix.add(xs, xs, n, bodies=b, **kw) for b, n in cases]
+    while any(mix.state(s)[0] == 0 for s in slots):
+        mix.step(1)
+    for k, (b, n) in enumerate(cases):
+        alone = iff.Batch(iff.IF_FP64)
+        s = alone.add(xs, xs, n, bodies=b, **kw)
+        alone.step(n)
+        for fld in ("u", "v", "p", "uf", "vf", "flags"):
+            assert np.array_equal(alone.get(s, fld), mix.get(slots[k], fld)), (k, fld)
+        alone.close()
+    mix.close()
