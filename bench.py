@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--check-every", type=int, default=40)
     ap.add_argument("--fp64-tile-y", type=int, default=8)
     ap.add_argument("--omega", type=float, default=1.998)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
 
 
@@ -101,6 +105,72 @@ def cpu_baseline(cfg, gpu_case, st, omega):
                        "Poisson sweeps, %.1f s)" % (out[0], out[2], t1 - t0))
 
 
+def batched_c4(iff, configs, a):
+    """The batched workload (256 seeded uneven cases, 128^2..1024^2) in one
+    resident batch, mixed precision, against the plain FP64 single-case path
+    on a sample of the same cases.  Both are timed over the first
+    --batch-steps steps and expressed per 200-step case."""
+    cases = configs.c4_cases(256)
+    b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
+    b.tune(0, 16, 2, a.check_every)
+    t0 = time.perf_counter()
+    slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
+                   ppe_tol=1e-6, check_every=a.check_every) for c in cases]
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.step(a.batch_steps)
+    t_b = time.perf_counter() - t0
+    info = b.info()
+    divs = [b.state(s)[2]["max_div"] for s in slots]
+    cells = sum((len(c["xf"]) - 1) * (len(c["yf"]) - 1) for c in cases)
+    b.close()
+    per_step = t_b / a.batch_steps
+    cps = len(cases) / (200.0 * per_step)
+    gbs = info["cell_sweeps"] * 13.0 / (info["sweep_ms"] * 1e-3) / 1e9 if info["sweep_ms"] > 0 else None
+    # plain FP64 single-case path on every 32nd case, run one after another
+    tot = 0.0
+    sample = cases[::32]
+    for c in sample:
+        s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
+                     ppe_tol=1e-6, check_every=a.check_every)
+        s.tune(0, 8, 2, a.check_every)
+        t0 = time.perf_counter()
+        s.step(a.batch_steps)
+        tot += time.perf_counter() - t0
+        s.close()
+    single_cps = len(sample) / (200.0 * tot / a.batch_steps)
+    log("batched: %.3f s/step for 256 cases, %.2f cases/s (FP64 single-case %.3f cases/s)"
+        % (per_step, cps, single_cps))
+    return dict(value=round(cps, 3), unit="cases/s", cases=len(cases), steps_timed=a.batch_steps,
+                ms_per_batched_step=round(1e3 * per_step, 2), mcell_steps_per_s=round(cells / per_step / 1e6, 2),
+                sweep_gbs=round(gbs, 1) if gbs else None, max_div=max(divs),
+                fp64_single_case_cases_per_s=round(single_cps, 4),
+                speedup_vs_fp64_single=round(cps / single_cps, 2), setup_s=round(t_setup, 2),
+                sample="all 256 cases resident, first %d of 200 steps timed; FP64 single-case on "
+                       "8 of them (every 32nd), same steps; rates per 200-step case" % a.batch_steps)
+
+
+def serving_c5(iff, configs, a):
+    """A bounded sample of the serving workload: the seeded Poisson-arrival
+    trace (50 req/s) with grids clipped to <= 512^2 and step counts / 20."""
+    from immerseflow_amd import serve
+    trace = []
+    for t, c in configs.c5_trace(a.serve_requests):
+        n = min(len(c["xf"]) - 1, 512)
+        xf = configs.uniform_faces(n, 20.0)
+        c = dict(c, xf=xf, yf=xf.copy(), dt=0.5 * 20.0 / n, steps=max(5, c["steps"] // 20), ppe_tol=1e-6,
+                 check_every=a.check_every)
+        trace.append((t, c))
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune=(0, 16, 2, a.check_every))
+    log("serving:", summ)
+    return dict(value=round(summ["cases_per_s"], 3), unit="cases/s", requests=summ["requests"],
+                completed=summ["completed"], p50_s=round(summ["p50_s"], 4), p99_s=round(summ["p99_s"], 4),
+                mcell_steps_per_s=round(summ["mcell_steps_per_s"], 2), max_div=summ["max_div"],
+                max_resident=summ["max_resident"], budget_bytes=16 << 30,
+                sample="first %d requests of the seeded trace (mean 50 req/s), grids clipped to <= 512^2, "
+                       "steps/20 (5..50), 20%% heaving cylinders" % a.serve_requests)
+
+
 def gpu_omega(cfg):
     """The library's default relaxation: 2/(1 + sin(pi/N)), N the larger cell count."""
     import math
@@ -157,6 +227,9 @@ def main():
         cpu = cpu_baseline(cfg, case, mixed["stats"], a.omega if a.omega > 0 else gpu_omega(cfg))
         log("cpu baseline", cpu)
 
+    batched = None if a.no_batch else batched_c4(iff, configs, a)
+    serving = None if a.no_serve else serving_c5(iff, configs, a)
+
     st = mixed["stats"]
     out = {
         "metric": "Mcell-steps/s + % HBM roofline; batched cases/s and p50/p99 latency at div<=1e-6",
@@ -179,6 +252,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64": fp64,
+        "batched": batched,
+        "serving": serving,
         "step_bytes_per_cell": round(step_bpc, 1),
     }
     case.close()

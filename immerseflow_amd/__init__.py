@@ -326,9 +326,11 @@ class Batch:
         _check(_lib().if_batch_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def info(self):
-        r, b, ms, n = C.c_int(), C.c_size_t(), C.c_double(), C.c_longlong()
-        _check(_lib().if_batch_info(self._h, C.byref(r), C.byref(b), C.byref(ms), C.byref(n)))
-        return {"resident": r.value, "bytes": b.value, "sweep_ms": ms.value, "launches": n.value}
+        r, b, ms, n, cs = C.c_int(), C.c_size_t(), C.c_double(), C.c_longlong(), C.c_double()
+        _check(_lib().if_batch_info(self._h, C.byref(r), C.byref(b), C.byref(ms), C.byref(n),
+                                    C.byref(cs)))
+        return {"resident": r.value, "bytes": b.value, "sweep_ms": ms.value, "launches": n.value,
+                "cell_sweeps": cs.value}
 
 
 # ---------------------------------------------------------------------------

@@ -98,7 +98,7 @@ SIGNATURES = [
     ("if_batch_get_field", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, C.c_size_t]),
     ("if_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("if_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_size_t),
-                                _dp, C.POINTER(C.c_longlong)]),
+                                _dp, C.POINTER(C.c_longlong), _dp]),
     ("if_case_footprint", C.c_int, [C.POINTER(IfCaseDesc), C.c_int, C.POINTER(C.c_size_t)]),
 ]
 

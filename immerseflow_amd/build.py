@@ -63,6 +63,8 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([cc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    _run([cc] + COMMON + HIP_FLAGS + [os.path.join(CSRC, "hbm_copy.hip"), "-o",
+                                     os.path.join(PKG, "hbm_copy")])
     cli_src = os.path.join(CSRC, "cli.cpp")
     if os.path.exists(cli_src):
         _run([cc] + COMMON + [cli_src, "-o", CLI, "-L" + PKG, "-limmerseflow_amd",

@@ -265,6 +265,7 @@ static if_status batch_create(int precision, size_t budget, if_batch **out)
         return fail(IF_ERR_ARG, "if_batch_create: unknown precision");
     auto b = std::make_unique<if_batch>();
     b->eng = std::make_unique<Engine>(precision, budget);
+    b->eng->timing = true;
     if_status rc = b->eng->init();
     if (rc != IF_OK) return rc;
     *out = b.release();
@@ -281,7 +282,6 @@ void if_batch_destroy(if_batch *b) { delete b; }
 if_status if_batch_add(if_batch *b, const if_case_desc *d, int nsteps, int *slot)
 {
     if (!b || !d) return fail(IF_ERR_ARG, "if_batch_add: null argument");
-    if (b->eng->resident() == 0 && d->check_every > 0) b->eng->check_k = d->check_every;
     IFF_GUARD(b->eng->add(*d, nsteps, slot));
 }
 
@@ -309,13 +309,15 @@ if_status if_batch_remove(if_batch *b, int slot)
     IFF_GUARD(b->eng->remove(slot));
 }
 
-if_status if_batch_info(if_batch *b, int *resident, size_t *bytes, double *ms, long long *launches)
+if_status if_batch_info(if_batch *b, int *resident, size_t *bytes, double *ms, long long *launches,
+                        double *cell_sweeps)
 {
     if (!b) return fail(IF_ERR_ARG, "null batch");
     if (resident) *resident = b->eng->resident();
     if (bytes) *bytes = b->eng->in_use();
     if (ms) *ms = b->eng->sweep_ms;
     if (launches) *launches = b->eng->sweep_launches;
+    if (cell_sweeps) *cell_sweeps = b->eng->sweep_cells;
     return IF_OK;
 }
 

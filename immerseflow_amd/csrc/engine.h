@@ -61,6 +61,7 @@ public:
     bool timing = false;
     double sweep_ms = 0.0;
     long long sweep_launches = 0;
+    double sweep_cells = 0.0;  // interior cell-sweeps inside the timed chunks
     int spec_depth = 3;
 
     static size_t footprint(const if_case_desc &d, int prec);

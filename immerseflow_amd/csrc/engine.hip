@@ -740,6 +740,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
                     sweep_launches += (K + m_eff - 1) / m_eff;
                 }
             }
+            for (int id : order_)
+                sweep_cells += (double)std::min(all[id].ppe_sweeps, nfull * K) *
+                               ((double)slots_[id]->NX * slots_[id]->NY);
         }
         for (int id : order_) {
             Slot &s = *slots_[id];

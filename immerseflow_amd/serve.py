@@ -1,0 +1,92 @@
+"""Continuous-batching request loop for the fractional step.
+
+Requests (a grid, bodies, Re, dt and a step count) arrive on a trace clock.
+Between batched steps the loop admits waiting requests into the resident
+batch, first come first served, while their HBM footprint fits the budget.
+Every batched step advances all resident cases by one fractional step: one
+kernel launch per pass covers the tiles of all of them.  A case that reaches
+its step count (or fails) leaves the batch at once and frees its memory.
+Each request records its arrival, admission and completion times.
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import IF_MIXED, Batch
+
+
+def _add(batch, c):
+    return batch.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"],
+                     bodies=c.get("bodies", ()), ppe_tol=c.get("ppe_tol", 1e-6),
+                     ppe_omega=c.get("ppe_omega", 0.0), ad_itermax=c.get("ad_itermax", 25),
+                     check_every=c.get("check_every", 40), t0=c.get("t0", 0.0))
+
+
+def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(0, 16, 2, 40),
+          time_scale=1.0, keep_fields=False, log=None):
+    """Run a request trace [(arrival_s, case_dict), ...] to completion.
+
+    Returns (records, summary).  records: one dict per request with
+    arrival / admitted / finished times (s, trace clock), latency, state,
+    steps, cells and the final max |div|."""
+    batch = Batch(precision, hbm_budget)
+    batch.tune(*tune)
+    order = sorted(range(len(trace)), key=lambda k: trace[k][0])
+    pending = collections.deque()
+    active = {}
+    records = []
+    nxt = 0
+    t0 = time.perf_counter()
+    batched_steps = 0
+    max_resident = 0
+    while nxt < len(order) or pending or active:
+        now = time.perf_counter() - t0
+        while nxt < len(order) and trace[order[nxt]][0] * time_scale <= now:
+            k = order[nxt]
+            pending.append((k, trace[k][0] * time_scale, trace[k][1]))
+            nxt += 1
+        while pending:
+            k, arr, c = pending[0]
+            slot = _add(batch, c)
+            if slot is None:
+                if not active:
+                    raise RuntimeError("request %d does not fit the HBM budget alone" % k)
+                break
+            pending.popleft()
+            active[slot] = (k, arr, time.perf_counter() - t0, c)
+        max_resident = max(max_resident, len(active))
+        if not active:
+            if nxt < len(order):
+                time.sleep(max(0.0, trace[order[nxt]][0] * time_scale - (time.perf_counter() - t0)))
+            continue
+        batch.step(1)
+        batched_steps += 1
+        now = time.perf_counter() - t0
+        for slot in list(active):
+            state, steps_done, st = batch.state(slot)
+            if state != 0:
+                k, arr, adm, c = active.pop(slot)
+                rec = dict(id=k, arrival=arr, admitted=adm, finished=now, latency=now - arr,
+                           queue=adm - arr, state=state, steps=steps_done,
+                           cells=(len(c["xf"]) - 1) * (len(c["yf"]) - 1), max_div=st["max_div"])
+                if keep_fields:
+                    rec["u"] = batch.get(slot, "u")
+                    rec["v"] = batch.get(slot, "v")
+                    rec["p"] = batch.get(slot, "p")
+                records.append(rec)
+                batch.remove(slot)
+                if log:
+                    log("request %d done: %d steps, latency %.3f s" % (k, steps_done, rec["latency"]))
+    wall = time.perf_counter() - t0
+    batch.close()
+    lat = np.array([r["latency"] for r in records])
+    ok = [r for r in records if r["state"] == 1]
+    summary = dict(requests=len(records), completed=len(ok), wall_s=wall,
+                   cases_per_s=len(ok) / wall if wall > 0 else 0.0,
+                   p50_s=float(np.percentile(lat, 50)) if len(lat) else None,
+                   p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
+                   mcell_steps_per_s=sum(r["cells"] * r["steps"] for r in ok) / wall / 1e6,
+                   max_div=max((r["max_div"] for r in ok), default=None),
+                   batched_steps=batched_steps, max_resident=max_resident)
+    return records, summary

@@ -222,8 +222,13 @@ if_status if_batch_get_field(if_batch *b, int slot, int which, double *out,
                              size_t n);
 /* Release a slot (frees its device memory back to the budget). */
 if_status if_batch_remove(if_batch *b, int slot);
+/* Resident cases, device bytes charged to the budget, and the event timing
+ * of Poisson sweep launches over full convergence-check chunks:
+ * milliseconds, launches, and interior cell-sweeps done in them (for the
+ * achieved-bandwidth figure of batched launches). */
 if_status if_batch_info(if_batch *b, int *resident, size_t *bytes_in_use,
-                        double *sweep_ms, long long *sweep_launches);
+                        double *sweep_ms, long long *sweep_launches,
+                        double *cell_sweeps);
 /* As if_case_tune, for every case of the batch. */
 if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int sweeps_per_launch,
                         int check_every);

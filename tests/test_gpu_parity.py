@@ -234,6 +234,7 @@ def test_batch_alone_equals_in_mix(require_gpu):
     _say("batch alone vs mix")
     kw = dict(ppe_tol=1e-8, check_every=10, ppe_omega=1.85)
     mix = iff.Batch(iff.IF_FP64)
+    mix.tune(0, 16, 2, 10)
     slots = [mix.add(xf, yf, n, dt=dt, re=re, bodies=b, **kw)
              for (xf, yf, b, dt, re), n in zip(specs, steps)]
     while any(mix.state(s)[0] == 0 for s in slots):
@@ -242,6 +243,7 @@ def test_batch_alone_equals_in_mix(require_gpu):
         st, done, _ = mix.state(slots[k])
         assert st == 1 and done == n
         alone = iff.Batch(iff.IF_FP64)
+        alone.tune(0, 16, 2, 10)
         s = alone.add(xf, yf, n, dt=dt, re=re, bodies=b, **kw)
         alone.step(n)
         for f in ("u", "v", "p", "uf", "vf"):
