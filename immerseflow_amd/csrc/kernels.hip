@@ -393,12 +393,53 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *GW(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+// SOR update of one cell.  FP64 evaluates the oracle's expression
+// (IEEE division) so results stay bit-identical to it; the FP32 error sweeps
+// of the mixed path multiply by the hardware reciprocal instead (their
+// result is refined in FP64, and every FP32 sweep kernel uses this form).
+template <typename T>
+__device__ __forceinline__ T sor_update(T pc, T r, T sum, T ap, T om)
+{
+    if constexpr (sizeof(T) == 8) {
+        const T gs = (r - sum) / ap;
+        return pc + om * (gs - pc);
+    } else {
+        const T gs = (r - sum) * __builtin_amdgcn_rcpf(ap);
+        return pc + om * (gs - pc);
+    }
+}
+
+// Native (clang ext_vector_type) vectors for 8- and 16-byte accesses.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned char u8x2 __attribute__((ext_vector_type(2)));
+typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
+template <typename T, int V> struct VecT;
+template <> struct VecT<float, 2> { using type = f32x2; };
+template <> struct VecT<float, 4> { using type = f32x4; };
+template <> struct VecT<double, 2> { using type = f64x2; };
+template <int V> struct VecB;
+template <> struct VecB<2> { using type = u8x2; };
+template <> struct VecB<4> { using type = u8x4; };
+
+template <typename VT>
+__device__ __forceinline__ VT gload_vec(const void *p)
+{
+    return *(const __attribute__((address_space(1))) VT *)p;
+}
+template <typename VT>
+__device__ __forceinline__ void gstore_vec(void *p, VT v)
+{
+    *(__attribute__((address_space(1))) VT *)p = v;
+}
+
 // One red-black half-sweep (colour h&1) on the LDS tile, over the region
 // that is still valid after h half-sweeps: the tile plus a ring of
 // R = 2M-1-h cells.  All sizes are compile-time constants.
 template <typename T, int TY, int M, int h>
-__device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M - 2],
-                                              const uint8_t (*sf)[kTileX + 4 * M - 2],
+__device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
+                                              const uint8_t (*sf)[kTileX + 4 * M],
                                               const T *mxp, const T *mxm, const T *myp, const T *mym,
                                               int gpar, T om)
 {
@@ -417,24 +458,22 @@ __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (
         // global parity of (lx, ly) is (gpar + lx + ly) & 1
         lx += ((gpar + lx + ly) ^ colour) & 1;
         if (lx >= H + kTileX + R) continue;
-        const uint8_t f = sf[ly - 1][lx - 1];
+        const uint8_t f = sf[ly][lx];
         if (!(f & F_FLUID)) continue;
-        const T aE = (f & F_OPEN_E) ? mxp[lx - 1] : (T)0;
-        const T aW = (f & F_OPEN_W) ? mxm[lx - 1] : (T)0;
-        const T aN = (f & F_OPEN_N) ? myp[ly - 1] : (T)0;
-        const T aS = (f & F_OPEN_S) ? mym[ly - 1] : (T)0;
+        const T aE = (f & F_OPEN_E) ? mxp[lx] : (T)0;
+        const T aW = (f & F_OPEN_W) ? mxm[lx] : (T)0;
+        const T aN = (f & F_OPEN_N) ? myp[ly] : (T)0;
+        const T aS = (f & F_OPEN_S) ? mym[ly] : (T)0;
         const T ap = -((aE + aW) + (aN + aS));
         if (ap == (T)0) continue;
         const T sum = (aE * sp[ly][lx + 1] + aW * sp[ly][lx - 1]) + (aN * sp[ly + 1][lx] + aS * sp[ly - 1][lx]);
-        const T pc = sp[ly][lx];
-        const T gs = (sr[ly - 1][lx - 1] - sum) / ap;
-        sp[ly][lx] = pc + om * (gs - pc);
+        sp[ly][lx] = sor_update<T>(sp[ly][lx], sr[ly][lx], sum, ap, om);
     }
 }
 
 template <typename T, int TY, int M, int h>
-__device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M - 2],
-                                               const uint8_t (*sf)[kTileX + 4 * M - 2], const T *mxp,
+__device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
+                                               const uint8_t (*sf)[kTileX + 4 * M], const T *mxp,
                                                const T *mxm, const T *myp, const T *mym, int gpar, T om,
                                                int nst)
 {
@@ -450,21 +489,26 @@ __device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T 
 // Red-black SOR, M full sweeps per launch on an LDS tile with a 2M halo
 // (temporal blocking).  Reads the unknown from buffer src, writes dst: the
 // ping-pong keeps tiles from reading halos that other tiles are writing,
-// and the result equals M global red-black sweeps bit for bit.
+// and the result equals M global red-black sweeps.
 // T = double: unknown p, right side rhs.  T = float (mixed precision):
 // unknown e (error), right side r32; the first launch of a chunk starts
-// from e = 0.
+// from e = 0.  Tiles are loaded and stored with aligned vector accesses
+// (the field layout puts interior column 1 on a 256-byte boundary and the
+// halo width is even).
 template <typename T, int TY, int M>
 __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *tiles,
                                                int ntiles, int ntiles_pad, int s, int K)
 {
     constexpr int H = 2 * M;
     constexpr int W = kTileX + 2 * H, HT = TY + 2 * H;
-    constexpr int W1 = kTileX + 2 * (H - 1), HT1 = TY + 2 * (H - 1);
+    constexpr int V = (sizeof(T) == 4 && (H % 4) == 0) ? 4 : 2;  // elements per vector access
+    constexpr int WV = W / V;
+    using VT = typename VecT<T, V>::type;
+    using VB = typename VecB<V>::type;
     __shared__ T sp[HT][W];
-    __shared__ T sr[HT1][W1];
-    __shared__ uint8_t sf[HT1][W1];
-    __shared__ T mxp[W1], mxm[W1], myp[HT1], mym[HT1];
+    __shared__ T sr[HT][W];
+    __shared__ uint8_t sf[HT][W];
+    __shared__ T mxp[W], mxm[W], myp[HT], mym[HT];
 
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
@@ -501,36 +545,52 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     }
 
     const int i0 = 1 + tc.tx * kTileX, j0 = 1 + tc.ty * TY;
-    // load phase: unknown with halo H; rhs, flags and metric with halo H-1
-#pragma unroll 4
-    for (int k = threadIdx.x; k < W * HT; k += 256) {
-        const int lx = k % W, ly = k / W;
-        const int gi = i0 - H + lx, gj = j0 - H + ly;
-        T val = (T)0;
-        if (!zero_src && gi >= 0 && gi < nx && gj >= 0 && gj < ny) val = G(src)[gi + gj * P];
-        sp[ly][lx] = val;
-    }
-#pragma unroll 4
-    for (int k = threadIdx.x; k < W1 * HT1; k += 256) {
-        const int lx = k % W1, ly = k / W1;
-        const int gi = i0 - (H - 1) + lx, gj = j0 - (H - 1) + ly;
-        T rv = (T)0;
-        uint8_t fv = 0;
-        if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= ny - 2) {
-            rv = G(rhs)[gi + gj * P];
-            fv = G(c.flags)[gi + gj * P];
+    // load phase: unknown, right side and flags with halo H (vector loads
+    // where the chunk lies inside the array / the interior)
+#pragma unroll 2
+    for (int k = threadIdx.x; k < HT * WV; k += 256) {
+        const int ly = k / WV, cx = k - ly * WV;
+        const int gj = j0 - H + ly, gi = i0 - H + cx * V;
+        T *pd = &sp[ly][cx * V];
+        T *rd = &sr[ly][cx * V];
+        uint8_t *fd = &sf[ly][cx * V];
+        const bool rowin = gj >= 0 && gj < ny;
+        const bool rowint = gj >= 1 && gj <= ny - 2;
+        const size_t o = (size_t)gj * P + gi;
+        if (!zero_src && rowin && gi >= 0 && gi + V - 1 < nx) {
+            const VT v = gload_vec<VT>(src + o);
+#pragma unroll
+            for (int q = 0; q < V; q++) pd[q] = v[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < V; q++)
+                pd[q] = (!zero_src && rowin && gi + q >= 0 && gi + q < nx) ? G(src)[o + q] : (T)0;
         }
-        sr[ly][lx] = rv;
-        sf[ly][lx] = fv;
+        if (rowint && gi >= 1 && gi + V - 1 <= nx - 2) {
+            const VT v = gload_vec<VT>(rhs + o);
+            const VB fb = gload_vec<VB>(c.flags + o);
+#pragma unroll
+            for (int q = 0; q < V; q++) {
+                rd[q] = v[q];
+                fd[q] = fb[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < V; q++) {
+                const bool in = rowint && gi + q >= 1 && gi + q <= nx - 2;
+                rd[q] = in ? G(rhs)[o + q] : (T)0;
+                fd[q] = in ? G(c.flags)[o + q] : (uint8_t)0;
+            }
+        }
     }
-    if (threadIdx.x < W1) {
-        const int gi = i0 - (H - 1) + threadIdx.x;
+    if (threadIdx.x < W) {
+        const int gi = i0 - H + threadIdx.x;
         const bool in = gi >= 1 && gi <= nx - 2;
         mxp[threadIdx.x] = in ? G(cxp)[gi] : (T)0;
         mxm[threadIdx.x] = in ? G(cxm)[gi] : (T)0;
-    } else if (threadIdx.x >= 128 && threadIdx.x < 128 + HT1) {
+    } else if (threadIdx.x >= 128 && threadIdx.x < 128 + HT) {
         const int q = threadIdx.x - 128;
-        const int gj = j0 - (H - 1) + q;
+        const int gj = j0 - H + q;
         const bool in = gj >= 1 && gj <= ny - 2;
         myp[q] = in ? G(cyp)[gj] : (T)0;
         mym[q] = in ? G(cym)[gj] : (T)0;
@@ -542,11 +602,25 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, om, 2 * meff);
 
     // write the tile (all interior cells: non-fluid values pass through)
+    constexpr int VW = 16 / sizeof(T);  // 16-byte stores; i0 is 256-byte aligned
+    constexpr int RW = kTileX / VW;
+    using VS = typename VecT<T, VW>::type;
 #pragma unroll
-    for (int k = threadIdx.x; k < kTileX * TY; k += 256) {
-        const int lx = k & (kTileX - 1), ly = k / kTileX;
-        const int gi = i0 + lx, gj = j0 + ly;
-        if (gi <= nx - 2 && gj <= ny - 2) GW(dst)[gi + gj * P] = sp[H + ly][H + lx];
+    for (int k = threadIdx.x; k < RW * TY; k += 256) {
+        const int ly = k / RW, cx = k - ly * RW;
+        const int gi = i0 + cx * VW, gj = j0 + ly;
+        if (gj > ny - 2) continue;
+        const T *sv = &sp[H + ly][H + cx * VW];
+        if (gi + VW - 1 <= nx - 2) {
+            VS v;
+#pragma unroll
+            for (int q = 0; q < VW; q++) v[q] = sv[q];
+            gstore_vec<VS>(dst + (size_t)gj * P + gi, v);
+        } else {
+#pragma unroll
+            for (int q = 0; q < VW; q++)
+                if (gi + q <= nx - 2) GW(dst)[(size_t)gj * P + gi + q] = sv[q];
+        }
     }
 }
 

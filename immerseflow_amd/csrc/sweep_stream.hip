@@ -26,6 +26,19 @@ __device__ __forceinline__ int tile_of_block_w(int pad)
     return (b & 7) * (pad >> 3) + (b >> 3);
 }
 
+// SOR update (same expression as the LDS kernel, kernels.hip).
+template <typename T>
+__device__ __forceinline__ T sor_update(T pc, T r, T sum, T ap, T om)
+{
+    if constexpr (sizeof(T) == 8) {
+        const T gs = (r - sum) / ap;
+        return pc + om * (gs - pc);
+    } else {
+        const T gs = (r - sum) * __builtin_amdgcn_rcpf(ap);
+        return pc + om * (gs - pc);
+    }
+}
+
 // DPP wave shifts (GFX9 family): lane l receives lane l-1 (from_west) or
 // lane l+1 (from_east); the edge lane receives 0.
 __device__ __forceinline__ int dpp_w(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
@@ -144,8 +157,7 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
                         const T ap = -((aE + aW) + (aN + aS));
                         if (ap != (T)0) {
                             const T sum = (aE * pE + aW * pW) + (aN * Pr[sn] + aS * Pr[ss]);
-                            const T gs = (Rr[sj] - sum) / ap;
-                            Pr[sj] = pc + om * (gs - pc);
+                            Pr[sj] = sor_update<T>(pc, Rr[sj], sum, ap, om);
                         }
                     }
                 }
