@@ -19,10 +19,10 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libimmerseflow_amd.so")
 CLI = os.path.join(PKG, "immerseflow")
 
-HIP_SOURCES = ["kernels.hip", "sweep_stream.hip", "geometry.hip", "engine.hip", "compat.hip"]
+HIP_SOURCES = ["kernels.hip", "sweep_stream.hip", "sweep_reg.hip", "geometry.hip", "engine.hip", "compat.hip"]
 CPP_SOURCES = ["io.cpp", "abi.cpp"]
 
-COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+COMMON = ["-O3", "-fPIC", "-std=c++20", "-ffp-contract=off",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           "-Wall", "-Wno-unused-function", "-Wno-unused-parameter"]
 HIP_FLAGS = ["-x", "hip", "--offload-arch=gfx950",

@@ -18,6 +18,11 @@ bool stream_variant_ok(int ty, int m);
 int stream_out_cols(int m);
 void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const CaseDev *cs,
                              const int2 *tiles, int nt, int s, int K);
+// register-tile red-black SOR (sweep_reg.hip)
+bool reg_variant_ok(int ty, int m);
+int reg_out_cols(int m);
+void launch_ppe_sweep_reg(hipStream_t st, int prec, int ty, int m, const CaseDev *cs,
+                          const int2 *tiles, int nt, int s, int K);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
                       int nt, int K, int M);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);

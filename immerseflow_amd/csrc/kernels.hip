@@ -16,18 +16,10 @@
 // (and of the FP64 oracle), compiled without FP contraction.
 #include "common.h"
 #include "kernels.h"
+#include "sor.h"
 
 namespace iff {
 
-// ---------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int tile_of_block(int ntiles_pad)
-{
-    const int b = blockIdx.x;
-    const int per = ntiles_pad >> 3;
-    return (b & 7) * per + (b >> 3);
-}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -378,62 +370,6 @@ struct PpeAxis {
     const T *cp, *cm;
 };
 
-// Global-memory accessors: pointers read from the case descriptor are
-// generic; casting them to the global address space makes the compiler
-// emit global_load/store instead of flat accesses (which also count against
-// the LDS wait counter).
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T *G(const T *p)
-{
-    return (const __attribute__((address_space(1))) T *)p;
-}
-template <typename T>
-__device__ __forceinline__ __attribute__((address_space(1))) T *GW(T *p)
-{
-    return (__attribute__((address_space(1))) T *)p;
-}
-
-// SOR update of one cell.  FP64 evaluates the oracle's expression
-// (IEEE division) so results stay bit-identical to it; the FP32 error sweeps
-// of the mixed path multiply by the hardware reciprocal instead (their
-// result is refined in FP64, and every FP32 sweep kernel uses this form).
-template <typename T>
-__device__ __forceinline__ T sor_update(T pc, T r, T sum, T ap, T om)
-{
-    if constexpr (sizeof(T) == 8) {
-        const T gs = (r - sum) / ap;
-        return pc + om * (gs - pc);
-    } else {
-        const T gs = (r - sum) * __builtin_amdgcn_rcpf(ap);
-        return pc + om * (gs - pc);
-    }
-}
-
-// Native (clang ext_vector_type) vectors for 8- and 16-byte accesses.
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef unsigned char u8x2 __attribute__((ext_vector_type(2)));
-typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
-template <typename T, int V> struct VecT;
-template <> struct VecT<float, 2> { using type = f32x2; };
-template <> struct VecT<float, 4> { using type = f32x4; };
-template <> struct VecT<double, 2> { using type = f64x2; };
-template <int V> struct VecB;
-template <> struct VecB<2> { using type = u8x2; };
-template <> struct VecB<4> { using type = u8x4; };
-
-template <typename VT>
-__device__ __forceinline__ VT gload_vec(const void *p)
-{
-    return *(const __attribute__((address_space(1))) VT *)p;
-}
-template <typename VT>
-__device__ __forceinline__ void gstore_vec(void *p, VT v)
-{
-    *(__attribute__((address_space(1))) VT *)p = v;
-}
-
 // One red-black half-sweep (colour h&1) on the LDS tile, over the region
 // that is still valid after h half-sweeps: the tile plus a ring of
 // R = 2M-1-h cells.  All sizes are compile-time constants.
@@ -464,10 +400,10 @@ __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (
         const T aW = (f & F_OPEN_W) ? mxm[lx] : (T)0;
         const T aN = (f & F_OPEN_N) ? myp[ly] : (T)0;
         const T aS = (f & F_OPEN_S) ? mym[ly] : (T)0;
-        const T ap = -((aE + aW) + (aN + aS));
-        if (ap == (T)0) continue;
-        const T sum = (aE * sp[ly][lx + 1] + aW * sp[ly][lx - 1]) + (aN * sp[ly + 1][lx] + aS * sp[ly - 1][lx]);
-        sp[ly][lx] = sor_update<T>(sp[ly][lx], sr[ly][lx], sum, ap, om);
+        const T sa = (aE + aW) + (aN + aS);
+        if (sa == (T)0) continue;
+        sp[ly][lx] = sor_cell<T>(sp[ly][lx], sr[ly][lx], aE, sp[ly][lx + 1], aW, sp[ly][lx - 1], aN,
+                                 sp[ly + 1][lx], aS, sp[ly - 1][lx], sa, om);
     }
 }
 

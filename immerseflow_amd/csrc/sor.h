@@ -1,0 +1,107 @@
+// Pieces shared by the red-black SOR sweep kernels (kernels.hip,
+// sweep_stream.hip, sweep_reg.hip): global-address accessors, vector types
+// and the per-cell update.  Every sweep kernel evaluates the update with
+// sor_cell, so tile shapes and kernels change the launch pattern, never a
+// result bit.
+#pragma once
+#include "common.h"
+
+namespace iff {
+
+// Tile of this workgroup: blocks are dealt round-robin over the 8 XCDs, so
+// block b runs on XCD b & 7; consecutive tiles (neighbours in the grid) are
+// mapped to the same XCD and share its L2.  ntiles_pad is a multiple of 8.
+__device__ __forceinline__ int tile_of_block(int ntiles_pad)
+{
+    const int b = blockIdx.x;
+    const int per = ntiles_pad >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// Global-memory accessors: pointers read from the case descriptor are
+// generic; casting them to the global address space makes the compiler
+// emit global_load/store instead of flat accesses (which also count against
+// the LDS wait counter).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *G(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *GW(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// Native (clang ext_vector_type) vectors for 2-, 8- and 16-byte accesses.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef unsigned char u8x2 __attribute__((ext_vector_type(2)));
+typedef unsigned char u8x4 __attribute__((ext_vector_type(4)));
+template <typename T, int V> struct VecT;
+template <> struct VecT<float, 2> { using type = f32x2; };
+template <> struct VecT<float, 4> { using type = f32x4; };
+template <> struct VecT<double, 2> { using type = f64x2; };
+template <int V> struct VecB;
+template <> struct VecB<2> { using type = u8x2; };
+template <> struct VecB<4> { using type = u8x4; };
+
+template <typename VT>
+__device__ __forceinline__ VT gload_vec(const void *p)
+{
+    return *(const __attribute__((address_space(1))) VT *)p;
+}
+template <typename VT>
+__device__ __forceinline__ void gstore_vec(void *p, VT v)
+{
+    *(__attribute__((address_space(1))) VT *)p = v;
+}
+
+// SOR update of one cell of the 5-point Poisson operator.  aE..aS are the
+// face coefficients (zero on closed faces), sa = (aE + aW) + (aN + aS) =
+// -ap, r the right side, om the relaxation factor.
+// FP64: the oracle's expression and order, no contraction, IEEE division:
+//   sum = (aE pE + aW pW) + (aN pN + aS pS);  gs = (r - sum) / ap;
+//   p' = pc + om (gs - pc)
+// so the FP64 sweeps are bit-identical to it.  FP32 (the error equation
+// of the mixed path, refined in FP64 afterwards): the same update
+// rearranged for fused multiply-adds and the hardware reciprocal,
+//   s = aS pS + (aN pN + (aW pW + (aE pE - r)));  p' = (om / sa) s + (1 - om) pc,
+// evaluated identically by every FP32 sweep kernel.
+template <typename T>
+__device__ __forceinline__ T sor_cell(T pc, T r, T aE, T pE, T aW, T pW, T aN, T pN, T aS, T pS, T sa,
+                                      T om)
+{
+    if constexpr (sizeof(T) == 8) {
+        const T sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
+        const T gs = (r - sum) / (-sa);
+        return pc + om * (gs - pc);
+    } else {
+        const T s = __builtin_fmaf(aS, pS, __builtin_fmaf(aN, pN, __builtin_fmaf(aW, pW, __builtin_fmaf(aE, pE, -r))));
+        return __builtin_fmaf(om * __builtin_amdgcn_rcpf(sa), s, (1.0f - om) * pc);
+    }
+}
+
+// DPP wave shifts (GFX9 family): lane l receives lane l-1 (from_west) or
+// lane l+1 (from_east); the edge lane receives 0 (bound_ctrl), so no
+// old value has to be set up first.
+__device__ __forceinline__ int dpp_w(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_e(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
+
+__device__ __forceinline__ float from_west(float x) { return __int_as_float(dpp_w(__float_as_int(x))); }
+__device__ __forceinline__ float from_east(float x) { return __int_as_float(dpp_e(__float_as_int(x))); }
+__device__ __forceinline__ double from_west(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_w((int)(b & 0xffffffffLL)), hi = dpp_w((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double from_east(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_e((int)(b & 0xffffffffLL)), hi = dpp_e((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+}  // namespace iff

@@ -1,0 +1,328 @@
+// Register-tile red-black SOR for gfx950.
+//
+// A workgroup of four wave64s owns a tile of 128 columns by 4*NRW rows
+// (halo H = 2M on every side included): lane l of every wave holds the
+// column pair (c0+2l, c0+2l+1) -- one red and one black cell of each row --
+// and wave w holds rows w*NRW .. w*NRW+NRW-1, all in registers.  In a
+// half-sweep each lane therefore updates exactly one cell per row, and
+// which of its two cells that is depends only on the row, so the whole
+// wave follows one path with every lane busy:
+//   * east/west neighbours: the lane's other cell, or the adjacent lane's
+//     through a DPP wave shift;
+//   * north/south neighbours: the lane's own registers, except at the
+//     wave's first and last row, whose neighbours come from the adjacent
+//     waves through a small LDS exchange (one barrier per half-sweep).
+// Rows whose cells are all fluid with four open faces (nearly all of them)
+// take a fast path whose coefficients are the metric itself (x: per lane,
+// y: per row); a row holding any other cell evaluates every lane with its
+// face flags.  Both paths compute the same expression (sor.h), so the
+// result does not depend on the path, the tile shape or the kernel.
+//
+// M sweeps per launch on the tile with its 2M halo (temporal blocking):
+// after half-sweep h only the tile plus a ring of 2M-1-h cells is still
+// exact, so after 2M half-sweeps the interior (TW x TY) is, and only it is
+// stored.  Reads come from buffer src and writes go to dst (ping-pong), so
+// tiles never read halos that other tiles are writing.  HBM traffic is one
+// read of the unknown, the right side and the flags, plus the halo, and
+// one write of the unknown per M sweeps.
+#include "common.h"
+#include "kernels.h"
+#include "sor.h"
+
+#include <utility>
+
+namespace iff {
+
+namespace {
+
+constexpr int kRegWaves = 4;
+
+template <typename T, int NRW>
+struct RegTile {
+    T pa[NRW], pb[NRW];  // unknown, slots a (even column) and b (odd column)
+    T ra[NRW], rb[NRW];  // right side
+    T cyp[NRW], cym[NRW], cy[NRW];
+};
+
+// One cell of a half-sweep.  Fast path (FAST): a fluid cell whose faces
+// are all open except those on the domain boundary; the coefficients are
+// the metric with the boundary faces' entries zeroed, so both paths give
+// the same bits for such a cell.  Flag path: the oracle's rule, coefficient
+// zero on a closed face, no update of a non-fluid cell or one with ap = 0.
+template <typename T, bool FAST>
+__device__ __forceinline__ T reg_cell(int f, T pc, T r, T pE, T pW, T pN, T pS, T cxp, T cxm, T cx,
+                                      T cyp, T cym, T cy, T om)
+{
+    if constexpr (FAST) {
+        return sor_cell<T>(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, cx + cy, om);
+    } else {
+        const T aE = (f & F_OPEN_E) ? cxp : (T)0;
+        const T aW = (f & F_OPEN_W) ? cxm : (T)0;
+        const T aN = (f & F_OPEN_N) ? cyp : (T)0;
+        const T aS = (f & F_OPEN_S) ? cym : (T)0;
+        const T sa = (aE + aW) + (aN + aS);
+        const T nv = sor_cell<T>(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sa, om);
+        return ((f & F_FLUID) && sa != (T)0) ? nv : pc;
+    }
+}
+
+template <typename T, int NRW, int Q, bool FAST, int k>
+__device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb, T nva, T nvb, T cxpa,
+                                        T cxma, T cxa, T cxpb, T cxmb, T cxb, T om)
+{
+    constexpr int ks = k > 0 ? k - 1 : 0, kn = k < NRW - 1 ? k + 1 : 0;
+    if constexpr ((k & 1) == Q) {
+        const T pS = k == 0 ? sva : R.pa[ks];
+        const T pN = k == NRW - 1 ? nva : R.pa[kn];
+        const T pE = R.pb[k], pW = from_west(R.pb[k]);
+        R.pa[k] = reg_cell<T, FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, cxa, R.cyp[k],
+                                    R.cym[k], R.cy[k], om);
+    } else {
+        const T pS = k == 0 ? svb : R.pb[ks];
+        const T pN = k == NRW - 1 ? nvb : R.pb[kn];
+        const T pE = from_east(R.pa[k]), pW = R.pa[k];
+        R.pb[k] = reg_cell<T, FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, cxb, R.cyp[k],
+                                    R.cym[k], R.cy[k], om);
+    }
+}
+
+// Half-sweep on the wave's rows.  Q: rows k with (k & 1) == Q update slot
+// a, the others slot b.  sv/nv: the rows just south/north of the wave.
+// ALLFAST: every row of the wave is interior and takes the fast path (no
+// branches); otherwise smask marks the rows that take the flag path and
+// kmask the rows outside the domain interior, which are not updated.
+template <typename T, int NRW, int Q, bool ALLFAST>
+__device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl)[64], int lane,
+                                         unsigned smask, unsigned kmask, T sva, T svb, T nva, T nvb,
+                                         T cxpa, T cxma, T cxa, T cxpb, T cxmb, T cxb, T om)
+{
+    if constexpr (ALLFAST) {
+        [&]<int... K>(std::integer_sequence<int, K...>) {
+            (reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb, cxb, om), ...);
+        }(std::make_integer_sequence<int, NRW>{});
+    } else {
+        int f[NRW];  // flags of every row, read in one batch
+#pragma unroll
+        for (int k = 0; k < NRW; k++) f[k] = fl[k][lane];
+        [&]<int... K>(std::integer_sequence<int, K...>) {
+            (([&] {
+                 if ((kmask >> K) & 1u) return;
+                 if ((smask >> K) & 1u)
+                     reg_row<T, NRW, Q, false, K>(R, f[K], sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb,
+                                                  cxb, om);
+                 else
+                     reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb, cxb,
+                                                 om);
+             }()),
+             ...);
+        }(std::make_integer_sequence<int, NRW>{});
+    }
+}
+
+}  // namespace
+
+template <typename T, int M, int NRW>
+__global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
+                                                   int ntiles_pad, int s, int K)
+{
+    constexpr int NW = kRegWaves;
+    constexpr int H = 2 * M;
+    constexpr int TW = 128 - 2 * H;   // stored columns per tile
+    constexpr int TY = NW * NRW - 2 * H;  // stored rows per tile
+    static_assert(TY > 0 && NRW <= 32, "tile shape");
+    using V2 = typename VecT<T, 2>::type;
+    __shared__ V2 xch[2][NW][2][64];  // [parity][wave][first row, last row][lane]
+    // Face flags (slot a in bits 0-7, slot b in 8-15), read by the rows that
+    // take the flag path; kept in LDS rather than registers.
+    __shared__ uint16_t sfl[NW][NRW][64];
+
+    const int t = tile_of_block(ntiles_pad);
+    if (t >= ntiles) return;  // whole workgroup
+    const int2 tl = tiles[t];
+    const CaseDev &c = cs[tl.x];
+    const Ctrl *ct = c.ctrl;
+    if (!ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    if (s >= kexec) return;
+    int meff = kexec - s;
+    if (meff > M) meff = M;
+    const int nst = 2 * meff;  // half-sweeps of this launch
+    const int L = s / M;       // launch index inside the chunk
+
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const T *src;
+    T *dst;
+    const T *rhs;
+    const T *cxp_, *cxm_, *cyp_, *cym_;
+    bool zero_src = false;
+    T om;
+    if constexpr (sizeof(T) == 8) {
+        src = c.p[(ct->ppe_base + L) & 1];
+        dst = c.p[(ct->ppe_base + L + 1) & 1];
+        rhs = c.rhs;
+        cxp_ = c.x.cp; cxm_ = c.x.cm; cyp_ = c.y.cp; cym_ = c.y.cm;
+        om = c.omega;
+    } else {
+        src = c.e[L & 1];
+        dst = c.e[(L + 1) & 1];
+        zero_src = (L == 0);
+        rhs = c.r32;
+        cxp_ = c.xf.cp; cxm_ = c.xf.cm; cyp_ = c.yf.cp; cym_ = c.yf.cm;
+        om = c.omega_f;
+    }
+    const int ntxs = (nx - 2 + TW - 1) / TW;
+    const int tx = tl.y % ntxs, ty = tl.y / ntxs;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0 = 1 + tx * TW - H;            // first tile column
+    const int jt0 = 1 + ty * TY - H + w * NRW;  // first row of this wave
+    const int ga = c0 + 2 * lane;               // slot a column; slot b is ga + 1
+    const bool ina = ga >= 1 && ga <= nx - 2;
+    const bool inb = ga + 1 >= 1 && ga + 1 <= nx - 2;
+    // x coefficients of the fast path: zero on the domain boundary faces.
+    // A slot outside the interior gets cx = 1, so the fast path computes a
+    // finite value there (never stored, and only ever multiplied by the
+    // zero coefficient of a boundary face).
+    const T cxpa = (ina && ga != nx - 2) ? G(cxp_)[ga] : (T)0;
+    const T cxma = (ina && ga != 1) ? G(cxm_)[ga] : (T)0;
+    const T cxpb = (inb && ga + 1 != nx - 2) ? G(cxp_)[ga + 1] : (T)0;
+    const T cxmb = (inb && ga + 1 != 1) ? G(cxm_)[ga + 1] : (T)0;
+    const T cxa = ina ? cxpa + cxma : (T)1, cxb = inb ? cxpb + cxmb : (T)1;
+    // faces a fluid cell of each slot has open when it is a fast-path cell
+    const int xa = (ga != nx - 2 ? F_OPEN_E : 0) | (ga != 1 ? F_OPEN_W : 0);
+    const int xb = (ga + 1 != nx - 2 ? F_OPEN_E : 0) | (ga + 1 != 1 ? F_OPEN_W : 0);
+    constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
+
+    RegTile<T, NRW> R;
+    unsigned smask = 0, kmask = 0;
+#pragma unroll
+    for (int k = 0; k < NRW; k++) {
+        const int gj = jt0 + k;
+        const bool rowin = gj >= 0 && gj < ny;
+        const bool rowint = gj >= 1 && gj <= ny - 2;
+        T a = (T)0, b = (T)0, x = (T)0, y = (T)0;
+        int f = 0;
+        if (rowin) {
+            const size_t o = (size_t)gj * P + ga;
+            if (!zero_src) {
+                if (ga >= 0 && ga + 1 < nx) {
+                    const V2 v = gload_vec<V2>(src + o);
+                    a = v[0];
+                    b = v[1];
+                } else {
+                    if (ga >= 0 && ga < nx) a = G(src)[o];
+                    if (ga + 1 >= 0 && ga + 1 < nx) b = G(src)[o + 1];
+                }
+            }
+            if (rowint) {
+                if (ina && inb) {
+                    const V2 v = gload_vec<V2>(rhs + o);
+                    const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+                    x = v[0];
+                    y = v[1];
+                    f = (int)fb[0] | ((int)fb[1] << 8);
+                } else {
+                    if (ina) {
+                        x = G(rhs)[o];
+                        f = G(c.flags)[o];
+                    }
+                    if (inb) {
+                        y = G(rhs)[o + 1];
+                        f |= (int)G(c.flags)[o + 1] << 8;
+                    }
+                }
+            }
+        }
+        R.pa[k] = a;
+        R.pb[k] = b;
+        R.ra[k] = x;
+        R.rb[k] = y;
+        sfl[w][k][lane] = (uint16_t)f;
+        const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
+        const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
+        R.cyp[k] = yp;
+        R.cym[k] = ym;
+        R.cy[k] = yp + ym;
+        if (!rowint) {
+            kmask |= 1u << k;
+        } else {
+            const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
+            const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
+            const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
+            const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
+            if (__any(!(fa && fb))) smask |= 1u << k;
+        }
+    }
+
+    const int Z = (c0 + jt0) & 1;  // parity of slot a in the wave's first row
+    for (int h = 0; h < nst; h++) {
+        const int b = h & 1;
+        xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        __syncthreads();
+        V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
+        if (w > 0) sv = xch[b][w - 1][1][lane];
+        if (w < NW - 1) nv = xch[b][w + 1][0][lane];
+        // half-sweep h has colour h & 1: slot a of row k has parity Z + k.
+        // A wave of interior fast-path rows (the common case) runs
+        // branch-free.
+#define IFF_HALF(Q_, F_)                                                                             \
+    reg_half<T, NRW, Q_, F_>(R, sfl[w], lane, smask, kmask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, \
+                             cxpb, cxmb, cxb, om)
+        if ((smask | kmask) == 0) {
+            if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+        } else {
+            if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+        }
+#undef IFF_HALF
+    }
+
+    // store the interior of the tile: rows H .. H+TY-1, columns H .. H+TW-1
+    if (lane >= H / 2 && lane < 64 - H / 2) {
+#pragma unroll
+        for (int k = 0; k < NRW; k++) {
+            const int rr = w * NRW + k;
+            const int gj = jt0 + k;
+            if (rr < H || rr >= H + TY || gj > ny - 2) continue;
+            const size_t o = (size_t)gj * P + ga;
+            if (ga + 1 <= nx - 2)
+                gstore_vec<V2>(dst + o, V2{R.pa[k], R.pb[k]});
+            else if (ga <= nx - 2)
+                GW(dst)[o] = R.pa[k];
+        }
+    }
+}
+
+template <typename T, int M, int NRW>
+static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K)
+{
+    const int pad = ((nt + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NRW>), dim3(pad), dim3(64 * kRegWaves), 0, st, cs, tiles, nt,
+                       pad, s, K);
+}
+
+// Stored tile: 128 - 4M columns by tile_y rows; tile_y + 4M rows per
+// workgroup, split over its four waves.
+bool reg_variant_ok(int ty, int m)
+{
+    return (m == 1 || m == 2 || m == 4) && (ty == 16 || ty == 32 || ty == 48);
+}
+
+int reg_out_cols(int m) { return 128 - 4 * m; }
+
+void launch_ppe_sweep_reg(hipStream_t st, int prec, int ty, int m, const CaseDev *cs, const int2 *tiles,
+                          int nt, int s, int K)
+{
+#define IFF_RG(T, M_, TY_) \
+    if (m == M_ && ty == TY_) { reg_one<T, M_, (TY_ + 4 * M_) / kRegWaves>(st, cs, tiles, nt, s, K); return; }
+#define IFF_RG_ALL(T)                                              \
+    IFF_RG(T, 1, 16) IFF_RG(T, 1, 32) IFF_RG(T, 1, 48)             \
+    IFF_RG(T, 2, 16) IFF_RG(T, 2, 32) IFF_RG(T, 2, 48)             \
+    IFF_RG(T, 4, 16) IFF_RG(T, 4, 32) IFF_RG(T, 4, 48)
+    if (prec == IF_FP64) { IFF_RG_ALL(double) } else { IFF_RG_ALL(float) }
+#undef IFF_RG_ALL
+#undef IFF_RG
+}
+
+}  // namespace iff

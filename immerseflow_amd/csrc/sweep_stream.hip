@@ -15,49 +15,9 @@
 // read of p, rhs and flags (plus the halo) and one write of p per M sweeps.
 #include "common.h"
 #include "kernels.h"
+#include "sor.h"
 
 namespace iff {
-
-// Block index remapped so consecutive blocks share an XCD (blocks are dealt
-// round-robin over the 8 XCDs); pad is a multiple of 8.
-__device__ __forceinline__ int tile_of_block_w(int pad)
-{
-    const int b = blockIdx.x;
-    return (b & 7) * (pad >> 3) + (b >> 3);
-}
-
-// SOR update (same expression as the LDS kernel, kernels.hip).
-template <typename T>
-__device__ __forceinline__ T sor_update(T pc, T r, T sum, T ap, T om)
-{
-    if constexpr (sizeof(T) == 8) {
-        const T gs = (r - sum) / ap;
-        return pc + om * (gs - pc);
-    } else {
-        const T gs = (r - sum) * __builtin_amdgcn_rcpf(ap);
-        return pc + om * (gs - pc);
-    }
-}
-
-// DPP wave shifts (GFX9 family): lane l receives lane l-1 (from_west) or
-// lane l+1 (from_east); the edge lane receives 0.
-__device__ __forceinline__ int dpp_w(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
-__device__ __forceinline__ int dpp_e(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, false); }
-
-__device__ __forceinline__ float from_west(float x) { return __int_as_float(dpp_w(__float_as_int(x))); }
-__device__ __forceinline__ float from_east(float x) { return __int_as_float(dpp_e(__float_as_int(x))); }
-__device__ __forceinline__ double from_west(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = dpp_w((int)(b & 0xffffffffLL)), hi = dpp_w((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double from_east(double x)
-{
-    const long long b = __double_as_longlong(x);
-    const int lo = dpp_e((int)(b & 0xffffffffLL)), hi = dpp_e((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 template <typename T, int M, int TY>
 __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const int2 *tiles,
@@ -67,7 +27,7 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
     constexpr int WO = 64 - 2 * H;  // output columns per wave
     constexpr int L = 2 * M + 2;    // rows alive in the ring
     const int lane = threadIdx.x & 63;
-    const int t = tile_of_block_w(nblk_pad) * 4 + (threadIdx.x >> 6);
+    const int t = tile_of_block(nblk_pad) * 4 + (threadIdx.x >> 6);
     if (t >= ntiles) return;  // whole wave: no barriers in this kernel
     const int2 tl = tiles[t];
     const CaseDev &c = cs[tl.x];
@@ -154,10 +114,9 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
                         const T aW = (f & F_OPEN_W) ? cmx : (T)0;
                         const T aN = (f & F_OPEN_N) ? CY[sj] : (T)0;
                         const T aS = (f & F_OPEN_S) ? MY[sj] : (T)0;
-                        const T ap = -((aE + aW) + (aN + aS));
-                        if (ap != (T)0) {
-                            const T sum = (aE * pE + aW * pW) + (aN * Pr[sn] + aS * Pr[ss]);
-                            Pr[sj] = sor_update<T>(pc, Rr[sj], sum, ap, om);
+                        const T sa = (aE + aW) + (aN + aS);
+                        if (sa != (T)0) {
+                            Pr[sj] = sor_cell<T>(pc, Rr[sj], aE, pE, aW, pW, aN, Pr[sn], aS, Pr[ss], sa, om);
                         }
                     }
                 }

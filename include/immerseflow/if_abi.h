@@ -185,10 +185,12 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
                          size_t cap, size_t *n);
 /* Poisson sweep launch shape.  kernel 0: LDS-tiled red-black SOR, tile
  * height 8/16/32 cells; kernel 1: register-streaming red-black SOR (one
- * wave64 per column strip), strip height 16/32/64/128 rows.  Either does
- * 1, 2 or 4 red-black sweeps per launch (temporal blocking); check_every is
- * the number of sweeps between convergence checks.  Results do not depend
- * on the kernel, the tile height or the sweeps per launch. */
+ * wave64 per column strip), strip height 16/32/64/128 rows; kernel 2:
+ * register-tile red-black SOR (four wave64s hold a 128-column tile in
+ * registers), tile height 16/32/48 rows.  Each does 1, 2 or 4 red-black
+ * sweeps per launch (temporal blocking); check_every is the number of
+ * sweeps between convergence checks.  Results do not depend on the kernel,
+ * the tile height or the sweeps per launch. */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Accumulated event timing of the Poisson sweep kernel (milliseconds,
