@@ -246,9 +246,10 @@ class Case:
         return np.array(lst[:k], dtype=np.int32), recs, k
 
     def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
-        """Poisson sweep kernel (0 LDS-tiled, 1 register-streaming, 2
-        register-tile), tile or strip height, red-black sweeps per launch,
-        sweeps per check.  Every kernel and shape gives the same result."""
+        """Poisson sweep kernel (0 LDS-tiled, 1 register-streaming, 2/3
+        register-tile with 4/8 waves), tile or strip height, red-black
+        sweeps per launch, sweeps per check.  Every kernel and shape gives
+        the same result."""
         _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def sweep_timing(self, reset=False):

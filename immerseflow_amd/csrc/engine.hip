@@ -562,7 +562,7 @@ if_status Engine::refresh()
         cs[k] = s.dev;
         for (int t = 0; t < s.ntx * s.nty; t++) tb.push_back(make_int2(k, t));
         const int ntys = (s.NY + sty - 1) / sty;
-        const int wo = sweep_kind == 1 ? stream_out_cols(sweep_m) : sweep_kind == 2 ? reg_out_cols(sweep_m) : 0;
+        const int wo = sweep_kind == 1 ? stream_out_cols(sweep_m) : sweep_kind >= 2 ? reg_out_cols(sweep_m) : 0;
         const int ntxs = wo > 0 ? (s.NX + wo - 1) / wo : s.ntx;
         for (int t = 0; t < ntxs * ntys; t++) tsw.push_back(make_int2(k, t));
         max_ng_ = std::max(max_ng_, s.gcap);
@@ -701,9 +701,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             const bool tm = timing && (u - 1) < 64;
             if (tm) hipEventRecord(tev_[2 * (u - 1)], st_);
             for (int s = 0; s < K; s += m_eff) {
-                if (sweep_kind == 2 && !jac)
-                    launch_ppe_sweep_reg(st_, prec_, tiles_sweep_ty_, m_eff, d_cases_, d_tiles_sweep_,
-                                         nt_sweep_, s, K);
+                if (sweep_kind >= 2 && !jac)
+                    launch_ppe_sweep_reg(st_, prec_, sweep_kind == 3 ? 8 : 4, tiles_sweep_ty_, m_eff, d_cases_,
+                                         d_tiles_sweep_, nt_sweep_, s, K);
                 else if (sweep_kind == 1 && !jac)
                     launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, m_eff, d_cases_, d_tiles_sweep_,
                                             nt_sweep_, s, K);

@@ -69,6 +69,17 @@ __device__ __forceinline__ void gstore_vec(void *p, VT v)
 // rearranged for fused multiply-adds and the hardware reciprocal,
 //   s = aS pS + (aN pN + (aW pW + (aE pE - r)));  p' = (om / sa) s + (1 - om) pc,
 // evaluated identically by every FP32 sweep kernel.
+// FP32 relaxation weight om / sa of a cell (sa != 0).
+__device__ __forceinline__ float sor_weight(float sa, float om) { return om * __builtin_amdgcn_rcpf(sa); }
+
+// FP32 update with the cell's weight w = sor_weight(sa, om) already formed.
+__device__ __forceinline__ float sor_cell_w(float pc, float r, float aE, float pE, float aW, float pW,
+                                            float aN, float pN, float aS, float pS, float w, float om)
+{
+    const float s = __builtin_fmaf(aS, pS, __builtin_fmaf(aN, pN, __builtin_fmaf(aW, pW, __builtin_fmaf(aE, pE, -r))));
+    return __builtin_fmaf(w, s, (1.0f - om) * pc);
+}
+
 template <typename T>
 __device__ __forceinline__ T sor_cell(T pc, T r, T aE, T pE, T aW, T pW, T aN, T pN, T aS, T pS, T sa,
                                       T om)
@@ -78,8 +89,7 @@ __device__ __forceinline__ T sor_cell(T pc, T r, T aE, T pE, T aW, T pW, T aN, T
         const T gs = (r - sum) / (-sa);
         return pc + om * (gs - pc);
     } else {
-        const T s = __builtin_fmaf(aS, pS, __builtin_fmaf(aN, pN, __builtin_fmaf(aW, pW, __builtin_fmaf(aE, pE, -r))));
-        return __builtin_fmaf(om * __builtin_amdgcn_rcpf(sa), s, (1.0f - om) * pc);
+        return sor_cell_w(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sor_weight(sa, om), om);
     }
 }
 

@@ -1,6 +1,6 @@
 // Register-tile red-black SOR for gfx950.
 //
-// A workgroup of four wave64s owns a tile of 128 columns by 4*NRW rows
+// A workgroup of NW (4 or 8) wave64s owns a tile of 128 columns by NW*NRW rows
 // (halo H = 2M on every side included): lane l of every wave holds the
 // column pair (c0+2l, c0+2l+1) -- one red and one black cell of each row --
 // and wave w holds rows w*NRW .. w*NRW+NRW-1, all in registers.  In a
@@ -35,16 +35,42 @@ namespace iff {
 
 namespace {
 
-constexpr int kRegWaves = 4;
-
 template <typename T, int NRW>
 struct RegTile {
     T pa[NRW], pb[NRW];  // unknown, slots a (even column) and b (odd column)
     T ra[NRW], rb[NRW];  // right side
-    T cyp[NRW], cym[NRW], cy[NRW];
+    T cyp[NRW], cym[NRW];
+    T cy[NRW];           // FP64: cyp + cym of the fast path
+    T wa[NRW], wb[NRW];  // FP32: the cell's weight om/sa, 0 for a cell not updated
 };
 
-// One cell of a half-sweep.  Fast path (FAST): a fluid cell whose faces
+// a if face bit F of flags f is set, else +0: the bit, sign-extended to a
+// full mask, ANDed with a's bits (two VALU operations, no compare).
+template <typename T, int F>
+__device__ __forceinline__ T face_coef(int f, T a)
+{
+    const int m = __builtin_amdgcn_sbfe(f, __builtin_ctz(F), 1);
+    if constexpr (sizeof(T) == 8)
+        return __longlong_as_double(__double_as_longlong(a) & (long long)m);
+    else
+        return __int_as_float(__float_as_int(a) & m);
+}
+
+// One cell of a half-sweep, FP32 (the error equation of the mixed path).
+// Every cell takes this one branch-free form: the error is zero in every
+// non-fluid cell (each solve starts from zero and never updates them), and
+// a face is closed exactly when it lies on the domain boundary or next to a
+// non-fluid cell, so the metric coefficients (zeroed on boundary faces)
+// give the same neighbour sum as the flagged ones.  The closed faces enter
+// through the cell's weight w = om / sa (sa over the open faces), formed
+// once per launch; w = 0 leaves a non-fluid cell at zero.
+__device__ __forceinline__ float reg_cell_w(float pc, float r, float pE, float pW, float pN, float pS,
+                                            float cxp, float cxm, float cyp, float cym, float w, float om)
+{
+    return sor_cell_w(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, w, om);
+}
+
+// One cell of a half-sweep, FP64.  Fast path (FAST): a fluid cell whose faces
 // are all open except those on the domain boundary; the coefficients are
 // the metric with the boundary faces' entries zeroed, so both paths give
 // the same bits for such a cell.  Flag path: the oracle's rule, coefficient
@@ -56,10 +82,10 @@ __device__ __forceinline__ T reg_cell(int f, T pc, T r, T pE, T pW, T pN, T pS, 
     if constexpr (FAST) {
         return sor_cell<T>(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, cx + cy, om);
     } else {
-        const T aE = (f & F_OPEN_E) ? cxp : (T)0;
-        const T aW = (f & F_OPEN_W) ? cxm : (T)0;
-        const T aN = (f & F_OPEN_N) ? cyp : (T)0;
-        const T aS = (f & F_OPEN_S) ? cym : (T)0;
+        const T aE = face_coef<T, F_OPEN_E>(f, cxp);
+        const T aW = face_coef<T, F_OPEN_W>(f, cxm);
+        const T aN = face_coef<T, F_OPEN_N>(f, cyp);
+        const T aS = face_coef<T, F_OPEN_S>(f, cym);
         const T sa = (aE + aW) + (aN + aS);
         const T nv = sor_cell<T>(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sa, om);
         return ((f & F_FLUID) && sa != (T)0) ? nv : pc;
@@ -75,25 +101,30 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
         const T pS = k == 0 ? sva : R.pa[ks];
         const T pN = k == NRW - 1 ? nva : R.pa[kn];
         const T pE = R.pb[k], pW = from_west(R.pb[k]);
-        R.pa[k] = reg_cell<T, FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, cxa, R.cyp[k],
-                                    R.cym[k], R.cy[k], om);
+        if constexpr (sizeof(T) == 4)
+            R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, R.cyp[k], R.cym[k], R.wa[k], om);
+        else
+            R.pa[k] = reg_cell<T, FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, cxa, R.cyp[k],
+                                        R.cym[k], R.cy[k], om);
     } else {
         const T pS = k == 0 ? svb : R.pb[ks];
         const T pN = k == NRW - 1 ? nvb : R.pb[kn];
         const T pE = from_east(R.pa[k]), pW = R.pa[k];
-        R.pb[k] = reg_cell<T, FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, cxb, R.cyp[k],
-                                    R.cym[k], R.cy[k], om);
+        if constexpr (sizeof(T) == 4)
+            R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, R.cyp[k], R.cym[k], R.wb[k], om);
+        else
+            R.pb[k] = reg_cell<T, FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, cxb, R.cyp[k],
+                                        R.cym[k], R.cy[k], om);
     }
 }
 
 // Half-sweep on the wave's rows.  Q: rows k with (k & 1) == Q update slot
 // a, the others slot b.  sv/nv: the rows just south/north of the wave.
-// ALLFAST: every row of the wave is interior and takes the fast path (no
-// branches); otherwise smask marks the rows that take the flag path and
-// kmask the rows outside the domain interior, which are not updated.
+// ALLFAST: every row of the wave takes the fast path (no branches);
+// otherwise smask marks the rows that take the flag path.
 template <typename T, int NRW, int Q, bool ALLFAST>
 __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl)[64], int lane,
-                                         unsigned smask, unsigned kmask, T sva, T svb, T nva, T nvb,
+                                         unsigned smask, T sva, T svb, T nva, T nvb,
                                          T cxpa, T cxma, T cxa, T cxpb, T cxmb, T cxb, T om)
 {
     if constexpr (ALLFAST) {
@@ -106,7 +137,6 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
         for (int k = 0; k < NRW; k++) f[k] = fl[k][lane];
         [&]<int... K>(std::integer_sequence<int, K...>) {
             (([&] {
-                 if ((kmask >> K) & 1u) return;
                  if ((smask >> K) & 1u)
                      reg_row<T, NRW, Q, false, K>(R, f[K], sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb,
                                                   cxb, om);
@@ -121,11 +151,10 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
 
 }  // namespace
 
-template <typename T, int M, int NRW>
-__global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
-                                                   int ntiles_pad, int s, int K)
+template <typename T, int M, int NW, int NRW>
+__global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
+                                                       int ntiles_pad, int s, int K)
 {
-    constexpr int NW = kRegWaves;
     constexpr int H = 2 * M;
     constexpr int TW = 128 - 2 * H;   // stored columns per tile
     constexpr int TY = NW * NRW - 2 * H;  // stored rows per tile
@@ -134,7 +163,8 @@ __global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2
     __shared__ V2 xch[2][NW][2][64];  // [parity][wave][first row, last row][lane]
     // Face flags (slot a in bits 0-7, slot b in 8-15), read by the rows that
     // take the flag path; kept in LDS rather than registers.
-    __shared__ uint16_t sfl[NW][NRW][64];
+    constexpr int kFlw = sizeof(T) == 8 ? NW : 1;  // FP64 only
+    __shared__ uint16_t sfl[kFlw][NRW][64];
 
     const int t = tile_of_block(ntiles_pad);
     if (t >= ntiles) return;  // whole workgroup
@@ -195,7 +225,7 @@ __global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2
     constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
 
     RegTile<T, NRW> R;
-    unsigned smask = 0, kmask = 0;
+    unsigned smask = 0;
 #pragma unroll
     for (int k = 0; k < NRW; k++) {
         const int gj = jt0 + k;
@@ -238,15 +268,26 @@ __global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2
         R.pb[k] = b;
         R.ra[k] = x;
         R.rb[k] = y;
-        sfl[w][k][lane] = (uint16_t)f;
+        if constexpr (sizeof(T) == 8) sfl[w][k][lane] = (uint16_t)f;
         const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
         const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
         R.cyp[k] = yp;
         R.cym[k] = ym;
-        R.cy[k] = yp + ym;
-        if (!rowint) {
-            kmask |= 1u << k;
-        } else {
+        // Rows outside the interior (ghost rows, rows beyond the array)
+        // take the fast path with zero y coefficients and cy = 1 (FP32:
+        // weight 0): their values stay finite, are never stored and reach
+        // interior cells only through the zero coefficients of the
+        // domain's boundary faces.
+        R.cy[k] = rowint ? yp + ym : (T)1;
+        if constexpr (sizeof(T) == 4) {
+            const int fa = f & 0xff, fb = f >> 8;
+            const float sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
+                               (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
+            const float sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
+                               (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
+            R.wa[k] = ((fa & F_FLUID) && sa_a != 0.0f) ? sor_weight(sa_a, om) : 0.0f;
+            R.wb[k] = ((fb & F_FLUID) && sa_b != 0.0f) ? sor_weight(sa_b, om) : 0.0f;
+        } else if (rowint) {
             const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
             const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
             const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
@@ -268,9 +309,9 @@ __global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2
         // A wave of interior fast-path rows (the common case) runs
         // branch-free.
 #define IFF_HALF(Q_, F_)                                                                             \
-    reg_half<T, NRW, Q_, F_>(R, sfl[w], lane, smask, kmask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, \
+    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, \
                              cxpb, cxmb, cxb, om)
-        if ((smask | kmask) == 0) {
+        if (sizeof(T) == 4 || smask == 0) {
             if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
         } else {
             if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
@@ -294,32 +335,39 @@ __global__ __launch_bounds__(256) void k_rbsor_reg(const CaseDev *cs, const int2
     }
 }
 
-template <typename T, int M, int NRW>
+template <typename T, int M, int NW, int NRW>
 static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_rbsor_reg<T, M, NRW>), dim3(pad), dim3(64 * kRegWaves), 0, st, cs, tiles, nt,
-                       pad, s, K);
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt, pad, s,
+                       K);
 }
 
 // Stored tile: 128 - 4M columns by tile_y rows; tile_y + 4M rows per
-// workgroup, split over its four waves.
-bool reg_variant_ok(int ty, int m)
+// workgroup, split over its waves (4 or 8).
+bool reg_variant_ok(int waves, int ty, int m)
 {
-    return (m == 1 || m == 2 || m == 4) && (ty == 16 || ty == 32 || ty == 48);
+    if (waves == 4) return (m == 1 || m == 2 || m == 4) && (ty == 16 || ty == 32 || ty == 48);
+    if (waves == 8) return (m == 2 && (ty == 24 || ty == 40 || ty == 56)) || (m == 4 && (ty == 32 || ty == 48 || ty == 64));
+    return false;
 }
 
 int reg_out_cols(int m) { return 128 - 4 * m; }
 
-void launch_ppe_sweep_reg(hipStream_t st, int prec, int ty, int m, const CaseDev *cs, const int2 *tiles,
-                          int nt, int s, int K)
+void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
+                          const int2 *tiles, int nt, int s, int K)
 {
-#define IFF_RG(T, M_, TY_) \
-    if (m == M_ && ty == TY_) { reg_one<T, M_, (TY_ + 4 * M_) / kRegWaves>(st, cs, tiles, nt, s, K); return; }
-#define IFF_RG_ALL(T)                                              \
-    IFF_RG(T, 1, 16) IFF_RG(T, 1, 32) IFF_RG(T, 1, 48)             \
-    IFF_RG(T, 2, 16) IFF_RG(T, 2, 32) IFF_RG(T, 2, 48)             \
-    IFF_RG(T, 4, 16) IFF_RG(T, 4, 32) IFF_RG(T, 4, 48)
+#define IFF_RG(T, W_, M_, TY_)                                                            \
+    if (waves == W_ && m == M_ && ty == TY_) {                                            \
+        reg_one<T, M_, W_, (TY_ + 4 * M_) / W_>(st, cs, tiles, nt, s, K);                \
+        return;                                                                           \
+    }
+#define IFF_RG_ALL(T)                                                                     \
+    IFF_RG(T, 4, 1, 16) IFF_RG(T, 4, 1, 32) IFF_RG(T, 4, 1, 48)                           \
+    IFF_RG(T, 4, 2, 16) IFF_RG(T, 4, 2, 32) IFF_RG(T, 4, 2, 48)                           \
+    IFF_RG(T, 4, 4, 16) IFF_RG(T, 4, 4, 32) IFF_RG(T, 4, 4, 48)                           \
+    IFF_RG(T, 8, 2, 24) IFF_RG(T, 8, 2, 40) IFF_RG(T, 8, 2, 56)                           \
+    IFF_RG(T, 8, 4, 32) IFF_RG(T, 8, 4, 48) IFF_RG(T, 8, 4, 64)
     if (prec == IF_FP64) { IFF_RG_ALL(double) } else { IFF_RG_ALL(float) }
 #undef IFF_RG_ALL
 #undef IFF_RG

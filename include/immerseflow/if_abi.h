@@ -187,7 +187,9 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * height 8/16/32 cells; kernel 1: register-streaming red-black SOR (one
  * wave64 per column strip), strip height 16/32/64/128 rows; kernel 2:
  * register-tile red-black SOR (four wave64s hold a 128-column tile in
- * registers), tile height 16/32/48 rows.  Each does 1, 2 or 4 red-black
+ * registers), tile height 16/32/48 rows; kernel 3: the same with eight
+ * wave64s per tile, tile height 24/40/56 (2 sweeps per launch) or 32/48/64
+ * (4 sweeps per launch).  Each does 1, 2 or 4 red-black
  * sweeps per launch (temporal blocking); check_every is the number of
  * sweeps between convergence checks.  Results do not depend on the kernel,
  * the tile height or the sweeps per launch. */

@@ -152,7 +152,7 @@ def test_physics_fp64_matches_oracle(require_gpu, name):
                                          (1, 16, 1), (1, 32, 2), (1, 64, 2), (1, 128, 4),
                                          (1, 16, 4), (1, 32, 1),
                                          (2, 16, 1), (2, 32, 2), (2, 16, 2), (2, 48, 2), (2, 32, 4),
-                                         (2, 48, 4)])
+                                         (2, 48, 4), (3, 24, 2), (3, 48, 4)])
 def test_sweep_variants_are_identical(require_gpu, kind, ty, m):
     """Kernel, tile height and temporal blocking change the launch shape,
     never the result: equal to the oracle's red-black SOR sweep for sweep
@@ -185,8 +185,8 @@ def test_sweep_variants_are_identical(require_gpu, kind, ty, m):
         gc.close()
 
 
-@pytest.mark.parametrize("ty,m", [(16, 2), (32, 4), (48, 1)])
-def test_register_tile_across_tiles_and_edges(require_gpu, ty, m):
+@pytest.mark.parametrize("kind,ty,m", [(2, 16, 2), (2, 32, 4), (2, 48, 1), (3, 40, 2), (3, 64, 4)])
+def test_register_tile_across_tiles_and_edges(require_gpu, kind, ty, m):
     """Register-tile sweep on a grid several tiles wide: bodies cut by tile
     boundaries and lying next to the domain edge, partial tiles at the east
     and north edges; FP64 equal to the oracle, mixed equal to the LDS kernel."""
@@ -194,10 +194,10 @@ def test_register_tile_across_tiles_and_edges(require_gpu, ty, m):
     yf = stretched_faces(140, 2.8, 0.997)
     bodies = [iff.ellipse(2.4, 1.3, 0.45, 0.1, np.radians(20.0)),
               iff.circle(5.72, 0.35, 0.2)]
-    _say("register tile edges", ty, m)
+    _say("register tile edges", kind, ty, m)
     for prec in (iff.IF_FP64, iff.IF_MIXED):
         oc, gc = _pair(xf, yf, bodies, prec=prec, dt=0.005, check_every=9, ppe_tol=1e-8)
-        gc.tune(2, ty, m, 9)
+        gc.tune(kind, ty, m, 9)
         ref = None
         if prec == iff.IF_MIXED:
             ref = iff.Case(xf, yf, precision=prec, bodies=bodies, dt=0.005, re=100.0, ppe_omega=1.8,
