@@ -35,17 +35,28 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--kernel", type=int, default=0, help="0 LDS-tiled, 1 register-streaming")
-    ap.add_argument("--tile-y", type=int, default=16)
-    ap.add_argument("--sweeps-per-launch", type=int, default=2)
+    ap.add_argument("--kernel", type=int, default=3,
+                    help="0 LDS-tiled, 1 register-streaming, 2/3 register-tile (4/8 waves)")
+    ap.add_argument("--tile-y", type=int, default=32)
+    ap.add_argument("--sweeps-per-launch", type=int, default=4)
     ap.add_argument("--check-every", type=int, default=40)
-    ap.add_argument("--fp64-tile-y", type=int, default=8)
+    ap.add_argument("--fp64-tile-y", type=int, default=32)
     ap.add_argument("--omega", type=float, default=1.998)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
+
+
+def kernel_name(kind, ty, m, t):
+    """The Poisson sweep kernel a tune() choice launches (as rocprofv3 names it)."""
+    if kind == 0:
+        return "k_rbsor<%s, %d, %d>" % (t, ty, m)
+    if kind == 1:
+        return "k_rbsor_stream<%s, %d, %d>" % (t, m, ty)
+    nw = 4 if kind == 2 else 8
+    return "k_rbsor_reg<%s, %d, %d, %d>" % (t, m, nw, (ty + 4 * m) // nw)
 
 
 def log(*a):
@@ -112,7 +123,7 @@ def batched_c4(iff, configs, a):
     --batch-steps steps and expressed per 200-step case."""
     cases = configs.c4_cases(256)
     b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
-    b.tune(0, 16, 2, a.check_every)
+    b.tune(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
     t0 = time.perf_counter()
     slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
                    ppe_tol=1e-6, check_every=a.check_every) for c in cases]
@@ -133,7 +144,6 @@ def batched_c4(iff, configs, a):
     for c in sample:
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-6, check_every=a.check_every)
-        s.tune(0, 8, 2, a.check_every)
         t0 = time.perf_counter()
         s.step(a.batch_steps)
         tot += time.perf_counter() - t0
@@ -161,7 +171,8 @@ def serving_c5(iff, configs, a):
         c = dict(c, xf=xf, yf=xf.copy(), dt=0.5 * 20.0 / n, steps=max(5, c["steps"] // 20), ppe_tol=1e-6,
                  check_every=a.check_every)
         trace.append((t, c))
-    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune=(0, 16, 2, a.check_every))
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30,
+                               tune=(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every))
     log("serving:", summ)
     return dict(value=round(summ["cases_per_s"], 3), unit="cases/s", requests=summ["requests"],
                 completed=summ["completed"], p50_s=round(summ["p50_s"], 4), p99_s=round(summ["p99_s"], 4),
@@ -199,14 +210,16 @@ def main():
     avg_ms = mixed["sweep_ms"] / max(1, mixed["sweep_launches"])
     achieved = sweep_bpc * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_rbsor_c2.json")
+    tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c2.json")
+    kname = kernel_name(a.kernel, a.tile_y, a.sweeps_per_launch, "float")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("mixed_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("kernel") == kname:  # measured for this launch shape only
+            traffic = tj.get("mixed_bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel=("k_rbsor<float,%d,%d>" if a.kernel == 0 else "k_rbsor_stream<float,%d,%d>")
-                % ((a.tile_y, a.sweeps_per_launch) if a.kernel == 0 else (a.sweeps_per_launch, a.tile_y)),
+                kernel=kname,
                 bytes_per_launch=sweep_bpc * cells, mean_launch_us=round(avg_ms * 1e3, 3),
                 launches_timed=mixed["sweep_launches"])
 

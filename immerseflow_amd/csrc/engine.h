@@ -56,8 +56,8 @@ public:
     Slot *slot(int k) { return (k >= 0 && k < (int)slots_.size()) ? slots_[k].get() : nullptr; }
 
     // Poisson sweep variant and check cadence (autotuned or user-set)
-    int sweep_kind = 0;  // 0: LDS-tiled, 1: register-streaming, 2/3: register-tile (4/8 waves)
-    int sweep_ty = 16, sweep_m = 2, check_k = 10;
+    int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3: register-tile (4/8 waves)
+    int sweep_ty = 32, sweep_m = 4, check_k = 10;
     bool timing = false;
     double sweep_ms = 0.0;
     long long sweep_launches = 0;

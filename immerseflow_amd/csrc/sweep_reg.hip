@@ -225,74 +225,116 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
 
     RegTile<T, NRW> R;
-    unsigned smask = 0;
+    int lf[NRW];  // flags: slot a in bits 0-7, slot b in bits 8-15
+    // Loads.  A tile lying inside the interior (the common case) loads
+    // every row with aligned 8-byte vectors and no per-lane tests.
+    const bool tile_in = c0 >= 1 && c0 + 127 <= nx - 2 && jt0 >= 1 && jt0 + NRW - 1 <= ny - 2;
+    if (tile_in) {
+#pragma unroll
+        for (int k = 0; k < NRW; k++) {
+            const size_t o = (size_t)(jt0 + k) * P + ga;
+            const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + o);
+            const V2 rv = gload_vec<V2>(rhs + o);
+            const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+            R.pa[k] = pv[0];
+            R.pb[k] = pv[1];
+            R.ra[k] = rv[0];
+            R.rb[k] = rv[1];
+            lf[k] = (int)fb[0] | ((int)fb[1] << 8);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NRW; k++) {
+            const int gj = jt0 + k;
+            const bool rowin = gj >= 0 && gj < ny;
+            const bool rowint = gj >= 1 && gj <= ny - 2;
+            T a = (T)0, b = (T)0, x = (T)0, y = (T)0;
+            int f = 0;
+            if (rowin) {
+                const size_t o = (size_t)gj * P + ga;
+                if (!zero_src) {
+                    if (ga >= 0 && ga + 1 < nx) {
+                        const V2 v = gload_vec<V2>(src + o);
+                        a = v[0];
+                        b = v[1];
+                    } else {
+                        if (ga >= 0 && ga < nx) a = G(src)[o];
+                        if (ga + 1 >= 0 && ga + 1 < nx) b = G(src)[o + 1];
+                    }
+                }
+                if (rowint) {
+                    if (ina && inb) {
+                        const V2 v = gload_vec<V2>(rhs + o);
+                        const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+                        x = v[0];
+                        y = v[1];
+                        f = (int)fb[0] | ((int)fb[1] << 8);
+                    } else {
+                        if (ina) {
+                            x = G(rhs)[o];
+                            f = G(c.flags)[o];
+                        }
+                        if (inb) {
+                            y = G(rhs)[o + 1];
+                            f |= (int)G(c.flags)[o + 1] << 8;
+                        }
+                    }
+                }
+            }
+            R.pa[k] = a;
+            R.pb[k] = b;
+            R.ra[k] = x;
+            R.rb[k] = y;
+            lf[k] = f;
+        }
+    }
+
+    // Per-row coefficients, fast-path test and (FP32) per-cell weights.
+    // Rows outside the interior (ghost rows, rows beyond the array) take
+    // the fast path with zero y coefficients and cy = 1 (FP32: weight 0):
+    // their values stay finite, are never stored and reach interior cells
+    // only through the zero coefficients of the domain's boundary faces.
+    unsigned smask = 0;  // FP64: rows holding a cell off the fast path
 #pragma unroll
     for (int k = 0; k < NRW; k++) {
         const int gj = jt0 + k;
-        const bool rowin = gj >= 0 && gj < ny;
         const bool rowint = gj >= 1 && gj <= ny - 2;
-        T a = (T)0, b = (T)0, x = (T)0, y = (T)0;
-        int f = 0;
-        if (rowin) {
-            const size_t o = (size_t)gj * P + ga;
-            if (!zero_src) {
-                if (ga >= 0 && ga + 1 < nx) {
-                    const V2 v = gload_vec<V2>(src + o);
-                    a = v[0];
-                    b = v[1];
-                } else {
-                    if (ga >= 0 && ga < nx) a = G(src)[o];
-                    if (ga + 1 >= 0 && ga + 1 < nx) b = G(src)[o + 1];
-                }
-            }
-            if (rowint) {
-                if (ina && inb) {
-                    const V2 v = gload_vec<V2>(rhs + o);
-                    const u8x2 fb = gload_vec<u8x2>(c.flags + o);
-                    x = v[0];
-                    y = v[1];
-                    f = (int)fb[0] | ((int)fb[1] << 8);
-                } else {
-                    if (ina) {
-                        x = G(rhs)[o];
-                        f = G(c.flags)[o];
-                    }
-                    if (inb) {
-                        y = G(rhs)[o + 1];
-                        f |= (int)G(c.flags)[o + 1] << 8;
-                    }
-                }
-            }
-        }
-        R.pa[k] = a;
-        R.pb[k] = b;
-        R.ra[k] = x;
-        R.rb[k] = y;
+        const int f = lf[k];
         if constexpr (sizeof(T) == 8) sfl[w][k][lane] = (uint16_t)f;
         const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
         const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
         R.cyp[k] = yp;
         R.cym[k] = ym;
-        // Rows outside the interior (ghost rows, rows beyond the array)
-        // take the fast path with zero y coefficients and cy = 1 (FP32:
-        // weight 0): their values stay finite, are never stored and reach
-        // interior cells only through the zero coefficients of the
-        // domain's boundary faces.
         R.cy[k] = rowint ? yp + ym : (T)1;
-        if constexpr (sizeof(T) == 4) {
-            const int fa = f & 0xff, fb = f >> 8;
-            const float sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
-                               (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
-            const float sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
-                               (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
-            R.wa[k] = ((fa & F_FLUID) && sa_a != 0.0f) ? sor_weight(sa_a, om) : 0.0f;
-            R.wb[k] = ((fb & F_FLUID) && sa_b != 0.0f) ? sor_weight(sa_b, om) : 0.0f;
-        } else if (rowint) {
+        bool offfast = false;
+        if (rowint) {
             const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
             const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
             const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
             const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
-            if (__any(!(fa && fb))) smask |= 1u << k;
+            offfast = __any(!(fa && fb));
+        }
+        if constexpr (sizeof(T) == 4) {
+            if (!rowint) {
+                R.wa[k] = 0.0f;
+                R.wb[k] = 0.0f;
+            } else if (!offfast) {
+                // fast-path cells: sa = (cxp + cxm) + (cyp + cym)
+                R.wa[k] = sor_weight(cxa + R.cy[k], om);
+                R.wb[k] = sor_weight(cxb + R.cy[k], om);
+            } else {
+                // sa over the open faces; the same bits as above for a
+                // fast-path cell
+                const int fa = f & 0xff, fb = f >> 8;
+                const float sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
+                                   (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
+                const float sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
+                                   (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
+                R.wa[k] = ((fa & F_FLUID) && sa_a != 0.0f) ? sor_weight(sa_a, om) : 0.0f;
+                R.wb[k] = ((fb & F_FLUID) && sa_b != 0.0f) ? sor_weight(sa_b, om) : 0.0f;
+            }
+        } else {
+            if (offfast) smask |= 1u << k;
         }
     }
 
