@@ -39,7 +39,7 @@ def parse():
                     help="0 LDS-tiled, 1 register-streaming, 2/3 register-tile (4/8 waves)")
     ap.add_argument("--tile-y", type=int, default=32)
     ap.add_argument("--sweeps-per-launch", type=int, default=4)
-    ap.add_argument("--check-every", type=int, default=40)
+    ap.add_argument("--check-every", type=int, default=160)
     ap.add_argument("--fp64-tile-y", type=int, default=32)
     ap.add_argument("--omega", type=float, default=1.998)
     ap.add_argument("--no-batch", action="store_true")

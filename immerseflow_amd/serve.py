@@ -20,10 +20,10 @@ def _add(batch, c):
     return batch.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"],
                      bodies=c.get("bodies", ()), ppe_tol=c.get("ppe_tol", 1e-6),
                      ppe_omega=c.get("ppe_omega", 0.0), ad_itermax=c.get("ad_itermax", 25),
-                     check_every=c.get("check_every", 40), t0=c.get("t0", 0.0))
+                     check_every=c.get("check_every", 160), t0=c.get("t0", 0.0))
 
 
-def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(0, 16, 2, 40),
+def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(3, 32, 4, 160),
           time_scale=1.0, keep_fields=False, log=None):
     """Run a request trace [(arrival_s, case_dict), ...] to completion.
 
