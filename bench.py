@@ -119,8 +119,10 @@ def cpu_baseline(cfg, gpu_case, st, omega):
 def batched_c4(iff, configs, a):
     """The batched workload (256 seeded uneven cases, 128^2..1024^2) in one
     resident batch, mixed precision, against the plain FP64 single-case path
-    on a sample of the same cases.  Both are timed over the first
-    --batch-steps steps and expressed per 200-step case."""
+    (library defaults, one case at a time) on a sample of the same cases.
+    Step 1 runs untimed on both; the batch is then autotuned (launch shape
+    and check interval, timed separately) and steps 2 .. 1+--batch-steps are
+    timed on both, expressed per 200-step case."""
     cases = configs.c4_cases(256)
     b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
     b.tune(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
@@ -128,6 +130,12 @@ def batched_c4(iff, configs, a):
     slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
                    ppe_tol=1e-6, check_every=a.check_every) for c in cases]
     t_setup = time.perf_counter() - t0
+    b.step(1)
+    t0 = time.perf_counter()
+    tr = b.autotune()
+    t_tune = time.perf_counter() - t0
+    log("batch autotune:", tr, "%.3f s" % t_tune)
+    i0 = b.info()
     t0 = time.perf_counter()
     b.step(a.batch_steps)
     t_b = time.perf_counter() - t0
@@ -137,13 +145,15 @@ def batched_c4(iff, configs, a):
     b.close()
     per_step = t_b / a.batch_steps
     cps = len(cases) / (200.0 * per_step)
-    gbs = info["cell_sweeps"] * 13.0 / (info["sweep_ms"] * 1e-3) / 1e9 if info["sweep_ms"] > 0 else None
+    ms = info["sweep_ms"] - i0["sweep_ms"]
+    gbs = (info["cell_sweeps"] - i0["cell_sweeps"]) * 13.0 / (ms * 1e-3) / 1e9 if ms > 0 else None
     # plain FP64 single-case path on every 32nd case, run one after another
     tot = 0.0
     sample = cases[::32]
     for c in sample:
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-6, check_every=a.check_every)
+        s.step(1)
         t0 = time.perf_counter()
         s.step(a.batch_steps)
         tot += time.perf_counter() - t0
@@ -156,8 +166,13 @@ def batched_c4(iff, configs, a):
                 sweep_gbs=round(gbs, 1) if gbs else None, max_div=max(divs),
                 fp64_single_case_cases_per_s=round(single_cps, 4),
                 speedup_vs_fp64_single=round(cps / single_cps, 2), setup_s=round(t_setup, 2),
-                sample="all 256 cases resident, first %d of 200 steps timed; FP64 single-case on "
-                       "8 of them (every 32nd), same steps; rates per 200-step case" % a.batch_steps)
+                autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
+                              check_every=tr["check_every"], sweep_ns_per_cell=round(tr["sweep_ns"], 4),
+                              check_in_sweeps=round(tr["check_sweeps"], 2), typical_sweeps=tr["typical_sweeps"],
+                              candidates=tr["candidates"], seconds=round(t_tune, 3)),
+                sample="all 256 cases resident; step 1 untimed, then autotune, then steps 2..%d of 200 timed; "
+                       "FP64 single-case (library defaults) on 8 of them (every 32nd), same steps; rates per "
+                       "200-step case" % (1 + a.batch_steps))
 
 
 def serving_c5(iff, configs, a):
@@ -171,8 +186,7 @@ def serving_c5(iff, configs, a):
         c = dict(c, xf=xf, yf=xf.copy(), dt=0.5 * 20.0 / n, steps=max(5, c["steps"] // 20), ppe_tol=1e-6,
                  check_every=a.check_every)
         trace.append((t, c))
-    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30,
-                               tune=(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every))
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune="auto")
     log("serving:", summ)
     return dict(value=round(summ["cases_per_s"], 3), unit="cases/s", requests=summ["requests"],
                 completed=summ["completed"], p50_s=round(summ["p50_s"], 4), p99_s=round(summ["p99_s"], 4),

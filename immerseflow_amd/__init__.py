@@ -164,6 +164,10 @@ def make_desc(xf, yf, *, mode=IF_MODE_PHYSICS, precision=IF_FP64, dt, re, conv_f
     return d, (xf, yf, ba)
 
 
+def _tune_dict(r):
+    return {k: getattr(r, k) for k, _ in r._fields_}
+
+
 def _stats_dict(st):
     return {k: getattr(st, k) for k, _ in _abi.IfStepStats._fields_}
 
@@ -252,6 +256,14 @@ class Case:
         the same result."""
         _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
+    def autotune(self):
+        """Time every Poisson sweep launch shape on the current fields, keep
+        the fastest and choose the check interval (see if_case_autotune).
+        Returns the choice as a dict; the case's state is unchanged."""
+        r = _abi.IfTuneResult()
+        _check(_lib().if_case_autotune(self._h, C.byref(r)), "autotune")
+        return _tune_dict(r)
+
     def sweep_timing(self, reset=False):
         ms, n = C.c_double(), C.c_longlong()
         _check(_lib().if_case_sweep_timing(self._h, 1 if reset else 0, C.byref(ms), C.byref(n)))
@@ -326,6 +338,12 @@ class Batch:
 
     def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
         _check(_lib().if_batch_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
+
+    def autotune(self):
+        """As Case.autotune, over every running case of the batch."""
+        r = _abi.IfTuneResult()
+        _check(_lib().if_batch_autotune(self._h, C.byref(r)), "autotune")
+        return _tune_dict(r)
 
     def info(self):
         r, b, ms, n, cs = C.c_int(), C.c_size_t(), C.c_double(), C.c_longlong(), C.c_double()

@@ -65,6 +65,12 @@ class IfGhost(C.Structure):
                 ("biy", C.c_double), ("ipx", C.c_double), ("ipy", C.c_double)]
 
 
+class IfTuneResult(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("tile_y", C.c_int), ("sweeps_per_launch", C.c_int),
+                ("check_every", C.c_int), ("sweep_ns", C.c_double), ("check_sweeps", C.c_double),
+                ("typical_sweeps", C.c_int), ("candidates", C.c_int)]
+
+
 # (name, restype, argtypes) of every declared entry point
 SIGNATURES = [
     ("if_last_error", C.c_char_p, []),
@@ -87,6 +93,8 @@ SIGNATURES = [
                                  C.c_size_t, C.POINTER(C.c_size_t)]),
     ("if_case_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("if_batch_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("if_case_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("if_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
     ("if_case_sweep_timing", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     ("if_case_byte_model", C.c_int, [C.c_void_p, _dp, _dp]),
     ("if_batch_create", C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),

@@ -213,6 +213,29 @@ if_status if_case_tune(if_case *c, int kernel, int tile_y, int m, int check_ever
     return tune(*c->eng, kernel, tile_y, m, check_every);
 }
 
+static if_status case_autotune(if_case *c, if_tune_result *out)
+{
+    // a single case is resident but idle between calls: open it for one
+    // step while timing, then close it again (its state is otherwise untouched)
+    if_status rc = c->eng->set_target(c->slot, 1);
+    if (rc != IF_OK) return rc;
+    rc = c->eng->autotune(out);
+    const if_status rc2 = c->eng->set_target(c->slot, 0);
+    return rc != IF_OK ? rc : rc2;
+}
+
+if_status if_case_autotune(if_case *c, if_tune_result *out)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "autotune: physics mode only");
+    IFF_GUARD(case_autotune(c, out));
+}
+
+if_status if_batch_autotune(if_batch *b, if_tune_result *out)
+{
+    if (!b) return fail(IF_ERR_ARG, "null batch");
+    IFF_GUARD(b->eng->autotune(out));
+}
+
 if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int m, int check_every)
 {
     if (!b) return fail(IF_ERR_ARG, "null batch");

@@ -48,6 +48,7 @@ public:
     if_status set_field(int slot, int which, const double *in, size_t n);
     if_status ghosts(int slot, int32_t *out, if_ghost *recs, size_t cap, size_t *n);
     if_status set_target(int slot, int nsteps);
+    if_status autotune(if_tune_result *out);
     if_status sync();
 
     int resident() const;

@@ -785,6 +785,160 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
     return IF_OK;
 }
 
+// Launch-shape autotuner.  Times the candidate Poisson sweep shapes on the
+// running cases' current fields: each candidate runs a few launches that
+// read the solution and write only the solver's scratch buffers (FP64:
+// launch 0 of a chunk reads p[base] and writes p[base^1]; mixed: launch 1
+// reads e[1] and writes e[0]); the convergence check likewise writes only
+// scratch (the other p buffer, r32, tile partials).  The control blocks are
+// saved before and restored after, and reset to the saved copy before each
+// timed check (a check flips the case's buffer parity), so neither the
+// solution nor the loop state changes.
+if_status Engine::autotune(if_tune_result *out)
+{
+    if (dirty_) {
+        if_status rc = refresh();
+        if (rc != IF_OK) return rc;
+    }
+    if (order_.empty()) return fail(IF_ERR_STATE, "autotune: no running case");
+    for (int id : order_)
+        if (slots_[id]->dev.ppe_solver == IF_PPE_JACOBI)
+            return fail(IF_ERR_STATE, "autotune: red-black SOR cases only");
+    std::vector<Ctrl> ctl;
+    if_status rc = read_ctrl_sync(ctl);
+    if (rc != IF_OK) return rc;
+    double cells = 0.0;
+    std::vector<int> last;
+    for (int id : order_) {
+        if (!ctl[id].active || ctl[id].failed) continue;
+        cells += (double)slots_[id]->NX * slots_[id]->NY;
+        if (slots_[id]->last.ppe_sweeps > 0) last.push_back(slots_[id]->last.ppe_sweeps);
+    }
+    if (cells <= 0.0) return fail(IF_ERR_STATE, "autotune: no running case");
+    const int n = (int)order_.size();
+    const int nall = std::max(1, (int)slots_.size());
+
+    Ctrl *saved = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (saved) (void)hipFree(saved);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    if (hipMalloc(&saved, sizeof(Ctrl) * nall) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return fail(IF_ERR_NOMEM, "autotune: scratch allocation failed");
+    }
+    if (hipMemcpyAsync(saved, d_ctrl_, sizeof(Ctrl) * nall, hipMemcpyDeviceToDevice, st_) != hipSuccess) {
+        cleanup();
+        return fail(IF_ERR_HIP, "autotune: control-block copy failed");
+    }
+
+    struct Shape {
+        int kind, ty, m;
+    };
+    static const Shape cands[] = {{3, 32, 4}, {3, 48, 4}, {3, 64, 4}, {3, 24, 2}, {3, 40, 2},
+                                  {2, 16, 4}, {2, 32, 4}, {2, 16, 2}, {0, 16, 2}, {0, 32, 4}};
+    constexpr int kReps = 6;
+    auto sweep_launch = [&](const Shape &c, int s, int K) {
+        if (c.kind >= 2)
+            launch_ppe_sweep_reg(st_, prec_, c.kind == 3 ? 8 : 4, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
+                                 s, K);
+        else if (c.kind == 1)
+            launch_ppe_sweep_stream(st_, prec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+        else
+            launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+    };
+    auto elapsed = [&](float *ms) -> bool {
+        return hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(ms, e0, e1) == hipSuccess;
+    };
+    const Shape keep{sweep_kind, sweep_ty, sweep_m};
+    Shape best = keep;
+    double best_ns = 1e300;
+    int ncand = 0;
+    for (const Shape &c : cands) {
+        sweep_kind = c.kind;
+        sweep_ty = c.ty;
+        sweep_m = c.m;
+        rc = refresh();
+        if (rc != IF_OK) break;
+        const int s0 = prec_ == IF_FP64 ? 0 : c.m;
+        launch_tune_arm(st_, d_cases_, n, d_ctrl_, saved);
+        sweep_launch(c, s0, 2 * c.m);  // warm-up
+        sweep_launch(c, s0, 2 * c.m);
+        (void)hipEventRecord(e0, st_);
+        for (int r = 0; r < kReps; r++) sweep_launch(c, s0, 2 * c.m);
+        (void)hipEventRecord(e1, st_);
+        float ms = 0.f;
+        if (!elapsed(&ms)) {
+            rc = fail(IF_ERR_HIP, "autotune: timing failed");
+            break;
+        }
+        const double ns = 1e6 * (double)ms / (kReps * (double)c.m * cells);
+        ncand++;
+        if (ns < best_ns) {
+            best_ns = ns;
+            best = c;
+        }
+    }
+    double check_sweeps = 0.0;
+    if (rc == IF_OK) {
+        sweep_kind = best.kind;
+        sweep_ty = best.ty;
+        sweep_m = best.m;
+        rc = refresh();
+    }
+    if (rc == IF_OK) {
+        (void)hipEventRecord(e0, st_);
+        for (int r = 0; r < kReps; r++) {
+            launch_tune_arm(st_, d_cases_, n, d_ctrl_, saved);
+            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 2 * best.m, best.m);
+        }
+        (void)hipEventRecord(e1, st_);
+        float ms = 0.f;
+        if (!elapsed(&ms)) rc = fail(IF_ERR_HIP, "autotune: timing failed");
+        else check_sweeps = 1e6 * (double)ms / kReps / (best_ns * cells);
+    }
+    // restore the loop state
+    const bool restored =
+        hipMemcpyAsync(d_ctrl_, saved, sizeof(Ctrl) * nall, hipMemcpyDeviceToDevice, st_) == hipSuccess &&
+        hipStreamSynchronize(st_) == hipSuccess;
+    cleanup();
+    if (!restored) return fail(IF_ERR_HIP, "autotune: control-block restore failed");
+    if (rc != IF_OK) {
+        sweep_kind = keep.kind;
+        sweep_ty = keep.ty;
+        sweep_m = keep.m;
+        mark_dirty();
+        return rc;
+    }
+    int S = 0;
+    if (!last.empty()) {
+        std::nth_element(last.begin(), last.begin() + last.size() / 2, last.end());
+        S = last[last.size() / 2];
+    }
+    if (S > 0) {
+        const double k = std::sqrt(2.0 * S * std::max(check_sweeps, 0.0));
+        const int m = best.m;
+        int K = (int)std::lround(k / m) * m;
+        K = std::max(std::max(4, m), std::min(1024, K));
+        check_k = K;
+    }
+    mark_dirty();
+    if (out) {
+        out->kernel = best.kind;
+        out->tile_y = best.ty;
+        out->sweeps_per_launch = best.m;
+        out->check_every = check_k;
+        out->sweep_ns = best_ns;
+        out->check_sweeps = check_sweeps;
+        out->typical_sweeps = S;
+        out->candidates = ncand;
+    }
+    return IF_OK;
+}
+
 if_status Engine::slot_state(int k, int *state, int *steps_done, if_step_stats *st)
 {
     Slot *s = slot(k);

@@ -984,6 +984,25 @@ void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *til
         hipLaunchKernelGGL(k_project<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
 }
 
+// Launch-shape timing: reset every running case's control block to the
+// saved copy (saved mirrors the control array that starts at ctrl0) and
+// re-open its Poisson loop, so every timed launch sees the same state.
+__global__ void k_tune_arm(const CaseDev *cs, int ncase, const Ctrl *ctrl0, const Ctrl *saved)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncase) return;
+    Ctrl *c = cs[k].ctrl;
+    *c = saved[c - ctrl0];
+    if (!c->active || c->failed) return;
+    c->ppe_done = 0;
+    c->ppe_sweeps = 0;
+}
+
+void launch_tune_arm(hipStream_t st, const CaseDev *cs, int ncase, const Ctrl *ctrl0, const Ctrl *saved)
+{
+    hipLaunchKernelGGL(k_tune_arm, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase, ctrl0, saved);
+}
+
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase)
 {
     hipLaunchKernelGGL(k_step_end, dim3(ncase), dim3(256), 0, st, cs, ncase);

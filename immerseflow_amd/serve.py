@@ -23,15 +23,22 @@ def _add(batch, c):
                      check_every=c.get("check_every", 160), t0=c.get("t0", 0.0))
 
 
-def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(3, 32, 4, 160),
+def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
           time_scale=1.0, keep_fields=False, log=None):
     """Run a request trace [(arrival_s, case_dict), ...] to completion.
 
+    tune: (kernel, tile_y, sweeps_per_launch, check_every), or "auto" to run
+    the autotuner once on the resident batch after its first batched step.
     Returns (records, summary).  records: one dict per request with
     arrival / admitted / finished times (s, trace clock), latency, state,
     steps, cells and the final max |div|."""
     batch = Batch(precision, hbm_budget)
-    batch.tune(*tune)
+    auto = tune == "auto"
+    if not auto:
+        batch.tune(*tune)
+    else:
+        batch.tune(3, 32, 4, 160)
+    tuned = None
     order = sorted(range(len(trace)), key=lambda k: trace[k][0])
     pending = collections.deque()
     active = {}
@@ -62,6 +69,10 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(3, 32, 4, 160),
             continue
         batch.step(1)
         batched_steps += 1
+        if auto and tuned is None and any(batch.state(sl)[0] == 0 for sl in active):
+            tuned = batch.autotune()
+            if log:
+                log("autotuned: %s" % tuned)
         now = time.perf_counter() - t0
         for slot in list(active):
             state, steps_done, st = batch.state(slot)
@@ -88,5 +99,5 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune=(3, 32, 4, 160),
                    p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
                    mcell_steps_per_s=sum(r["cells"] * r["steps"] for r in ok) / wall / 1e6,
                    max_div=max((r["max_div"] for r in ok), default=None),
-                   batched_steps=batched_steps, max_resident=max_resident)
+                   batched_steps=batched_steps, max_resident=max_resident, autotune=tuned)
     return records, summary

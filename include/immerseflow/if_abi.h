@@ -137,6 +137,16 @@ typedef struct {
     double bix, biy, ipx, ipy;
 } if_ghost;
 
+/* Result of the launch-shape autotuner (if_case_autotune). */
+typedef struct {
+    int kernel, tile_y, sweeps_per_launch;  /* the fastest Poisson sweep shape */
+    int check_every;        /* chosen check interval in sweeps */
+    double sweep_ns;        /* its time per interior cell-sweep (all running cases) */
+    double check_sweeps;    /* one convergence check, in sweeps of that shape */
+    int typical_sweeps;     /* sweeps per solve the check interval was chosen for (0: none yet) */
+    int candidates;         /* shapes timed */
+} if_tune_result;
+
 typedef struct if_case if_case;
 typedef struct if_batch if_batch;
 
@@ -195,6 +205,18 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * the tile height or the sweeps per launch. */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
+/* Autotuner (no counterpart in the reference, whose loop is fixed):
+ * times every Poisson sweep launch shape of if_case_tune on the case's
+ * current fields (a few launches each, on the solver's scratch buffers;
+ * the solution and the loop state are left as they were) and keeps the
+ * fastest.  The check interval is then chosen from the measured cost of a
+ * convergence check c (in sweeps) and the sweep count S of the last solve:
+ * checking every K sweeps costs about S c / K checks plus K / 2 sweeps of
+ * overshoot, least at K = sqrt(2 S c) (rounded to whole launches, 4..1024).
+ * Before the first step S is unknown and the interval is left unchanged.
+ * Results do not depend on the shape; the interval changes only where the
+ * solve stops (still at the same tolerance). */
+if_status if_case_autotune(if_case *c, if_tune_result *out);
 /* Accumulated event timing of the Poisson sweep kernel (milliseconds,
  * launches) over the steps since the last reset. */
 if_status if_case_sweep_timing(if_case *c, int reset, double *ms,
@@ -236,6 +258,10 @@ if_status if_batch_info(if_batch *b, int *resident, size_t *bytes_in_use,
 /* As if_case_tune, for every case of the batch. */
 if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int sweeps_per_launch,
                         int check_every);
+/* As if_case_autotune, over every running case of the batch at once (the
+ * batched launches are what is timed; S is the median of the cases' last
+ * sweep counts). */
+if_status if_batch_autotune(if_batch *b, if_tune_result *out);
 /* Footprint in device bytes that if_batch_add would charge for desc. */
 if_status if_case_footprint(const if_case_desc *desc, int precision,
                             size_t *bytes);

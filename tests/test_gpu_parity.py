@@ -287,6 +287,60 @@ def test_batch_alone_equals_in_mix(require_gpu):
     mix.close()
 
 
+@pytest.mark.parametrize("prec", [iff.IF_FP64, iff.IF_MIXED])
+def test_autotune_leaves_state_and_matches_manual_tune(require_gpu, prec):
+    """The autotuner times launch shapes on the live fields but leaves the
+    solution and loop state as they were: afterwards the case continues
+    exactly like one given the chosen shape and interval by hand."""
+    xf, yf, bodies = CASES["ellipse"]
+    _say("autotune", prec)
+    kw = dict(precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8, ppe_itermax=200000,
+              ppe_tol=1e-9, check_every=10)
+    a, b = iff.Case(xf, yf, **kw), iff.Case(xf, yf, **kw)
+    r0 = a.autotune()                       # before any step: interval unchanged
+    assert r0["typical_sweeps"] == 0 and r0["check_every"] == 10 and r0["candidates"] == 10
+    a.step(2)
+    b.step(2)
+    r = a.autotune()
+    assert r["candidates"] == 10 and r["typical_sweeps"] > 0
+    assert r["check_every"] % r["sweeps_per_launch"] == 0 and 4 <= r["check_every"] <= 1024
+    assert r["sweep_ns"] > 0 and r["check_sweeps"] > 0
+    for f in ("u", "v", "p", "uf", "vf"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    b.tune(r["kernel"], r["tile_y"], r["sweeps_per_launch"], r["check_every"])
+    for _ in range(2):
+        sa, sb = a.step(1), b.step(1)
+        assert sa["ppe_sweeps"] == sb["ppe_sweeps"] and sa["ad_iters"] == sb["ad_iters"]
+    for f in ("u", "v", "p", "uf", "vf"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+def test_batch_autotune(require_gpu):
+    specs = [(np.linspace(0, 3, 61), np.linspace(0, 2, 41), [iff.circle(1.0, 1.0, 0.3)], 0.01, 120.0),
+             (stretched_faces(150, 4, 1.005), stretched_faces(90, 3, 1.0), [iff.ellipse(2, 1.5, 0.4, 0.1, 0.3)],
+              0.008, 300.0)]
+    _say("batch autotune")
+    kw = dict(ppe_tol=1e-8, check_every=10, ppe_omega=1.85)
+    bs = []
+    for _ in range(2):
+        bt = iff.Batch(iff.IF_MIXED)
+        sl = [bt.add(xf, yf, 6, dt=dt, re=re, bodies=bo, **kw) for xf, yf, bo, dt, re in specs]
+        bt.step(1)
+        bs.append((bt, sl))
+    r = bs[0][0].autotune()
+    assert r["candidates"] == 10 and r["typical_sweeps"] > 0
+    bs[1][0].tune(r["kernel"], r["tile_y"], r["sweeps_per_launch"], r["check_every"])
+    for bt, _ in bs:
+        bt.step(2)
+    for k in range(len(specs)):
+        for f in ("u", "v", "p"):
+            assert np.array_equal(bs[0][0].get(bs[0][1][k], f), bs[1][0].get(bs[1][1][k], f)), (k, f)
+    for bt, _ in bs:
+        bt.close()
+
+
 def test_batch_budget_admission(require_gpu):
     xf = np.linspace(0, 1, 65)
     need = iff.Batch.footprint(xf, xf, dt=0.01, re=100.0)
