@@ -79,13 +79,35 @@ def grid_metric(xf):
     return xc, dx
 
 
-def write_tecplot(path, xc, yc, f):
-    """write_results_to_file (postSim.cu:41-66)."""
+def write_tecplot(path, xc, yc, f, digits=0):
+    """write_results_to_file (postSim.cu:41-66); digits > 0 writes every
+    value at that many significant digits (17: a lossless checkpoint)."""
     xc = np.ascontiguousarray(xc, dtype=np.float64)
     yc = np.ascontiguousarray(yc, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64)
-    _check(_lib().if_write_tecplot(path.encode(), len(xc), len(yc), _dptr(xc), _dptr(yc),
-                                   _dptr(f)), "write " + path)
+    if digits:
+        rc = _lib().if_write_tecplot_digits(path.encode(), len(xc), len(yc), _dptr(xc), _dptr(yc),
+                                            _dptr(f), digits)
+    else:
+        rc = _lib().if_write_tecplot(path.encode(), len(xc), len(yc), _dptr(xc), _dptr(yc), _dptr(f))
+    _check(rc, "write " + path)
+
+
+def read_tecplot(path, nx, ny):
+    """Read a file in the writer's format: (x[nx], y[ny], f[nx*ny])."""
+    x, y, f = np.zeros(nx), np.zeros(ny), np.zeros(nx * ny)
+    _check(_lib().if_read_tecplot(path.encode(), nx, ny, _dptr(x), _dptr(y), _dptr(f)), "read " + path)
+    return x, y, f
+
+
+def face_coords(xc, yc):
+    """Coordinates of the uf ((nx-1) x (ny-2)) and vf ((nx-2) x (ny-1))
+    arrays in the reference layouts: face midpoints and cell rows."""
+    xc, yc = np.asarray(xc, dtype=np.float64), np.asarray(yc, dtype=np.float64)
+    return ((0.5 * (xc[:-1] + xc[1:]), yc[1:-1]), (xc[1:-1], 0.5 * (yc[:-1] + yc[1:])))
+
+
+CHECKPOINT_FILES = ("uc.dat", "vc.dat", "p.dat", "uf.dat", "vf.dat")
 
 
 def device_query():
@@ -255,6 +277,38 @@ class Case:
         sweeps per launch, sweeps per check.  Every kernel and shape gives
         the same result."""
         _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
+
+    def restart(self, u, v, p, uf=None, vf=None):
+        """Load u, v, p (and the faces; without them the faces are
+        interpolated from u, v) -- see if_case_restart."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (u, v, p, uf, vf)]
+        ptrs = [_dptr(a) if a is not None else None for a in arrs]
+        _check(_lib().if_case_restart(self._h, *ptrs), "restart")
+
+    def write_checkpoint(self, directory, digits=17):
+        """uc.dat, vc.dat, p.dat and the faces uf.dat, vf.dat in the
+        writer's format (17 digits: lossless)."""
+        import os
+        (ux, uy), (vx, vy) = face_coords(self.xc, self.yc)
+        for name, arr, xs, ys in (("uc.dat", self.get("u"), self.xc, self.yc),
+                                  ("vc.dat", self.get("v"), self.xc, self.yc),
+                                  ("p.dat", self.get("p"), self.xc, self.yc),
+                                  ("uf.dat", self.get("uf"), ux, uy), ("vf.dat", self.get("vf"), vx, vy)):
+            write_tecplot(os.path.join(directory, name), xs, ys, arr, digits=digits)
+
+    def restart_from(self, directory):
+        """Restart from write_checkpoint's files, or from the reference's
+        uc.dat/vc.dat/p.dat alone (faces then interpolated)."""
+        import os
+        nx, ny = len(self.xc), len(self.yc)
+        u = read_tecplot(os.path.join(directory, "uc.dat"), nx, ny)[2]
+        v = read_tecplot(os.path.join(directory, "vc.dat"), nx, ny)[2]
+        p = read_tecplot(os.path.join(directory, "p.dat"), nx, ny)[2]
+        uf = vf = None
+        if os.path.exists(os.path.join(directory, "uf.dat")) and os.path.exists(os.path.join(directory, "vf.dat")):
+            uf = read_tecplot(os.path.join(directory, "uf.dat"), nx - 1, ny - 2)[2]
+            vf = read_tecplot(os.path.join(directory, "vf.dat"), nx - 2, ny - 1)[2]
+        self.restart(u, v, p, uf, vf)
 
     def autotune(self):
         """Time every Poisson sweep launch shape on the current fields, keep

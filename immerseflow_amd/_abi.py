@@ -80,6 +80,9 @@ SIGNATURES = [
     ("if_read_grid", C.c_int, [C.c_char_p, C.c_int, _dp]),
     ("if_grid_metric", C.c_int, [C.c_int, _dp, _dp, _dp]),
     ("if_write_tecplot", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp]),
+    ("if_write_tecplot_digits", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]),
+    ("if_read_tecplot", C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp]),
+    ("if_case_restart", C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     ("if_case_create", C.c_int, [C.POINTER(IfCaseDesc), C.POINTER(C.c_void_p)]),
     ("if_case_destroy", None, [C.c_void_p]),
     ("if_case_ad_step", C.c_int, [C.c_void_p, C.POINTER(IfStepStats)]),

@@ -36,6 +36,11 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def rocm_include():
+    """ROCm's include directory (HIP runtime headers for the host-only C++)."""
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "include")
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
@@ -50,8 +55,8 @@ def build(verbose=False):
     for s in HIP_SOURCES:
         jobs.append([cc] + COMMON + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o",
                                                os.path.join(BUILD, s + ".o")])
-    for s in CPP_SOURCES:
-        jobs.append([cc] + COMMON + ["-c", os.path.join(CSRC, s), "-o",
+    for s in CPP_SOURCES:  # host-only C++ (HIP runtime API, no device code)
+        jobs.append([cc] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + rocm_include(), "-c", os.path.join(CSRC, s), "-o",
                                      os.path.join(BUILD, s + ".o")])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         outs = list(ex.map(_run, jobs))
@@ -67,8 +72,8 @@ def build(verbose=False):
                                      os.path.join(PKG, "hbm_copy")])
     cli_src = os.path.join(CSRC, "cli.cpp")
     if os.path.exists(cli_src):
-        _run([cc] + COMMON + [cli_src, "-o", CLI, "-L" + PKG, "-limmerseflow_amd",
-                              "-Wl,-rpath,$ORIGIN"])
+        _run([cc] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + rocm_include(), cli_src, "-x", "none", "-o", CLI,
+                              "-L" + PKG, "-limmerseflow_amd", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

@@ -61,6 +61,17 @@ if_status if_grid_metric(int nf, const double *xf, double *xc, double *dx)
     IFF_GUARD(grid_metric(nf, xf, xc, dx));
 }
 
+if_status if_write_tecplot_digits(const char *path, int nx, int ny, const double *x, const double *y,
+                                  const double *f, int digits)
+{
+    IFF_GUARD(write_tecplot_digits(path, nx, ny, x, y, f, digits));
+}
+
+if_status if_read_tecplot(const char *path, int nx, int ny, double *x, double *y, double *f)
+{
+    IFF_GUARD(read_tecplot(path, nx, ny, x, y, f));
+}
+
 if_status if_write_tecplot(const char *path, int nx, int ny, const double *x, const double *y,
                            const double *f)
 {
@@ -222,6 +233,13 @@ static if_status case_autotune(if_case *c, if_tune_result *out)
     rc = c->eng->autotune(out);
     const if_status rc2 = c->eng->set_target(c->slot, 0);
     return rc != IF_OK ? rc : rc2;
+}
+
+if_status if_case_restart(if_case *c, const double *u, const double *v, const double *p, const double *uf,
+                          const double *vf)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "restart: physics mode only");
+    IFF_GUARD(c->eng->restart(c->slot, u, v, p, uf, vf));
 }
 
 if_status if_case_autotune(if_case *c, if_tune_result *out)

@@ -46,6 +46,8 @@ public:
     if_status slot_state(int slot, int *state, int *steps_done, if_step_stats *st);
     if_status get_field(int slot, int which, double *out, size_t n);
     if_status set_field(int slot, int which, const double *in, size_t n);
+    if_status restart(int slot, const double *u, const double *v, const double *p, const double *uf,
+                      const double *vf);
     if_status ghosts(int slot, int32_t *out, if_ghost *recs, size_t cap, size_t *n);
     if_status set_target(int slot, int nsteps);
     if_status autotune(if_tune_result *out);

@@ -1068,12 +1068,59 @@ if_status Engine::set_field(int k, int which, const double *in, size_t n)
     if (ts && prec_ == IF_MIXED) {
         std::vector<float> hf(hd.begin(), hd.end());
         IFF_HIP(hipMemcpy(dst, hf.data(), np * 4, hipMemcpyHostToDevice));
-        if (which == IF_FIELD_U || which == IF_FIELD_V) {
-            // keep the other ping-pong buffer's ghost ring consistent
-        }
     } else {
         IFF_HIP(hipMemcpy(dst, hd.data(), np * 8, hipMemcpyHostToDevice));
     }
+    return IF_OK;
+}
+
+// Restart: load u, v, p (and the faces when given), then refresh the
+// ghost-cell velocities and, without given faces, interpolate the faces
+// from u and v -- the kernels and order of the case's initialisation, run
+// on this case alone.
+if_status Engine::restart(int k, const double *u, const double *v, const double *p, const double *uf,
+                          const double *vf)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "restart: empty slot");
+    if (!u || !v || !p) return fail(IF_ERR_ARG, "restart: u, v and p are required");
+    if ((uf == nullptr) != (vf == nullptr)) return fail(IF_ERR_ARG, "restart: give both uf and vf or neither");
+    const size_t ncell = (size_t)s->nx * s->ny;
+    if_status rc = set_field(k, IF_FIELD_U, u, ncell);
+    if (rc == IF_OK) rc = set_field(k, IF_FIELD_V, v, ncell);
+    if (rc == IF_OK) rc = set_field(k, IF_FIELD_P, p, ncell);
+    if (rc == IF_OK && uf) rc = set_field(k, IF_FIELD_UF, uf, (size_t)(s->nx - 1) * (s->ny - 2));
+    if (rc == IF_OK && vf) rc = set_field(k, IF_FIELD_VF, vf, (size_t)(s->nx - 2) * (s->ny - 1));
+    if (rc != IF_OK) return rc;
+    std::vector<Ctrl> all;
+    rc = read_ctrl_sync(all);
+    if (rc != IF_OK) return rc;
+    const Ctrl ct = all[k];
+    CaseDev *tmp = nullptr;
+    int2 *dtl = nullptr;
+    auto done = [&]() {
+        if (tmp) (void)hipFree(tmp);
+        if (dtl) (void)hipFree(dtl);
+    };
+    std::vector<int2> tl(s->ntx * s->nty);
+    for (int t = 0; t < (int)tl.size(); t++) tl[t] = make_int2(0, t);
+    if (hipMalloc(&tmp, sizeof(CaseDev)) != hipSuccess || hipMalloc(&dtl, sizeof(int2) * tl.size()) != hipSuccess) {
+        done();
+        return fail(IF_ERR_NOMEM, "restart: scratch allocation failed");
+    }
+    Ctrl on = ct;
+    on.active = 1;  // the ghost kernels act on active cases
+    bool ok = hipMemcpyAsync(tmp, &s->dev, sizeof(CaseDev), hipMemcpyHostToDevice, st_) == hipSuccess &&
+              hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_) == hipSuccess &&
+              hipMemcpyAsync(s->dev.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_) == hipSuccess;
+    if (ok) {
+        launch_ghost_vel(st_, prec_, tmp, 1, s->gcap, 1);
+        if (!uf) launch_faces(st_, prec_, tmp, dtl, (int)tl.size(), 0, 0);
+        ok = hipMemcpyAsync(s->dev.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_) == hipSuccess &&
+             hipStreamSynchronize(st_) == hipSuccess;
+    }
+    done();
+    if (!ok) return fail(IF_ERR_HIP, "restart: device update failed");
     return IF_OK;
 }
 

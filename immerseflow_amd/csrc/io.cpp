@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "io.h"
 
 namespace iff {
 
@@ -193,7 +194,16 @@ if_status grid_metric(int nf, const double *xf, double *xc, double *dx)
 if_status write_tecplot(const char *path, int nx, int ny, const double *x,
                         const double *y, const double *f)
 {
-    if (!path || !x || !y || !f || nx < 1 || ny < 1)
+    return write_tecplot_digits(path, nx, ny, x, y, f, 0);
+}
+
+// The same file with every value at `digits` significant digits ("%.*e");
+// digits = 0 is the reference's "%f".  17 digits round-trip FP64 exactly,
+// which makes the file a lossless checkpoint.
+if_status write_tecplot_digits(const char *path, int nx, int ny, const double *x,
+                               const double *y, const double *f, int digits)
+{
+    if (!path || !x || !y || !f || nx < 1 || ny < 1 || digits < 0 || digits > 17)
         return fail(IF_ERR_ARG, "write_tecplot: bad argument");
     FILE *fp = std::fopen(path, "w");
     if (!fp) return fail(IF_ERR_IO, std::string("Error opening file: ") + path);
@@ -202,12 +212,66 @@ if_status write_tecplot(const char *path, int nx, int ny, const double *x,
     std::fprintf(fp, "TITLE = \"Post Processing Tecplot\"\n");
     std::fprintf(fp, "VARIABLES = \"X\",\"Y\",\"T\"\n");
     std::fprintf(fp, "ZONE T=\"BIG ZONE\", I=%d, J=%d, DATAPACKING=POINT\n", nx, ny);
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++)
-            std::fprintf(fp, "%f,%f,%f\n", x[i], y[j], f[i + j * nx]);
+    if (digits == 0) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                std::fprintf(fp, "%f,%f,%f\n", x[i], y[j], f[i + (size_t)j * nx]);
+    } else {
+        const int d = digits - 1;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                std::fprintf(fp, "%.*e,%.*e,%.*e\n", d, x[i], d, y[j], d, f[i + (size_t)j * nx]);
+    }
     const bool bad = std::ferror(fp);
     std::fclose(fp);
     if (bad) return fail(IF_ERR_IO, std::string("write failed: ") + path);
+    return IF_OK;
+}
+
+// Reader of the writer's format (a restart source): the three header lines,
+// the ZONE line's I and J must equal nx and ny, then nx*ny rows "x,y,value"
+// (j outer).  x (nx values), y (ny values) may be null.  The predecessor
+// restarts from such files (test/UTIL_PRE_SIM.f90:130-149).
+if_status read_tecplot(const char *path, int nx, int ny, double *x, double *y, double *f)
+{
+    if (!path || !f || nx < 1 || ny < 1) return fail(IF_ERR_ARG, "read_tecplot: bad argument");
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return fail(IF_ERR_IO, std::string("Error opening file: ") + path);
+    std::vector<char> buf(1 << 20);
+    std::setvbuf(fp, buf.data(), _IOFBF, buf.size());
+    char line[512];
+    int fi = -1, fj = -1;
+    bool ok = true;
+    for (int h = 0; h < 3 && ok; h++) {
+        if (!std::fgets(line, sizeof(line), fp)) ok = false;
+        else if (h == 2) {
+            const char *pi = std::strstr(line, "I="), *pj = std::strstr(line, "J=");
+            if (!pi || !pj || std::strncmp(line, "ZONE", 4) != 0) ok = false;
+            else {
+                fi = std::atoi(pi + 2);
+                fj = std::atoi(pj + 2);
+            }
+        }
+    }
+    if (!ok || fi != nx || fj != ny) {
+        std::fclose(fp);
+        return fail(IF_ERR_PARSE, std::string(path) + ": header is not a " + std::to_string(nx) + " x " +
+                                      std::to_string(ny) + " Tecplot POINT zone");
+    }
+    for (int j = 0; j < ny && ok; j++)
+        for (int i = 0; i < nx && ok; i++) {
+            double a, b, v;
+            if (std::fscanf(fp, " %lf , %lf , %lf", &a, &b, &v) != 3) {
+                ok = false;
+                break;
+            }
+            if (x && j == 0) x[i] = a;
+            if (y && i == 0) y[j] = b;
+            f[i + (size_t)j * nx] = v;
+        }
+    std::fclose(fp);
+    if (!ok) return fail(IF_ERR_PARSE, std::string(path) + ": fewer than " + std::to_string((size_t)nx * ny) +
+                                           " data rows");
     return IF_OK;
 }
 

@@ -167,6 +167,16 @@ if_status if_grid_metric(int nf, const double *xf, double *xc, double *dx);
 /* if_write_tecplot replaces write_results_to_file (postSim.cu:41-66). */
 if_status if_write_tecplot(const char *path, int nx, int ny, const double *x,
                            const double *y, const double *f);
+/* The writer's format at `digits` significant digits (1..17; 0 = the
+ * reference's "%f").  17 digits are a lossless FP64 checkpoint.  No
+ * counterpart in the reference (its writer is fixed at "%f"). */
+if_status if_write_tecplot_digits(const char *path, int nx, int ny, const double *x,
+                                  const double *y, const double *f, int digits);
+/* Read a file in the writer's format (any digit count): the ZONE line's I/J
+ * must equal nx/ny; x (nx) and y (ny) may be NULL.  Restart source, as the
+ * predecessor's data files are (test/UTIL_PRE_SIM.f90:130-149). */
+if_status if_read_tecplot(const char *path, int nx, int ny, double *x, double *y,
+                          double *f);
 
 /* ---- single case ---------------------------------------------------------
  * if_case_create replaces allocation + readGridData + initializeData. */
@@ -185,6 +195,15 @@ if_status if_case_step(if_case *c, int nsteps, if_step_stats *st);
 if_status if_case_get_field(if_case *c, int which, double *out, size_t n);
 /* Load a field (restart); same layouts as if_case_get_field. */
 if_status if_case_set_field(if_case *c, int which, const double *in, size_t n);
+/* Restart (the deck's Restart flag, main.cu:27-30; predecessor
+ * test/UTIL_PRE_SIM.f90:130-158): load u, v, p (nx*ny, ghosts included)
+ * and the face velocities uf ((nx-1)*(ny-2)) and vf ((nx-2)*(ny-1)).  With
+ * uf and vf NULL the faces are interpolated from u and v with the blanked
+ * faces at the body velocity, as at initialisation and as the predecessor
+ * does.  Ghost-cell velocities are then refreshed.  The case's time is the
+ * t0 it was created with (create it with t0 = the restart time). */
+if_status if_case_restart(if_case *c, const double *u, const double *v, const double *p,
+                          const double *uf, const double *vf);
 /* Grid vectors of the case (xc, yc, dx, dy: nx and ny values). */
 if_status if_case_grid(if_case *c, double *xc, double *yc, double *dx,
                        double *dy);

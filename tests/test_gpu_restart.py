@@ -1,0 +1,65 @@
+"""GPU: restart (the deck's Restart flag; the predecessor's restart from
+data files).  A checkpoint written at 17 digits (u, v, p and the faces)
+restarts a case exactly: the continued run equals the uninterrupted one bit
+for bit.  The reference's own %f outputs (u, v, p only) restart a case whose
+faces are interpolated from u and v, which continues to the stop level."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import CASES, rel
+
+import immerseflow_amd as iff
+
+pytestmark = pytest.mark.gpu
+
+
+def _kw(prec):
+    xf, yf, bodies = CASES["ellipse"]
+    return xf, yf, dict(precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8, ppe_itermax=200000,
+                        ppe_tol=1e-9, check_every=10)
+
+
+@pytest.mark.parametrize("prec", [iff.IF_FP64, iff.IF_MIXED])
+def test_checkpoint_restart_is_exact(require_gpu, prec, tmp_path):
+    print("[gpu-test] checkpoint restart", prec, flush=True)
+    xf, yf, kw = _kw(prec)
+    a = iff.Case(xf, yf, **kw)
+    a.step(3)
+    a.write_checkpoint(str(tmp_path))
+    b = iff.Case(xf, yf, **kw)
+    b.restart_from(str(tmp_path))
+    for f in ("u", "v", "p", "uf", "vf"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    for _ in range(2):
+        sa, sb = a.step(1), b.step(1)
+        assert sa["ad_iters"] == sb["ad_iters"] and sa["ppe_sweeps"] == sb["ppe_sweeps"]
+    for f in ("u", "v", "p", "uf", "vf"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+def test_restart_from_reference_outputs(require_gpu, tmp_path):
+    """u, v, p alone (faces interpolated): from the reference's %f files the
+    run continues as from the same fields at full precision, to the files'
+    rounding, and reaches the stop level."""
+    print("[gpu-test] restart from %f files", flush=True)
+    xf, yf, kw = _kw(iff.IF_FP64)
+    a = iff.Case(xf, yf, **kw)
+    a.step(3)
+    (tmp_path / "ref").mkdir()
+    (tmp_path / "full").mkdir()
+    for name, field in (("uc.dat", "u"), ("vc.dat", "v"), ("p.dat", "p")):
+        iff.write_tecplot(str(tmp_path / "ref" / name), a.xc, a.yc, a.get(field))
+        iff.write_tecplot(str(tmp_path / "full" / name), a.xc, a.yc, a.get(field), digits=17)
+    b, c = iff.Case(xf, yf, **kw), iff.Case(xf, yf, **kw)
+    b.restart_from(str(tmp_path / "ref"))
+    c.restart_from(str(tmp_path / "full"))
+    assert np.array_equal(c.get("u"), a.get("u")) and rel(b.get("u"), a.get("u")) <= 1e-6
+    sb, sc = b.step(2), c.step(2)
+    for st in (sb, sc):
+        assert st["max_div"] <= kw["ppe_tol"] and st["failed"] == 0
+    for f in ("u", "v", "p"):
+        assert rel(b.get(f), c.get(f)) < 1e-4, f
+    for x in (a, b, c):
+        x.close()

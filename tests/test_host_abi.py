@@ -122,6 +122,49 @@ def test_writer_row_order_matches_reference_kat(tmp_path):
         assert float(ga[0]) == float(gb[0]) and float(ga[1]) == float(gb[1]) and ga[2] == gb[2]
 
 
+def test_checkpoint_writer_round_trips_exactly(tmp_path):
+    rng = np.random.default_rng(7)
+    x, y = np.cumsum(rng.random(5)), np.cumsum(rng.random(4)) - 1.0
+    f = rng.standard_normal(20) * 10.0 ** rng.integers(-12, 12, 20)
+    f[3], f[7] = 0.0, -0.0
+    out = tmp_path / "ck.dat"
+    iff.write_tecplot(str(out), x, y, f, digits=17)
+    gx, gy, gf = iff.read_tecplot(str(out), 5, 4)
+    assert np.array_equal(gf, f) and np.array_equal(gx, x) and np.array_equal(gy, y)
+    assert np.signbit(gf[7]) and not np.signbit(gf[3])
+
+
+def test_reader_reads_the_reference_format(tmp_path):
+    """The reference's own p.dat (printed at %f) reads back as its printed values."""
+    nx = ny = 52
+    x, y, f = iff.read_tecplot(os.path.join(GOLDEN, "p.dat"), nx, ny)
+    rows = open(os.path.join(GOLDEN, "p.dat")).read().splitlines()[3:]
+    assert len(rows) == nx * ny
+    for k in (0, 1, nx, nx * ny - 1, 1234):
+        a = [float(t) for t in rows[k].split(",")]
+        assert (x[k % nx], y[k // nx], f[k]) == (a[0], a[1], a[2])
+    # and the %f writer reproduces the file from what was read
+    out = tmp_path / "p.dat"
+    iff.write_tecplot(str(out), x, y, f)
+    assert out.read_bytes() == open(os.path.join(GOLDEN, "p.dat"), "rb").read()
+
+
+def test_reader_errors(tmp_path):
+    good = tmp_path / "g.dat"
+    iff.write_tecplot(str(good), np.arange(3.0), np.arange(2.0), np.arange(6.0))
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_tecplot(str(good), 2, 3)             # zone is 3 x 2
+    assert e.value.code == _abi.IF_ERR_PARSE
+    short = tmp_path / "s.dat"
+    short.write_text("\n".join(good.read_text().splitlines()[:-1]) + "\n")
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_tecplot(str(short), 3, 2)
+    assert e.value.code == _abi.IF_ERR_PARSE
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_tecplot(str(tmp_path / "missing.dat"), 3, 2)
+    assert e.value.code == _abi.IF_ERR_IO
+
+
 def test_writer_unwritable_path():
     with pytest.raises(iff.ImmerseFlowError) as e:
         iff.write_tecplot("/nonexistent_dir/x.dat", np.zeros(2), np.zeros(2), np.zeros(4))
