@@ -310,6 +310,25 @@ class Case:
             vf = read_tecplot(os.path.join(directory, "vf.dat"), nx - 2, ny - 1)[2]
         self.restart(u, v, p, uf, vf)
 
+    def monitor(self, probes=(), capacity=65536):
+        """Record t, the body force (Fx, Fy) and u, v, p at the cells nearest
+        the probe points [(x, y), ...] after every step (if_case_monitor)."""
+        xy = np.ascontiguousarray(np.asarray(probes, dtype=np.float64).reshape(-1))
+        self._nprobe = len(xy) // 2
+        _check(_lib().if_case_monitor(self._h, self._nprobe, _dptr(xy) if len(xy) else None, capacity),
+               "monitor")
+
+    def monitor_read(self):
+        """Records since the last read: array (n, 3 + 3*nprobe) with columns
+        t, Fx, Fy, then u, v, p per probe."""
+        n, dropped = C.c_size_t(), C.c_size_t()
+        _check(_lib().if_case_monitor_read(self._h, None, 0, C.byref(n), C.byref(dropped)), "monitor")
+        out = np.zeros((n.value, 3 + 3 * self._nprobe))
+        if n.value:
+            _check(_lib().if_case_monitor_read(self._h, _dptr(out), n.value, C.byref(n), C.byref(dropped)),
+                   "monitor")
+        return out[:n.value]
+
     def autotune(self):
         """Time every Poisson sweep launch shape on the current fields, keep
         the fastest and choose the check interval (see if_case_autotune).

@@ -97,6 +97,9 @@ SIGNATURES = [
     ("if_case_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("if_batch_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("if_case_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("if_case_monitor", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
+    ("if_case_monitor_read", C.c_int, [C.c_void_p, _dp, C.c_size_t, C.POINTER(C.c_size_t),
+                                       C.POINTER(C.c_size_t)]),
     ("if_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
     ("if_case_sweep_timing", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     ("if_case_byte_model", C.c_int, [C.c_void_p, _dp, _dp]),

@@ -242,6 +242,18 @@ if_status if_case_restart(if_case *c, const double *u, const double *v, const do
     IFF_GUARD(c->eng->restart(c->slot, u, v, p, uf, vf));
 }
 
+if_status if_case_monitor(if_case *c, int nprobe, const double *xy, int capacity)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "monitor: physics mode only");
+    IFF_GUARD(c->eng->monitor(c->slot, nprobe, xy, capacity));
+}
+
+if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size_t *nrec, size_t *dropped)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "monitor: physics mode only");
+    IFF_GUARD(c->eng->monitor_read(c->slot, out, cap_records, nrec, dropped));
+}
+
 if_status if_case_autotune(if_case *c, if_tune_result *out)
 {
     if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "autotune: physics mode only");

@@ -113,6 +113,16 @@ struct Ctrl {
 };
 
 // Everything a kernel needs to know about one case.
+// Per-step monitors of a case (probes and body force), in device memory.
+constexpr int kMaxProbes = 32;
+struct Monitor {
+    int nprobe, cap;                   // probes; ring capacity in records
+    int pi[kMaxProbes], pj[kMaxProbes];  // probe cells (array indices)
+    double *part;                      // Fx, Fy partial per interior tile
+    double *ring;                      // cap records of 3 + 3*nprobe doubles
+    unsigned long long *count;         // records written since the last read
+};
+
 struct CaseDev {
     int nx, ny, P;
     int ntx, nty;          // tiles of the interior (kTileX x tile_y)
@@ -123,6 +133,7 @@ struct CaseDev {
     int ppe_solver;
     int ad_itermax, ppe_itermax, check_every;
     double dt, rdt, omega, ppe_tol, ad_tol, ubc, vbc;
+    double mu;             // 1/Re
     float omega_f;
     AxisD x, y;
     AxisF xf, yf;
@@ -138,6 +149,7 @@ struct CaseDev {
     Ctrl *ctrl;
     double *part;                  // one partial per interior tile
     int *gcount;                   // ghost-compaction scratch (per 1024 cells)
+    Monitor *mon;                  // null: no monitors
 };
 
 }  // namespace iff

@@ -27,6 +27,8 @@ struct Slot {
     std::vector<std::vector<double>> polyx, polyy;
     std::vector<BodyDev> bodies_dev;   // host image (device pointers inside)
     void *arena = nullptr;
+    void *mon_mem = nullptr;           // monitors (Monitor + partials + ring + count)
+    size_t mon_bytes = 0;
     size_t bytes = 0;
     int *ghost_counts = nullptr;       // scratch for the ghost compaction
     int ghost_blocks = 0;
@@ -51,6 +53,8 @@ public:
     if_status ghosts(int slot, int32_t *out, if_ghost *recs, size_t cap, size_t *n);
     if_status set_target(int slot, int nsteps);
     if_status autotune(if_tune_result *out);
+    if_status monitor(int slot, int nprobe, const double *xy, int cap);
+    if_status monitor_read(int slot, double *out, size_t cap, size_t *nrec, size_t *dropped);
     if_status sync();
 
     int resident() const;
@@ -92,6 +96,7 @@ private:
     int tiles_sweep_ty_ = 0;
     int max_ng_ = 0, max_ring_ = 0, max_cells_ = 0, max_ad_ = 0, max_ppe_ = 0;
     bool any_moving_ = false;
+    bool any_monitor_ = false;
     bool dirty_ = true;
     // speculation ring
     static constexpr int kRing = 8;

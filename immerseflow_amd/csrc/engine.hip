@@ -36,6 +36,8 @@ Engine::~Engine()
         if (s && s->arena) hipFree(s->arena);
     for (auto &s : slots_)
         if (s && s->dev.gh) hipFree(s->dev.gh);
+    for (auto &s : slots_)
+        if (s && s->mon_mem) hipFree(s->mon_mem);
     if (d_cases_) hipFree(d_cases_);
     if (d_ctrl_) hipFree(d_ctrl_);
     if (d_tiles_base_) hipFree(d_tiles_base_);
@@ -315,6 +317,8 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.ad_tol = d.ad_tol;
     c.ubc = d.ubc;
     c.vbc = d.vbc;
+    c.mu = 1.0 / d.re;
+    c.mon = nullptr;
     for (int b = 0; b < 2; b++) {
         c.u[b] = A + a.off_u[b];
         c.v[b] = A + a.off_v[b];
@@ -497,6 +501,7 @@ if_status Engine::remove(int k)
     IFF_HIP(hipStreamSynchronize(st_));
     if (s->arena) hipFree(s->arena);
     if (s->dev.gh) hipFree(s->dev.gh);
+    if (s->mon_mem) hipFree(s->mon_mem);
     used_ -= s->bytes;
     Ctrl z{};
     IFF_HIP(hipMemcpyAsync(d_ctrl_ + k, &z, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
@@ -556,6 +561,7 @@ if_status Engine::refresh()
     std::vector<int2> tb, tsw;
     max_ng_ = max_ring_ = max_cells_ = max_ad_ = max_ppe_ = 0;
     any_moving_ = false;
+    any_monitor_ = false;
     const int sty = sweep_ty;
     for (int k = 0; k < n; k++) {
         Slot &s = *slots_[order_[k]];
@@ -571,6 +577,7 @@ if_status Engine::refresh()
         max_ad_ = std::max(max_ad_, s.dev.ad_itermax);
         max_ppe_ = std::max(max_ppe_, s.dev.ppe_itermax);
         any_moving_ = any_moving_ || s.moving;
+        any_monitor_ = any_monitor_ || s.mon_mem != nullptr;
     }
     if (n > 0) IFF_HIP(hipMemcpyAsync(d_cases_, cs.data(), sizeof(CaseDev) * n, hipMemcpyHostToDevice, st_));
     auto up = [&](int2 *&d, int &cap, int &cnt, const std::vector<int2> &v) -> if_status {
@@ -726,6 +733,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         launch_pressure_bc(st_, d_cases_, n, max_ring_, max_ng_);
         launch_project(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
         launch_ghost_vel(st_, prec_, d_cases_, n, max_ng_, 1);
+        if (any_monitor_) launch_monitor(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_);
         launch_step_end(st_, d_cases_, n);
         for (int id : order_) slots_[id]->steps_host++;
 
@@ -1071,6 +1079,99 @@ if_status Engine::set_field(int k, int which, const double *in, size_t n)
     } else {
         IFF_HIP(hipMemcpy(dst, hd.data(), np * 8, hipMemcpyHostToDevice));
     }
+    return IF_OK;
+}
+
+// Monitors: probes at the interior cell nearest each (x, y) (per axis the
+// first nearest centre, as the predecessor's minloc,
+// test/UTIL_PRE_SIM.f90:189-194) and the body force, recorded on the
+// device after every step into a ring of cap records.
+if_status Engine::monitor(int k, int nprobe, const double *xy, int cap)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "monitor: empty slot");
+    if (nprobe < 0 || nprobe > kMaxProbes) return fail(IF_ERR_ARG, "monitor: 0..32 probes");
+    if (nprobe > 0 && !xy) return fail(IF_ERR_ARG, "monitor: null probe list");
+    if (cap < 1) return fail(IF_ERR_ARG, "monitor: ring capacity must be >= 1");
+    Monitor m{};
+    m.nprobe = nprobe;
+    m.cap = cap;
+    auto nearest = [](const std::vector<double> &c, double x) {
+        int best = 1;
+        for (int i = 2; i + 1 < (int)c.size(); i++)
+            if (std::fabs(c[i] - x) < std::fabs(c[best] - x)) best = i;
+        return best;
+    };
+    for (int q = 0; q < nprobe; q++) {
+        m.pi[q] = nearest(s->xc, xy[2 * q]);
+        m.pj[q] = nearest(s->yc, xy[2 * q + 1]);
+    }
+    const size_t ntl = (size_t)s->ntx * s->nty;
+    const size_t stride = 3 + 3 * (size_t)nprobe;
+    // layout: the Monitor record, then the partials, the ring and the count,
+    // each on a 256-byte boundary after the previous part's end
+    const size_t o_part = (sizeof(Monitor) + 255) & ~(size_t)255;
+    const size_t o_ring = o_part + ((sizeof(double) * 2 * ntl + 255) & ~(size_t)255);
+    const size_t o_cnt = o_ring + ((sizeof(double) * stride * cap + 255) & ~(size_t)255);
+    const size_t total = o_cnt + 256;
+    IFF_HIP(hipStreamSynchronize(st_));
+    if (s->mon_mem) {  // replaced
+        (void)hipFree(s->mon_mem);
+        s->mon_mem = nullptr;
+        s->dev.mon = nullptr;
+        used_ -= s->mon_bytes;
+        s->bytes -= s->mon_bytes;
+        s->mon_bytes = 0;
+    }
+    void *mem = nullptr;
+    if (hipMalloc(&mem, total) != hipSuccess) return fail(IF_ERR_NOMEM, "monitor: device allocation failed");
+    char *b = (char *)mem;
+    m.part = (double *)(b + o_part);
+    m.ring = (double *)(b + o_ring);
+    m.count = (unsigned long long *)(b + o_cnt);
+    const unsigned long long zero = 0;
+    if (hipMemcpy(b, &m, sizeof(Monitor), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m.count, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(mem);
+        return fail(IF_ERR_HIP, "monitor: set-up copy failed");
+    }
+    s->mon_mem = mem;
+    s->mon_bytes = total;  // charged to the case
+    s->bytes += total;
+    used_ += total;
+    s->dev.mon = (Monitor *)mem;
+    dirty_ = true;
+    return IF_OK;
+}
+
+// The records written since the last read, oldest first (at most cap of
+// them; older ones beyond the ring's capacity were overwritten: *dropped).
+if_status Engine::monitor_read(int k, double *out, size_t cap, size_t *nrec, size_t *dropped)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "monitor_read: empty slot");
+    if (!s->mon_mem) return fail(IF_ERR_STATE, "monitor_read: no monitors on this case");
+    Monitor m{};
+    IFF_HIP(hipStreamSynchronize(st_));
+    IFF_HIP(hipMemcpy(&m, s->mon_mem, sizeof(Monitor), hipMemcpyDeviceToHost));
+    unsigned long long cnt = 0;
+    IFF_HIP(hipMemcpy(&cnt, m.count, sizeof(cnt), hipMemcpyDeviceToHost));
+    const size_t stride = 3 + 3 * (size_t)m.nprobe;
+    const size_t held = (size_t)std::min<unsigned long long>(cnt, (unsigned long long)m.cap);
+    const size_t n = out ? std::min(held, cap) : held;
+    if (nrec) *nrec = n;
+    if (dropped) *dropped = (size_t)(cnt - held) + (held - n);
+    if (!out) return IF_OK;
+    std::vector<double> ring((size_t)m.cap * stride);
+    IFF_HIP(hipMemcpy(ring.data(), m.ring, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // oldest held record: index cnt - held; the newest n of them are returned
+    const unsigned long long first = cnt - n;
+    for (size_t r = 0; r < n; r++) {
+        const size_t slotr = (size_t)((first + r) % (unsigned long long)m.cap);
+        std::memcpy(out + r * stride, ring.data() + slotr * stride, stride * sizeof(double));
+    }
+    const unsigned long long zero = 0;
+    IFF_HIP(hipMemcpy(m.count, &zero, sizeof(zero), hipMemcpyHostToDevice));
     return IF_OK;
 }
 

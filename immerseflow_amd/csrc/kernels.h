@@ -27,6 +27,7 @@ void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, co
                       int nt, int K, int M);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);
+void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt);
 void launch_tune_arm(hipStream_t st, const CaseDev *cs, int ncase, const Ctrl *ctrl0, const Ctrl *saved);
 
 // immersed-boundary setup (geometry.hip); fresh != 0: only moving cases,

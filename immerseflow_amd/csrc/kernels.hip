@@ -984,6 +984,107 @@ void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *til
         hipLaunchKernelGGL(k_project<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
 }
 
+// ---------------------------------------------------------------------------
+// Monitors: the force the fluid exerts on the bodies, summed over the
+// blanked faces (a fluid cell's face toward a non-fluid cell), and u, v, p
+// at probe cells.  On a face with the body to the east the traction is
+// p (x) and mu (v_P - v_B) 2/dx (y); west: -p and the same shear; north:
+// p (y) and mu (u_P - u_B) 2/dy (x); south: -p and the same shear; each
+// times the face length.  p is the projection pressure, mu = 1/Re, the
+// body velocity that of the neighbouring body.  Per-tile sums, then a
+// per-case sum in tile order (deterministic).
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ __launch_bounds__(256) void k_monitor_part(const CaseDev *cs, const int2 *tiles, int ntiles,
+                                                      int ntiles_pad)
+{
+    TileCtx tc;
+    if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
+    const CaseDev &c = cs[tc.cid];
+    const Ctrl *ct = c.ctrl;
+    if (!c.mon || !ct->active) return;
+    const TS *u = fld<TS>(c.u[ct->ad_base]), *v = fld<TS>(c.v[ct->ad_base]);
+    const double *p = c.p[ct->ppe_base];
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
+    double fx = 0.0, fy = 0.0;
+    if (i <= nx - 2) {
+        const double dxi = c.x.d[i];
+        for (int r = threadIdx.x >> 6; r < c.tile_y; r += 4) {
+            const int j = 1 + tc.ty * c.tile_y + r;
+            if (j > ny - 2) break;
+            const int id = i + j * P;
+            const uint8_t f = c.flags[id];
+            if (!(f & F_FLUID)) continue;
+            const double dyj = c.y.d[j];
+            const double pc = p[id], uc = (double)u[id], vc = (double)v[id];
+            if (!(c.flags[id + 1] & F_FLUID)) {
+                fx += pc * dyj;
+                fy += c.mu * (vc - blank_vel(c, id + 1, id + 1, 1)) * 2.0 / dxi * dyj;
+            }
+            if (!(c.flags[id - 1] & F_FLUID)) {
+                fx -= pc * dyj;
+                fy += c.mu * (vc - blank_vel(c, id - 1, id - 1, 1)) * 2.0 / dxi * dyj;
+            }
+            if (!(c.flags[id + P] & F_FLUID)) {
+                fy += pc * dxi;
+                fx += c.mu * (uc - blank_vel(c, id + P, id + P, 0)) * 2.0 / dyj * dxi;
+            }
+            if (!(c.flags[id - P] & F_FLUID)) {
+                fy -= pc * dxi;
+                fx += c.mu * (uc - blank_vel(c, id - P, id - P, 0)) * 2.0 / dyj * dxi;
+            }
+        }
+    }
+    fx = block_sum(fx);
+    fy = block_sum(fy);
+    if (threadIdx.x == 0) {
+        c.mon->part[2 * tc.t] = fx;
+        c.mon->part[2 * tc.t + 1] = fy;
+    }
+}
+
+template <typename TS>
+__global__ void k_monitor_rec(const CaseDev *cs, int ncase)
+{
+    const CaseDev &c = cs[blockIdx.x];
+    const Ctrl *ct = c.ctrl;
+    if (!c.mon || !ct->active || threadIdx.x != 0) return;
+    Monitor &m = *c.mon;
+    const int nt = c.ntx * c.nty;
+    double fx = 0.0, fy = 0.0;
+    for (int k = 0; k < nt; k++) {
+        fx += m.part[2 * k];
+        fy += m.part[2 * k + 1];
+    }
+    const int stride = 3 + 3 * m.nprobe;
+    const unsigned long long n = *m.count;
+    double *rec = m.ring + (size_t)(n % (unsigned long long)m.cap) * stride;
+    rec[0] = ct->t + c.dt;  // the time this step ends at
+    rec[1] = fx;
+    rec[2] = fy;
+    const TS *u = fld<TS>(c.u[ct->ad_base]), *v = fld<TS>(c.v[ct->ad_base]);
+    const double *p = c.p[ct->ppe_base];
+    for (int k = 0; k < m.nprobe; k++) {
+        const int id = m.pi[k] + m.pj[k] * c.P;
+        rec[3 + 3 * k] = (double)u[id];
+        rec[4 + 3 * k] = (double)v[id];
+        rec[5 + 3 * k] = p[id];
+    }
+    *m.count = n + 1;
+}
+
+void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt)
+{
+    if (prec == IF_FP64) {
+        hipLaunchKernelGGL(k_monitor_part<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+        hipLaunchKernelGGL(k_monitor_rec<double>, dim3(ncase), dim3(64), 0, st, cs, ncase);
+    } else {
+        hipLaunchKernelGGL(k_monitor_part<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
+        hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase);
+    }
+}
+
 // Launch-shape timing: reset every running case's control block to the
 // saved copy (saved mirrors the control array that starts at ctrl0) and
 // re-open its Poisson loop, so every timed launch sees the same state.

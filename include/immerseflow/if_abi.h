@@ -224,6 +224,23 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * the tile height or the sweeps per launch. */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
+/* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
+ * body-force diagnostic with no counterpart there): after every step the
+ * device records t, the force (Fx, Fy) the fluid exerts on the bodies and
+ * u, v, p at each probe, into a ring of `capacity` records.  A probe is the
+ * interior cell whose centre is nearest (x, y) per axis, as the
+ * predecessor's minloc (:189-194).  The force sums, over the faces between
+ * a fluid and a body cell, the pressure times the face length and the wall
+ * shear mu (u_P - u_body) 2/d times the face length (mu = 1/Re; d the
+ * cell width normal to the face).  nprobe <= 32; xy holds x0 y0 x1 y1 ...
+ * Calling it again replaces the probes and empties the ring. */
+if_status if_case_monitor(if_case *c, int nprobe, const double *xy, int capacity);
+/* The records since the last read, oldest first, each 3 + 3*nprobe doubles
+ * (t, Fx, Fy, then u, v, p per probe); at most cap_records are copied (the
+ * newest), *dropped counts older ones lost to the ring or to cap_records.
+ * out may be NULL to query *nrec only (the ring is then kept). */
+if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size_t *nrec,
+                               size_t *dropped);
 /* Autotuner (no counterpart in the reference, whose loop is fixed):
  * times every Poisson sweep launch shape of if_case_tune on the case's
  * current fields (a few launches each, on the solver's scratch buffers;

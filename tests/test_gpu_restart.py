@@ -63,3 +63,58 @@ def test_restart_from_reference_outputs(require_gpu, tmp_path):
         assert rel(b.get(f), c.get(f)) < 1e-4, f
     for x in (a, b, c):
         x.close()
+
+
+# ---------------------------------------------------------------------------
+# monitors: probes and body force
+# ---------------------------------------------------------------------------
+def _force_numpy(xc, yc, dx, dy, flags, u, v, p, mu):
+    """The monitor's force rule restated on the host from the fields."""
+    nx, ny = len(xc), len(yc)
+    F = flags.reshape(ny, nx)
+    U, V, Pp = u.reshape(ny, nx), v.reshape(ny, nx), p.reshape(ny, nx)
+    fluid = (F & 1) != 0
+    fx = fy = 0.0
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            if not fluid[j, i]:
+                continue
+            if not fluid[j, i + 1]:
+                fx += Pp[j, i] * dy[j]
+                fy += mu * V[j, i] * 2.0 / dx[i] * dy[j]
+            if not fluid[j, i - 1]:
+                fx -= Pp[j, i] * dy[j]
+                fy += mu * V[j, i] * 2.0 / dx[i] * dy[j]
+            if not fluid[j + 1, i]:
+                fy += Pp[j, i] * dx[i]
+                fx += mu * U[j, i] * 2.0 / dy[j] * dx[i]
+            if not fluid[j - 1, i]:
+                fy -= Pp[j, i] * dx[i]
+                fx += mu * U[j, i] * 2.0 / dy[j] * dx[i]
+    return fx, fy
+
+
+def test_monitor_probes_and_force(require_gpu):
+    print("[gpu-test] monitors", flush=True)
+    xf, yf, kw = _kw(iff.IF_FP64)
+    c = iff.Case(xf, yf, **kw)
+    probes = [(2.5, 1.0), (0.3, 1.7), (3.9, 0.05)]
+    c.monitor(probes, capacity=8)
+    c.step(3)
+    rec = c.monitor_read()
+    assert rec.shape == (3, 3 + 3 * len(probes))
+    assert np.allclose(rec[:, 0], kw["dt"] * np.arange(1, 4), rtol=0, atol=1e-15)
+    u, v, p = c.get("u"), c.get("v"), c.get("p")
+    nx = len(c.xc)
+    for k, (x, y) in enumerate(probes):
+        i = 1 + int(np.argmin(np.abs(c.xc[1:-1] - x)))
+        j = 1 + int(np.argmin(np.abs(c.yc[1:-1] - y)))
+        assert tuple(rec[-1, 3 + 3 * k:6 + 3 * k]) == (u[i + j * nx], v[i + j * nx], p[i + j * nx])
+    fx, fy = _force_numpy(c.xc, c.yc, c.dx, c.dy, c.get("flags").astype(np.uint8), u, v, p, 1.0 / kw["re"])
+    assert abs(rec[-1, 1] - fx) <= 1e-12 * abs(fx) and abs(rec[-1, 2] - fy) <= 1e-12 * max(abs(fy), abs(fx))
+    assert fx > 0  # drag along the free stream
+    assert c.monitor_read().shape[0] == 0  # a read empties the ring
+    c.step(10)  # more steps than the ring holds: the newest 8 are kept
+    rec = c.monitor_read()
+    assert rec.shape[0] == 8 and np.allclose(rec[:, 0], kw["dt"] * np.arange(6, 14), rtol=0, atol=1e-14)
+    c.close()
