@@ -13,7 +13,7 @@ import numpy as np
 from . import _abi
 from ._abi import (IF_FIELD_FLAGS, IF_FIELD_IBLANK, IF_FIELD_P, IF_FIELD_U,  # noqa: F401
                    IF_FIELD_UF, IF_FIELD_V, IF_FIELD_VF, IF_FP64, IF_MIXED,
-                   IF_MODE_COMPAT, IF_MODE_PHYSICS, IF_PPE_JACOBI, IF_PPE_RBSOR,
+                   IF_MODE_COMPAT, IF_MODE_PHYSICS, IF_PPE_JACOBI, IF_PPE_RBSOR, IF_PPE_LINE,
                    IF_BODY_CIRCLE, IF_BODY_ELLIPSE, IF_BODY_POLYGON, IF_MOTION_HEAVE)
 
 FIELDS = {"u": IF_FIELD_U, "v": IF_FIELD_V, "p": IF_FIELD_P, "uf": IF_FIELD_UF,
@@ -455,7 +455,7 @@ class ImmerseFlow:
                   conv_factor=I["conv_factor"], ad_itermax=I["ad_itermax"],
                   ppe_itermax=I["ppe_itermax"], ppe_omega=I["ppe_omega"],
                   ppe_tol=I["ppe_tol"], check_every=I["check_every"],
-                  ppe_solver=IF_PPE_JACOBI if I["ppe_solver"] == 0 else IF_PPE_RBSOR,
+                  ppe_solver={0: IF_PPE_JACOBI, 2: IF_PPE_LINE}.get(I["ppe_solver"], IF_PPE_RBSOR),
                   ad_tol=1e-6, bodies=I["bodies"])
         self.case = Case(self.xf, self.yf, **kw)
         return self.case

@@ -10,7 +10,7 @@ IF_ERR_ARG, IF_ERR_IO, IF_ERR_PARSE, IF_ERR_HIP = -1, -2, -3, -4
 IF_ERR_NOMEM, IF_ERR_STATE, IF_ERR_NODEV, IF_ERR_BUDGET, IF_ERR_DIVERGED = -5, -6, -7, -8, -9
 IF_MODE_PHYSICS, IF_MODE_COMPAT = 0, 1
 IF_FP64, IF_MIXED = 0, 1
-IF_PPE_JACOBI, IF_PPE_RBSOR = 0, 1
+IF_PPE_JACOBI, IF_PPE_RBSOR, IF_PPE_LINE = 0, 1, 2
 IF_BODY_CIRCLE, IF_BODY_ELLIPSE, IF_BODY_POLYGON = 0, 1, 2
 IF_MOTION_NONE, IF_MOTION_HEAVE = 0, 1
 IF_FIELD_U, IF_FIELD_V, IF_FIELD_P, IF_FIELD_UF, IF_FIELD_VF, IF_FIELD_IBLANK, IF_FIELD_FLAGS = range(7)

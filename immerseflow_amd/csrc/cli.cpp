@@ -85,7 +85,7 @@ int main(int argc, char **argv)
     d.vbc = 0.0;
     d.ad_itermax = in->ad_itermax;
     d.ad_tol = 1e-6;  // ADSolver.cu:315
-    d.ppe_solver = in->ppe_solver == 0 ? IF_PPE_JACOBI : IF_PPE_RBSOR;
+    d.ppe_solver = in->ppe_solver == 0 ? IF_PPE_JACOBI : in->ppe_solver == 2 ? IF_PPE_LINE : IF_PPE_RBSOR;
     d.ppe_omega = in->ppe_omega;
     d.ppe_itermax = in->ppe_itermax;
     d.ppe_tol = in->ppe_tol;

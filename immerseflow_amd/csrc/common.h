@@ -131,6 +131,7 @@ struct CaseDev {
     int moving;            // bodies move: reclassify every step
     int prec;              // IF_FP64 / IF_MIXED
     int ppe_solver;
+    int ppe_m;             // sweeps per Poisson launch of this case's solver
     int ad_itermax, ppe_itermax, check_every;
     double dt, rdt, omega, ppe_tol, ad_tol, ubc, vbc;
     double mu;             // 1/Re

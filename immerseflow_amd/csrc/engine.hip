@@ -229,7 +229,7 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     if (!(d.dt > 0) || !(d.re > 0)) return fail(IF_ERR_ARG, "case: dt and Re must be > 0");
     if (d.nbody < 0 || d.nbody > IF_MAX_BODIES || (d.nbody > 0 && !d.bodies))
         return fail(IF_ERR_ARG, "case: bad body list");
-    if (d.ppe_solver != IF_PPE_JACOBI && d.ppe_solver != IF_PPE_RBSOR)
+    if (d.ppe_solver != IF_PPE_JACOBI && d.ppe_solver != IF_PPE_RBSOR && d.ppe_solver != IF_PPE_LINE)
         return fail(IF_ERR_ARG, "case: unknown Poisson solver");
     if (d.ad_itermax < 0 || d.ppe_itermax < 0) return fail(IF_ERR_ARG, "case: negative itermax");
     s.desc = d;
@@ -565,6 +565,7 @@ if_status Engine::refresh()
     const int sty = sweep_ty;
     for (int k = 0; k < n; k++) {
         Slot &s = *slots_[order_[k]];
+        s.dev.ppe_m = s.dev.ppe_solver == IF_PPE_RBSOR ? std::max(1, sweep_m) : 1;
         cs[k] = s.dev;
         for (int t = 0; t < s.ntx * s.nty; t++) tb.push_back(make_int2(k, t));
         const int ntys = (s.NY + sty - 1) / sty;
@@ -653,9 +654,15 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         if (order_.empty()) break;
         const int n = (int)order_.size();
         const int K = std::max(1, check_k);
-        bool jac = false;
-        for (int id : order_) jac = jac || slots_[id]->dev.ppe_solver == IF_PPE_JACOBI;
-        const int m_eff = jac ? 1 : M;
+        // Poisson solvers present (each kernel skips the other solvers' cases)
+        bool any_rb = false, any_jac = false, any_line = false;
+        for (int id : order_) {
+            const int sv = slots_[id]->dev.ppe_solver;
+            any_rb = any_rb || sv == IF_PPE_RBSOR;
+            any_jac = any_jac || sv == IF_PPE_JACOBI;
+            any_line = any_line || sv == IF_PPE_LINE;
+        }
+        const int m_eff = any_rb ? M : 1;  // launches per chunk: K / m_eff (rounded up)
         if (any_moving_) {
             if_status rc = update_moving();
             if (rc != IF_OK) return rc;
@@ -702,30 +709,32 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         int timed = 0;
         rc = spec(nunits, 1, [&](int u) {
             if (u == 0) {
-                launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 0, m_eff);
+                launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
                 return;
             }
             const bool tm = timing && (u - 1) < 64;
             if (tm) hipEventRecord(tev_[2 * (u - 1)], st_);
-            for (int s = 0; s < K; s += m_eff) {
-                if (sweep_kind >= 2 && !jac)
-                    launch_ppe_sweep_reg(st_, prec_, sweep_kind == 3 ? 8 : 4, tiles_sweep_ty_, m_eff, d_cases_,
-                                         d_tiles_sweep_, nt_sweep_, s, K);
-                else if (sweep_kind == 1 && !jac)
-                    launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, m_eff, d_cases_, d_tiles_sweep_,
-                                            nt_sweep_, s, K);
-                else if (jac)  // point Jacobi runs on the base tiling
-                    launch_ppe_sweep(st_, prec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_,
-                                     nt_base_, s, K);
-                else
-                    launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, tiles_sweep_ty_, m_eff, d_cases_,
-                                     d_tiles_sweep_, nt_sweep_, s, K);
+            for (int s = 0; s < K; s++) {
+                if (any_rb && s % M == 0) {
+                    if (sweep_kind >= 2)
+                        launch_ppe_sweep_reg(st_, prec_, sweep_kind == 3 ? 8 : 4, tiles_sweep_ty_, M, d_cases_,
+                                             d_tiles_sweep_, nt_sweep_, s, K);
+                    else if (sweep_kind == 1)
+                        launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
+                                                nt_sweep_, s, K);
+                    else
+                        launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
+                                         nt_sweep_, s, K);
+                }
+                if (any_jac)  // point Jacobi runs on the base tiling
+                    launch_ppe_sweep(st_, prec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
+                if (any_line) launch_line_sor(st_, prec_, d_cases_, n, max_ring_, s, K);
             }
             if (tm) {
                 hipEventRecord(tev_[2 * (u - 1) + 1], st_);
                 timed = u;
             }
-            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, K, m_eff);
+            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, K);
         });
         if (rc != IF_OK) return rc;
 
@@ -810,7 +819,7 @@ if_status Engine::autotune(if_tune_result *out)
     }
     if (order_.empty()) return fail(IF_ERR_STATE, "autotune: no running case");
     for (int id : order_)
-        if (slots_[id]->dev.ppe_solver == IF_PPE_JACOBI)
+        if (slots_[id]->dev.ppe_solver != IF_PPE_RBSOR)
             return fail(IF_ERR_STATE, "autotune: red-black SOR cases only");
     std::vector<Ctrl> ctl;
     if_status rc = read_ctrl_sync(ctl);
@@ -901,7 +910,7 @@ if_status Engine::autotune(if_tune_result *out)
         (void)hipEventRecord(e0, st_);
         for (int r = 0; r < kReps; r++) {
             launch_tune_arm(st_, d_cases_, n, d_ctrl_, saved);
-            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 2 * best.m, best.m);
+            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 2 * best.m);
         }
         (void)hipEventRecord(e1, st_);
         float ms = 0.f;

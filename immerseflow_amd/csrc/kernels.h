@@ -24,7 +24,9 @@ int reg_out_cols(int m);
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                           const int2 *tiles, int nt, int s, int K);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
-                      int nt, int K, int M);
+                      int nt, int K);
+// zebra line SOR, one sweep (both colours) at chunk position s
+void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny, int s, int K);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);
 void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt);

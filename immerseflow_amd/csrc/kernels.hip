@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
     const CaseDev &c = cs[tc.cid];
     const Ctrl *ct = c.ctrl;
-    if (!ct->active || ct->ppe_done) return;
+    if (c.ppe_solver != IF_PPE_RBSOR || !ct->active || ct->ppe_done) return;
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
     if (s >= kexec) return;
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
     const CaseDev &c = cs[tc.cid];
     const Ctrl *ct = c.ctrl;
-    if (!ct->active || ct->ppe_done) return;
+    if (c.ppe_solver != IF_PPE_JACOBI || !ct->active || ct->ppe_done) return;
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
     if (s >= kexec) return;
@@ -620,10 +620,114 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
     }
 }
 
+// Zebra line SOR (the deck's PPE_Solver 2, inputs/inputs.txt:14-15), one
+// colour of rows per launch: rows j odd (colour 1) first, then rows j even.
+// One thread per row solves the row's tridiagonal system (Thomas algorithm)
+// -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs for each fluid
+// cell with sa != 0 (identity rows elsewhere), then relaxes, p + om (x - p).
+// The forward-elimination coefficients and x go to the case's predictor
+// source arrays (free during the Poisson phase).  Reads the unknown from
+// buffer src (own row, and for colour 1 the rows above and below), writes
+// dst; colour 0 reads its neighbour rows from dst (just updated).  FP64 is
+// the oracle's expression and order (line_sor_sweep, oracle.c).
+template <typename T>
+__global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    const Ctrl *ct = c.ctrl;
+    if (c.ppe_solver != IF_PPE_LINE || !ct->active || ct->ppe_done) return;
+    int kexec = c.ppe_itermax - ct->ppe_sweeps;
+    if (kexec > K) kexec = K;
+    if (s >= kexec) return;
+    const int nx = c.nx, ny = c.ny, P = c.P;
+    const int j = 2 - colour + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j > ny - 2) return;
+    const T *src;
+    T *dst;
+    const T *rhs;
+    const T *cxp, *cxm, *cyp, *cym;
+    bool zero_src = false;
+    T om;
+    if constexpr (sizeof(T) == 8) {
+        src = c.p[(ct->ppe_base + s) & 1];
+        dst = c.p[(ct->ppe_base + s + 1) & 1];
+        rhs = c.rhs;
+        cxp = c.x.cp; cxm = c.x.cm; cyp = c.y.cp; cym = c.y.cm;
+        om = c.omega;
+    } else {
+        src = c.e[s & 1];
+        dst = c.e[(s + 1) & 1];
+        zero_src = (s == 0);
+        rhs = c.r32;
+        cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
+        om = c.omega_f;
+    }
+    const T *nbr = colour == 1 ? src : dst;  // rows above and below
+    const bool zero_nbr = colour == 1 && zero_src;
+    T *cpv = fld<T>(c.sx), *dpv = fld<T>(c.sy);
+    const size_t row = (size_t)j * P;
+    const T yp = cyp[j], ym = cym[j];
+    for (int i = 1; i <= nx - 2; i++) {
+        const size_t id = row + i;
+        const uint8_t f = c.flags[id];
+        const T pold = zero_src ? (T)0 : src[id];
+        T l = (T)0, b = (T)1, u = (T)0, d = pold;
+        if (f & F_FLUID) {
+            const T aE = (f & F_OPEN_E) ? cxp[i] : (T)0;
+            const T aW = (f & F_OPEN_W) ? cxm[i] : (T)0;
+            const T aN = (f & F_OPEN_N) ? yp : (T)0;
+            const T aS = (f & F_OPEN_S) ? ym : (T)0;
+            const T sa = (aE + aW) + (aN + aS);
+            if (sa != (T)0) {
+                const T pN = zero_nbr ? (T)0 : nbr[id + P], pS = zero_nbr ? (T)0 : nbr[id - P];
+                l = -aW;
+                b = sa;
+                u = -aE;
+                d = (aN * pN + aS * pS) - rhs[id];
+            }
+        }
+        if (i == 1) {
+            cpv[id] = u / b;
+            dpv[id] = d / b;
+        } else {
+            const T m = b - l * cpv[id - 1];
+            cpv[id] = u / m;
+            dpv[id] = (d - l * dpv[id - 1]) / m;
+        }
+    }
+    for (int i = nx - 3; i >= 1; i--) dpv[row + i] = dpv[row + i] - cpv[row + i] * dpv[row + i + 1];
+    for (int i = 1; i <= nx - 2; i++) {
+        const size_t id = row + i;
+        const uint8_t f = c.flags[id];
+        const T pold = zero_src ? (T)0 : src[id];
+        T val = pold;
+        if (f & F_FLUID) {
+            const T aE = (f & F_OPEN_E) ? cxp[i] : (T)0;
+            const T aW = (f & F_OPEN_W) ? cxm[i] : (T)0;
+            const T aN = (f & F_OPEN_N) ? yp : (T)0;
+            const T aS = (f & F_OPEN_S) ? ym : (T)0;
+            if ((aE + aW) + (aN + aS) != (T)0) val = pold + om * (dpv[id] - pold);
+        }
+        dst[id] = val;
+    }
+}
+
+void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny, int s, int K)
+{
+    const int rows = (max_ny - 1) / 2;  // rows of one colour, at most
+    const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)ncase);
+    for (int colour = 1; colour >= 0; colour--) {
+        if (prec == IF_FP64)
+            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(64), 0, st, cs, s, K, colour);
+        else
+            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(64), 0, st, cs, s, K, colour);
+    }
+}
+
 // FP64 residual r = rhs - A p (max norm per tile) after a chunk of K sweeps
 // done in launches of M sweeps (K = 0: the check before any sweep).
 __global__ __launch_bounds__(256) void k_ppe_resid(const CaseDev *cs, const int2 *tiles,
-                                                   int ntiles, int ntiles_pad, int K, int M)
+                                                   int ntiles, int ntiles_pad, int K)
 {
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
@@ -632,7 +736,7 @@ __global__ __launch_bounds__(256) void k_ppe_resid(const CaseDev *cs, const int2
     if (!ct->active || ct->ppe_done) return;
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
-    const int nl = (kexec + M - 1) / M;
+    const int nl = (kexec + c.ppe_m - 1) / c.ppe_m;  // launches of the chunk
     const double *p = c.p[(ct->ppe_base + nl) & 1];
     const int nx = c.nx, ny = c.ny, P = c.P;
     const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
@@ -664,8 +768,7 @@ __global__ __launch_bounds__(256) void k_ppe_resid(const CaseDev *cs, const int2
 // then the FP64 residual of p_new, stored as FP32 for the next chunk
 // (defect correction / iterative refinement).  K = 0: residual only.
 __global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, const int2 *tiles,
-                                                          int ntiles, int ntiles_pad, int K,
-                                                          int M)
+                                                          int ntiles, int ntiles_pad, int K)
 {
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
@@ -674,7 +777,7 @@ __global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, con
     if (!ct->active || ct->ppe_done) return;
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
-    const int nl = (kexec + M - 1) / M;
+    const int nl = (kexec + c.ppe_m - 1) / c.ppe_m;  // launches of the chunk
     const bool upd = kexec > 0;
     const double *p = c.p[ct->ppe_base];
     double *pn = c.p[1 - ct->ppe_base];
@@ -722,8 +825,7 @@ __global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, con
 // One block per case: the Poisson stopping rule on the device.
 // nl_rule: 0 = FP64 ping-pong (base advances by the launch count),
 //          1 = mixed (base flips once when an update happened).
-__global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int ncase, int K,
-                                                      int M, int mixed)
+__global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int ncase, int K, int mixed)
 {
     const CaseDev &c = cs[blockIdx.x];
     Ctrl *ct = c.ctrl;
@@ -735,7 +837,7 @@ __global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int nca
     if (threadIdx.x == 0) {
         int kexec = c.ppe_itermax - ct->ppe_sweeps;
         if (kexec > K) kexec = K;
-        const int nl = (kexec + M - 1) / M;
+        const int nl = (kexec + c.ppe_m - 1) / c.ppe_m;  // launches of the chunk
         ct->ppe_sweeps += kexec;
         ct->tot_ppe += kexec;
         if (mixed) {
@@ -965,15 +1067,14 @@ void launch_ppe_sweep(hipStream_t st, int prec, int solver, int ty, int m, const
 }
 
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
-                      int nt, int K, int M)
+                      int nt, int K)
 {
     if (prec == IF_FP64)
-        hipLaunchKernelGGL(k_ppe_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt), K, M);
+        hipLaunchKernelGGL(k_ppe_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt), K);
     else
         hipLaunchKernelGGL(k_ppe_update_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
-                           tile_pad(nt), K, M);
-    hipLaunchKernelGGL(k_ppe_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase, K, M,
-                       prec == IF_FP64 ? 0 : 1);
+                           tile_pad(nt), K);
+    hipLaunchKernelGGL(k_ppe_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase, K, prec == IF_FP64 ? 0 : 1);
 }
 
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt)

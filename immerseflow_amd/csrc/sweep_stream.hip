@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
     const int2 tl = tiles[t];
     const CaseDev &c = cs[tl.x];
     const Ctrl *ct = c.ctrl;
-    if (!ct->active || ct->ppe_done) return;
+    if (c.ppe_solver != IF_PPE_RBSOR || !ct->active || ct->ppe_done) return;
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
     if (s >= kexec) return;

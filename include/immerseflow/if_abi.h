@@ -46,7 +46,7 @@ enum {
 
 enum { IF_MODE_PHYSICS = 0, IF_MODE_COMPAT = 1 };
 enum { IF_FP64 = 0, IF_MIXED = 1 };
-enum { IF_PPE_JACOBI = 0, IF_PPE_RBSOR = 1 };
+enum { IF_PPE_JACOBI = 0, IF_PPE_RBSOR = 1, IF_PPE_LINE = 2 };
 enum { IF_BODY_CIRCLE = 0, IF_BODY_ELLIPSE = 1, IF_BODY_POLYGON = 2 };
 enum { IF_MOTION_NONE = 0, IF_MOTION_HEAVE = 1 };
 
@@ -105,7 +105,7 @@ typedef struct {
     double ubc, vbc;        /* Dirichlet velocity on the four sides */
     int ad_itermax;
     double ad_tol;          /* L1 sum of |dq| (ADSolver.cu:315), 1e-6 */
-    int ppe_solver;         /* IF_PPE_JACOBI / IF_PPE_RBSOR */
+    int ppe_solver;         /* IF_PPE_JACOBI / IF_PPE_RBSOR / IF_PPE_LINE (zebra line SOR) */
     double ppe_omega;
     int ppe_itermax;
     double ppe_tol;

@@ -822,6 +822,71 @@ double of_phys_residual(int nx, int ny, const double *dx, const double *dy,
     return m;
 }
 
+/* Zebra line SOR, the deck's PPE_Solver 2 "Line SOR" (inputs.txt:14-15):
+ * rows j odd first, then rows j even; each row's cells are solved together
+ * (tridiagonal in i, the Thomas algorithm) with the rows above and below
+ * held, then relaxed, p <- p + om (x - p).  Row equation of a fluid cell
+ * with sa = (aE + aW) + (aN + aS) != 0:
+ *     -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs;
+ * any other cell keeps its value (an identity row). */
+static void line_sor_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
+                           const double *rhs, double om, double *p)
+{
+    for (int colour = 1; colour >= 0; colour--) {
+#pragma omp parallel
+        {
+            double *cp = malloc(sizeof(double) * nx), *dp = malloc(sizeof(double) * nx);
+            double *x = malloc(sizeof(double) * nx);
+#pragma omp for schedule(static)
+            for (int j = 1; j < ny - 1; j++) {
+                if ((j & 1) != colour) continue;
+                for (int i = 1; i < nx - 1; i++) {
+                    const int id = IDX(i, j, nx);
+                    const uint8_t f = flags[id];
+                    double l = 0.0, b = 1.0, u = 0.0, d = p[id];
+                    if (f & OF_FLUID) {
+                        const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+                        const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+                        const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+                        const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+                        const double sa = (aE + aW) + (aN + aS);
+                        if (sa != 0.0) {
+                            l = -aW;
+                            b = sa;
+                            u = -aE;
+                            d = (aN * p[id + nx] + aS * p[id - nx]) - rhs[id];
+                        }
+                    }
+                    if (i == 1) {
+                        cp[i] = u / b;
+                        dp[i] = d / b;
+                    } else {
+                        const double m = b - l * cp[i - 1];
+                        cp[i] = u / m;
+                        dp[i] = (d - l * dp[i - 1]) / m;
+                    }
+                }
+                x[nx - 2] = dp[nx - 2];
+                for (int i = nx - 3; i >= 1; i--) x[i] = dp[i] - cp[i] * x[i + 1];
+                for (int i = 1; i < nx - 1; i++) {  /* relax the solved cells only */
+                    const int id = IDX(i, j, nx);
+                    const uint8_t f = flags[id];
+                    if (f & OF_FLUID) {
+                        const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+                        const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+                        const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+                        const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+                        if ((aE + aW) + (aN + aS) != 0.0) p[id] = p[id] + om * (x[i] - p[id]);
+                    }
+                }
+            }
+            free(cp);
+            free(dp);
+            free(x);
+        }
+    }
+}
+
 /* Red-black SOR sweep (colour (i+j)&1 == 0 first), the parallel form of the
  * deck's PPE_Solver 1 "Point GS" with relaxation w-PPE (inputs.txt:14-15);
  * weighted Jacobi (jacobiIteration, PPESolver.cu:13-31, with the RHS and
@@ -853,6 +918,10 @@ static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
 #pragma omp parallel for schedule(static)
         for (int j = 1; j < ny - 1; j++)
             for (int i = 1; i < nx - 1; i++) p[IDX(i, j, nx)] = tmp[IDX(i, j, nx)];
+        return;
+    }
+    if (prm->ppe_solver == 2) {
+        line_sor_sweep(c, nx, ny, flags, rhs, om, p);
         return;
     }
     for (int colour = 0; colour < 2; colour++) {

@@ -132,7 +132,7 @@ typedef struct {
     double ubc, vbc;      /* Dirichlet velocity on the four domain sides */
     int ad_itermax;
     double ad_tol;        /* L1 sum of |dq| (ADSolver.cu:315) */
-    int ppe_solver;       /* 0 = Jacobi, 1 = red-black SOR */
+    int ppe_solver;       /* 0 = Jacobi, 1 = red-black SOR, 2 = zebra line SOR */
     double ppe_omega;
     int ppe_itermax;
     double ppe_tol;       /* stop when dt*||b - A p||_inf <= ppe_tol */

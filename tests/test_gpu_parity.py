@@ -231,6 +231,50 @@ def test_weighted_jacobi_matches_oracle(require_gpu):
     gc.close()
 
 
+@pytest.mark.parametrize("name", ["circle-stretched", "stars"])
+def test_line_sor_matches_oracle(require_gpu, name):
+    """The deck's PPE_Solver 2 (zebra line SOR): FP64 equal to the oracle
+    sweep for sweep; mixed within 1e-4 of it."""
+    xf, yf, bodies = CASES[name]
+    _say("line SOR", name)
+    oc, gc = _pair(xf, yf, bodies, dt=0.01, ppe_solver=2, ppe_omega=1.5, ppe_tol=1e-9, check_every=7)
+    for _ in range(3):
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["ppe_sweeps"] == o[2] and g["ad_iters"] == o[0]
+        _compare_fields(oc, gc, 1e-10)
+    gc.close()
+    oc, gm = _pair(xf, yf, bodies, prec=iff.IF_MIXED, dt=0.01, ppe_solver=2, ppe_omega=1.5, ppe_tol=1e-9,
+                   check_every=7)
+    oc.step(2)
+    gm.step(2)
+    _compare_fields(oc, gm, 1e-4)
+    gm.close()
+
+
+def test_batch_mixes_poisson_solvers(require_gpu):
+    """Jacobi, red-black SOR and line SOR cases in one batch: each result
+    equals the same case alone (the kernels skip the other solvers' cases,
+    the checks use each case's own sweeps per launch)."""
+    xf, yf, bodies = CASES["circle-stretched"]
+    _say("batch of three Poisson solvers")
+    specs = [(0, 0.8), (1, 1.8), (2, 1.5)]
+    kw = dict(dt=0.01, re=100.0, bodies=bodies, ppe_tol=1e-8, check_every=12)
+    for prec in (iff.IF_FP64, iff.IF_MIXED):
+        mix = iff.Batch(prec)
+        slots = [mix.add(xf, yf, 2, ppe_solver=sv, ppe_omega=om, **kw) for sv, om in specs]
+        mix.step(2)
+        for k, (sv, om) in enumerate(specs):
+            alone = iff.Batch(prec)
+            sl = alone.add(xf, yf, 2, ppe_solver=sv, ppe_omega=om, **kw)
+            alone.step(2)
+            assert alone.state(sl)[2]["ppe_sweeps"] == mix.state(slots[k])[2]["ppe_sweeps"]
+            for f in ("u", "v", "p"):
+                assert np.array_equal(alone.get(sl, f), mix.get(slots[k], f)), (prec, sv, f)
+            alone.close()
+        mix.close()
+
+
 def test_mixed_precision_against_fp64_oracle(require_gpu):
     xf, yf, bodies = CASES["circle-stretched"]
     _say("mixed precision")

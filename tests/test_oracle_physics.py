@@ -124,7 +124,7 @@ def _gauge(p, cells, dx, dy, nx):
     return v - np.sum(w * v) / np.sum(w)
 
 
-@pytest.mark.parametrize("solver,omega", [(1, 1.7), (0, 0.8)])
+@pytest.mark.parametrize("solver,omega", [(1, 1.7), (0, 0.8), (2, 1.5)])
 def test_poisson_iteration_matches_direct_solve(solver, omega):
     xf = stretched_faces(20, 2.0, 1.04)
     yf = stretched_faces(16, 1.6, 1.02)
@@ -160,6 +160,52 @@ def test_poisson_iteration_matches_direct_solve(solver, omega):
     a = _gauge(p, cells, dx, dy, nx)
     b = _gauge(direct, cells, dx, dy, nx)
     assert np.max(np.abs(a - b)) <= 1e-8 * max(1.0, np.max(np.abs(b)))
+
+
+def test_line_sor_solves_each_row_exactly():
+    """One zebra line-SOR sweep at om = 1: the rows solved last (even j)
+    satisfy their own row equations, -aW x[i-1] + sa x[i] - aE x[i+1] =
+    aN pN + aS pS - rhs, to round-off -- the tridiagonal solve is exact
+    (coefficients restated from the operator's definition, PPESolver.cu:93-99)."""
+    xf = stretched_faces(23, 2.0, 1.03)
+    yf = stretched_faces(14, 1.6, 0.98)
+    xc, dx = O.metric(xf)
+    yc, dy = O.metric(yf)
+    nx, ny = len(xc), len(yc)
+    bs = O.BodySet([dict(kind=O.CIRCLE, xc=0.9, yc=0.8, r=0.3)])
+    flags, _ = O.classify(xc, yc, bs)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal(nx * ny)
+    p = rng.standard_normal(nx * ny)
+    p0 = p.copy()
+    pr = O.Params(**prm(dt=1.0, ppe_solver=2, ppe_omega=1.0, ppe_tol=0.0, ppe_itermax=1, check_every=1))
+    res = C.c_double()
+    assert O.lib().of_phys_poisson(nx, ny, O.dptr(dx), O.dptr(dy), O.u8ptr(flags), O.dptr(rhs),
+                                   C.byref(pr), O.dptr(p), C.byref(res)) == 1
+    cxp = [0.0] + [2 / (dx[i] * (dx[i] + dx[i + 1])) for i in range(1, nx - 1)] + [0.0]
+    cxm = [0.0] + [2 / (dx[i] * (dx[i] + dx[i - 1])) for i in range(1, nx - 1)] + [0.0]
+    cyp = [0.0] + [2 / (dy[j] * (dy[j] + dy[j + 1])) for j in range(1, ny - 1)] + [0.0]
+    cym = [0.0] + [2 / (dy[j] * (dy[j] + dy[j - 1])) for j in range(1, ny - 1)] + [0.0]
+    F, P, R, P0 = grid2(flags, nx, ny), grid2(p, nx, ny), grid2(rhs, nx, ny), grid2(p0, nx, ny)
+    worst, nsolved = 0.0, 0
+    for j in range(2, ny - 1, 2):
+        for i in range(1, nx - 1):
+            f = F[j, i]
+            aE = cxp[i] if f & O.OF_OPEN_E else 0.0
+            aW = cxm[i] if f & O.OF_OPEN_W else 0.0
+            aN = cyp[j] if f & O.OF_OPEN_N else 0.0
+            aS = cym[j] if f & O.OF_OPEN_S else 0.0
+            sa = (aE + aW) + (aN + aS)
+            if not (f & O.OF_FLUID) or sa == 0.0:
+                assert P[j, i] == P0[j, i]
+                continue
+            lhs = -aW * P[j, i - 1] + sa * P[j, i] - aE * P[j, i + 1]
+            r = lhs - ((aN * P[j + 1, i] + aS * P[j - 1, i]) - R[j, i])
+            worst = max(worst, abs(r) / (sa * max(1.0, abs(P[j, i]))))
+            nsolved += 1
+    assert nsolved > 50 and worst <= 1e-12
+    # the odd rows were solved first, against the old even rows, and then kept
+    assert not np.array_equal(P[1], P0[1])
 
 
 def test_poisson_second_order_on_manufactured_solution():
