@@ -203,12 +203,16 @@ static if_status tune(Engine &e, int kind, int ty, int m, int k)
         return fail(IF_ERR_ARG, "tune: LDS sweep needs tile_y 8/16/32 and 1/2/4 sweeps per launch");
     if (kind == 1 && !stream_variant_ok(ty, m))
         return fail(IF_ERR_ARG, "tune: streaming sweep needs strip height 16/32/64/128 and 1/2/4 sweeps");
-    if (kind == 2 && !reg_variant_ok(4, ty, m))
+    const int pr = e.precision();
+    if (kind == 2 && !reg_variant_ok(pr, 4, ty, m))
         return fail(IF_ERR_ARG, "tune: register-tile sweep needs tile_y 16/32/48 and 1/2/4 sweeps");
-    if (kind == 3 && !reg_variant_ok(8, ty, m))
-        return fail(IF_ERR_ARG, "tune: 8-wave register-tile sweep needs 2 sweeps with tile_y 24/40/56 "
-                                "or 4 sweeps with tile_y 32/48/64");
-    if (kind < 0 || kind > 3) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
+    if (kind == 3 && !reg_variant_ok(pr, 8, ty, m))
+        return fail(IF_ERR_ARG, "tune: 8-wave register-tile sweep needs 2 sweeps with tile_y 24/40/56, "
+                                "4 sweeps with tile_y 32/48/64 or 8 sweeps with tile_y 32/48/64");
+    if (kind == 4 && !reg_variant_ok(pr, 16, ty, m))
+        return fail(IF_ERR_ARG, "tune: 16-wave register-tile sweep needs 4 sweeps with tile_y 48 or 8 sweeps "
+                                "with tile_y 32/64 (96: mixed precision only)");
+    if (kind < 0 || kind > 4) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
     if (k < 1) return fail(IF_ERR_ARG, "tune: check_every must be >= 1");
     e.sweep_kind = kind;
     e.sweep_ty = ty;

@@ -60,10 +60,11 @@ public:
     int resident() const;
     size_t in_use() const { return used_; }
     size_t budget() const { return budget_; }
+    int precision() const { return prec_; }
     Slot *slot(int k) { return (k >= 0 && k < (int)slots_.size()) ? slots_[k].get() : nullptr; }
 
     // Poisson sweep variant and check cadence (autotuned or user-set)
-    int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3: register-tile (4/8 waves)
+    int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3/4: register-tile (4/8/16 waves)
     int sweep_ty = 32, sweep_m = 4, check_k = 10;
     bool timing = false;
     double sweep_ms = 0.0;

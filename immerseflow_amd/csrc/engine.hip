@@ -25,6 +25,9 @@ namespace iff {
 
 constexpr int kTileY = 16;  // base tiling (reductions, stencil passes)
 
+// waves per workgroup of the register-tile sweep kinds (2, 3, 4)
+static int reg_waves(int kind) { return kind == 4 ? 16 : kind == 3 ? 8 : 4; }
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget) {}
@@ -717,7 +720,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             for (int s = 0; s < K; s++) {
                 if (any_rb && s % M == 0) {
                     if (sweep_kind >= 2)
-                        launch_ppe_sweep_reg(st_, prec_, sweep_kind == 3 ? 8 : 4, tiles_sweep_ty_, M, d_cases_,
+                        launch_ppe_sweep_reg(st_, prec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_,
                                              d_tiles_sweep_, nt_sweep_, s, K);
                     else if (sweep_kind == 1)
                         launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
@@ -855,12 +858,14 @@ if_status Engine::autotune(if_tune_result *out)
     struct Shape {
         int kind, ty, m;
     };
-    static const Shape cands[] = {{3, 32, 4}, {3, 48, 4}, {3, 64, 4}, {3, 24, 2}, {3, 40, 2},
-                                  {2, 16, 4}, {2, 32, 4}, {2, 16, 2}, {0, 16, 2}, {0, 32, 4}};
+    // register tiles of 16, 8 and 4 waves at 8, 4 and 2 sweeps per launch,
+    // and the LDS-tiled kernel; shapes a precision lacks are skipped
+    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {3, 32, 8}, {3, 48, 8},
+                                  {3, 64, 8}, {3, 32, 4}, {3, 48, 4}, {3, 24, 2}, {2, 16, 4}, {0, 16, 2}};
     constexpr int kReps = 6;
     auto sweep_launch = [&](const Shape &c, int s, int K) {
         if (c.kind >= 2)
-            launch_ppe_sweep_reg(st_, prec_, c.kind == 3 ? 8 : 4, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
+            launch_ppe_sweep_reg(st_, prec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
                                  s, K);
         else if (c.kind == 1)
             launch_ppe_sweep_stream(st_, prec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
@@ -875,6 +880,7 @@ if_status Engine::autotune(if_tune_result *out)
     double best_ns = 1e300;
     int ncand = 0;
     for (const Shape &c : cands) {
+        if (c.kind >= 2 && !reg_variant_ok(prec_, reg_waves(c.kind), c.ty, c.m)) continue;
         sweep_kind = c.kind;
         sweep_ty = c.ty;
         sweep_m = c.m;

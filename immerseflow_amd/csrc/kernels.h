@@ -19,7 +19,7 @@ int stream_out_cols(int m);
 void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const CaseDev *cs,
                              const int2 *tiles, int nt, int s, int K);
 // register-tile red-black SOR (sweep_reg.hip)
-bool reg_variant_ok(int waves, int ty, int m);
+bool reg_variant_ok(int prec, int waves, int ty, int m);
 int reg_out_cols(int m);
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                           const int2 *tiles, int nt, int s, int K);

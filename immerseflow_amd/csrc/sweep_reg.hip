@@ -1,6 +1,6 @@
 // Register-tile red-black SOR for gfx950.
 //
-// A workgroup of NW (4 or 8) wave64s owns a tile of 128 columns by NW*NRW rows
+// A workgroup of NW (4, 8 or 16) wave64s owns a tile of 128 columns by NW*NRW rows
 // (halo H = 2M on every side included): lane l of every wave holds the
 // column pair (c0+2l, c0+2l+1) -- one red and one black cell of each row --
 // and wave w holds rows w*NRW .. w*NRW+NRW-1, all in registers.  In a
@@ -386,11 +386,17 @@ static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt
 }
 
 // Stored tile: 128 - 4M columns by tile_y rows; tile_y + 4M rows per
-// workgroup, split over its waves (4 or 8).
-bool reg_variant_ok(int waves, int ty, int m)
+// workgroup, split over its waves (4, 8 or 16).  M = 8 (8- and 16-wave
+// tiles) trades a wider halo for half the loads per sweep.  The FP64 tile
+// of 16 waves x 8 rows does not fit the 128 registers a 1024-lane
+// workgroup allows, so that shape is FP32 (mixed precision) only.
+bool reg_variant_ok(int prec, int waves, int ty, int m)
 {
     if (waves == 4) return (m == 1 || m == 2 || m == 4) && (ty == 16 || ty == 32 || ty == 48);
-    if (waves == 8) return (m == 2 && (ty == 24 || ty == 40 || ty == 56)) || (m == 4 && (ty == 32 || ty == 48 || ty == 64));
+    if (waves == 8)
+        return (m == 2 && (ty == 24 || ty == 40 || ty == 56)) || (m == 4 && (ty == 32 || ty == 48 || ty == 64)) ||
+               (m == 8 && (ty == 32 || ty == 48 || ty == 64));
+    if (waves == 16) return (m == 4 && ty == 48) || (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64)));
     return false;
 }
 
@@ -409,8 +415,15 @@ void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, co
     IFF_RG(T, 4, 2, 16) IFF_RG(T, 4, 2, 32) IFF_RG(T, 4, 2, 48)                           \
     IFF_RG(T, 4, 4, 16) IFF_RG(T, 4, 4, 32) IFF_RG(T, 4, 4, 48)                           \
     IFF_RG(T, 8, 2, 24) IFF_RG(T, 8, 2, 40) IFF_RG(T, 8, 2, 56)                           \
-    IFF_RG(T, 8, 4, 32) IFF_RG(T, 8, 4, 48) IFF_RG(T, 8, 4, 64)
-    if (prec == IF_FP64) { IFF_RG_ALL(double) } else { IFF_RG_ALL(float) }
+    IFF_RG(T, 8, 4, 32) IFF_RG(T, 8, 4, 48) IFF_RG(T, 8, 4, 64)                           \
+    IFF_RG(T, 8, 8, 32) IFF_RG(T, 8, 8, 48) IFF_RG(T, 8, 8, 64)                           \
+    IFF_RG(T, 16, 4, 48) IFF_RG(T, 16, 8, 32) IFF_RG(T, 16, 8, 64)
+    if (prec == IF_FP64) {
+        IFF_RG_ALL(double)
+    } else {
+        IFF_RG_ALL(float)
+        IFF_RG(float, 16, 8, 96)
+    }
 #undef IFF_RG_ALL
 #undef IFF_RG
 }
