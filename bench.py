@@ -35,12 +35,15 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--kernel", type=int, default=3,
-                    help="0 LDS-tiled, 1 register-streaming, 2/3 register-tile (4/8 waves)")
-    ap.add_argument("--tile-y", type=int, default=32)
-    ap.add_argument("--sweeps-per-launch", type=int, default=4)
-    ap.add_argument("--check-every", type=int, default=160)
+    ap.add_argument("--kernel", type=int, default=4,
+                    help="0 LDS-tiled, 1 register-streaming, 2/3/4 register-tile (4/8/16 waves)")
+    ap.add_argument("--tile-y", type=int, default=96)
+    ap.add_argument("--sweeps-per-launch", type=int, default=8)
+    ap.add_argument("--check-every", type=int, default=240)
+    ap.add_argument("--fp64-kernel", type=int, default=3)
     ap.add_argument("--fp64-tile-y", type=int, default=32)
+    ap.add_argument("--fp64-sweeps-per-launch", type=int, default=4)
+    ap.add_argument("--fp64-check-every", type=int, default=160)
     ap.add_argument("--omega", type=float, default=1.998)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2)
@@ -55,7 +58,7 @@ def kernel_name(kind, ty, m, t):
         return "k_rbsor<%s, %d, %d>" % (t, ty, m)
     if kind == 1:
         return "k_rbsor_stream<%s, %d, %d>" % (t, m, ty)
-    nw = 4 if kind == 2 else 8
+    nw = {2: 4, 3: 8, 4: 16}[kind]
     return "k_rbsor_reg<%s, %d, %d, %d>" % (t, m, nw, (ty + 4 * m) // nw)
 
 
@@ -63,12 +66,16 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_case(iff, cfg, precision, a, tile_y=None):
+def make_case(iff, cfg, precision, a):
+    if precision == iff.IF_FP64:
+        shape = (a.fp64_kernel, a.fp64_tile_y, a.fp64_sweeps_per_launch, a.fp64_check_every)
+    else:
+        shape = (a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
     c = iff.Case(cfg["xf"], cfg["yf"], precision=precision, dt=cfg["dt"], re=cfg["re"],
                  bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_RBSOR,
                  ppe_omega=a.omega, ppe_itermax=200000, ppe_tol=cfg["ppe_tol"],
-                 check_every=a.check_every)
-    c.tune(a.kernel, tile_y or a.tile_y, a.sweeps_per_launch, a.check_every)
+                 check_every=shape[3])
+    c.tune(*shape)
     return c
 
 
@@ -152,7 +159,7 @@ def batched_c4(iff, configs, a):
     sample = cases[::32]
     for c in sample:
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
-                     ppe_tol=1e-6, check_every=a.check_every)
+                     ppe_tol=1e-6, check_every=a.fp64_check_every)
         s.step(1)
         t0 = time.perf_counter()
         s.step(a.batch_steps)
@@ -239,7 +246,7 @@ def main():
 
     fp64 = None
     if not a.no_fp64 and a.fp64_steps > 0:
-        c64 = make_case(iff, cfg, iff.IF_FP64, a, a.fp64_tile_y)
+        c64 = make_case(iff, cfg, iff.IF_FP64, a)
         r64 = timed_run(c64, max(1, a.warmup), a.fp64_steps, cells, "fp64")
         b64, _ = c64.byte_model()
         avg64 = r64["sweep_ms"] / max(1, r64["sweep_launches"])

@@ -271,11 +271,12 @@ class Case:
         _check(_lib().if_case_ghosts(self._h, lst, recs, k, C.byref(n)))
         return np.array(lst[:k], dtype=np.int32), recs, k
 
-    def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
-        """Poisson sweep kernel (0 LDS-tiled, 1 register-streaming, 2/3
-        register-tile with 4/8 waves), tile or strip height, red-black
+    def tune(self, kernel, tile_y, sweeps_per_launch, check_every):
+        """Poisson sweep kernel (0 LDS-tiled, 1 register-streaming, 2/3/4
+        register-tile with 4/8/16 waves), tile or strip height, red-black
         sweeps per launch, sweeps per check.  Every kernel and shape gives
-        the same result."""
+        the same result.  Without a call the library uses 4/64/8 (mixed)
+        or 3/32/4 (FP64) and the case's check_every."""
         _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def restart(self, u, v, p, uf=None, vf=None):
@@ -409,7 +410,8 @@ class Batch:
         _check(_lib().if_batch_remove(self._h, slot), "remove")
         self._keep.pop(slot, None)
 
-    def tune(self, kernel=1, tile_y=32, sweeps_per_launch=2, check_every=10):
+    def tune(self, kernel, tile_y, sweeps_per_launch, check_every):
+        """As Case.tune, for every case of the batch."""
         _check(_lib().if_batch_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def autotune(self):

@@ -64,6 +64,7 @@ public:
     Slot *slot(int k) { return (k >= 0 && k < (int)slots_.size()) ? slots_[k].get() : nullptr; }
 
     // Poisson sweep variant and check cadence (autotuned or user-set)
+    // (FP64 defaults; the constructor sets 4, 64, 8 for mixed precision)
     int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3/4: register-tile (4/8/16 waves)
     int sweep_ty = 32, sweep_m = 4, check_k = 10;
     bool timing = false;

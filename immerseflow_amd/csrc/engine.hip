@@ -30,7 +30,17 @@ static int reg_waves(int kind) { return kind == 4 ? 16 : kind == 3 ? 8 : 4; }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget) {}
+Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget)
+{
+    // default Poisson launch shape, by measurement on the 2048 x 1024
+    // ellipse: FP32 sweeps are fastest on 16-wave tiles at 8 sweeps per
+    // launch; FP64 sweeps (IEEE division per cell) on 8-wave tiles at 4
+    if (prec != IF_FP64) {
+        sweep_kind = 4;
+        sweep_ty = 64;
+        sweep_m = 8;
+    }
+}
 
 Engine::~Engine()
 {

@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-from . import IF_MIXED, Batch
+from . import IF_FP64, IF_MIXED, Batch
 
 
 def _add(batch, c):
@@ -36,8 +36,8 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
     auto = tune == "auto"
     if not auto:
         batch.tune(*tune)
-    else:
-        batch.tune(3, 32, 4, 160)
+    else:   # until the autotuner has run: the library's measured default shape
+        batch.tune(*((3, 32, 4, 160) if precision == IF_FP64 else (4, 64, 8, 160)))
     tuned = None
     order = sorted(range(len(trace)), key=lambda k: trace[k][0])
     pending = collections.deque()

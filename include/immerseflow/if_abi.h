@@ -216,12 +216,15 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * height 8/16/32 cells; kernel 1: register-streaming red-black SOR (one
  * wave64 per column strip), strip height 16/32/64/128 rows; kernel 2:
  * register-tile red-black SOR (four wave64s hold a 128-column tile in
- * registers), tile height 16/32/48 rows; kernel 3: the same with eight
- * wave64s per tile, tile height 24/40/56 (2 sweeps per launch) or 32/48/64
- * (4 sweeps per launch).  Each does 1, 2 or 4 red-black
- * sweeps per launch (temporal blocking); check_every is the number of
- * sweeps between convergence checks.  Results do not depend on the kernel,
- * the tile height or the sweeps per launch. */
+ * registers), tile height 16/32/48 rows, 1, 2 or 4 sweeps per launch;
+ * kernel 3: the same with eight wave64s per tile, tile height 24/40/56
+ * (2 sweeps per launch), 32/48/64 (4) or 32/48/64 (8); kernel 4: sixteen
+ * wave64s per tile, tile height 48 (4 sweeps per launch) or 32/64/96 (8;
+ * 96 in mixed precision only).  Several red-black sweeps per launch is
+ * temporal blocking; check_every is the number of sweeps between
+ * convergence checks.  Results do not depend on the kernel, the tile
+ * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps
+ * (mixed); kernel 3, 32 rows, 4 sweeps (FP64). */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
