@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--fp64-sweeps-per-launch", type=int, default=4)
     ap.add_argument("--fp64-check-every", type=int, default=160)
     ap.add_argument("--omega", type=float, default=1.998)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the Poisson sweep kernels one by one instead of replaying a hipGraph")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
@@ -76,6 +78,7 @@ def make_case(iff, cfg, precision, a):
                  ppe_omega=a.omega, ppe_itermax=200000, ppe_tol=cfg["ppe_tol"],
                  check_every=shape[3])
     c.tune(*shape)
+    c.graphs(not a.no_graphs)
     return c
 
 
@@ -133,6 +136,7 @@ def batched_c4(iff, configs, a):
     cases = configs.c4_cases(256)
     b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
     b.tune(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
+    b.graphs(not a.no_graphs)
     t0 = time.perf_counter()
     slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
                    ppe_tol=1e-6, check_every=a.check_every) for c in cases]
@@ -160,6 +164,7 @@ def batched_c4(iff, configs, a):
     for c in sample:
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-6, check_every=a.fp64_check_every)
+        s.graphs(not a.no_graphs)
         s.step(1)
         t0 = time.perf_counter()
         s.step(a.batch_steps)
@@ -282,7 +287,8 @@ def main():
                    "precision": "mixed (FP32 stencils/sweeps, FP64 refinement)",
                    "poisson": "red-black SOR, stop max|div|<=%g" % cfg["ppe_tol"],
                    "sweeps_last_step": st["ppe_sweeps"], "ad_iters_last_step": st["ad_iters"],
-                   "max_div": st["max_div"], "parallelism": "single GPU"},
+                   "max_div": st["max_div"], "parallelism": "single GPU",
+                   "poisson_launch": "kernel launches" if a.no_graphs else "hipGraph replay per chunk"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64": fp64,

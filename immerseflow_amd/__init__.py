@@ -338,6 +338,11 @@ class Case:
         _check(_lib().if_case_autotune(self._h, C.byref(r)), "autotune")
         return _tune_dict(r)
 
+    def graphs(self, enable=True):
+        """Replay each chunk of Poisson sweeps from a captured hipGraph (the
+        default) or launch the kernels one by one; results are identical."""
+        _check(_lib().if_case_graphs(self._h, 1 if enable else 0), "graphs")
+
     def sweep_timing(self, reset=False):
         ms, n = C.c_double(), C.c_longlong()
         _check(_lib().if_case_sweep_timing(self._h, 1 if reset else 0, C.byref(ms), C.byref(n)))
@@ -413,6 +418,10 @@ class Batch:
     def tune(self, kernel, tile_y, sweeps_per_launch, check_every):
         """As Case.tune, for every case of the batch."""
         _check(_lib().if_batch_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
+
+    def graphs(self, enable=True):
+        """As Case.graphs."""
+        _check(_lib().if_batch_graphs(self._h, 1 if enable else 0), "graphs")
 
     def autotune(self):
         """As Case.autotune, over every running case of the batch."""

@@ -228,6 +228,13 @@ if_status if_case_tune(if_case *c, int kernel, int tile_y, int m, int check_ever
     return tune(*c->eng, kernel, tile_y, m, check_every);
 }
 
+if_status if_case_graphs(if_case *c, int enable)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "graphs: physics mode only");
+    c->eng->use_graphs = enable != 0;
+    return IF_OK;
+}
+
 static if_status case_autotune(if_case *c, if_tune_result *out)
 {
     // a single case is resident but idle between calls: open it for one
@@ -274,6 +281,13 @@ if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int m, int check_ev
 {
     if (!b) return fail(IF_ERR_ARG, "null batch");
     return tune(*b->eng, kernel, tile_y, m, check_every);
+}
+
+if_status if_batch_graphs(if_batch *b, int enable)
+{
+    if (!b) return fail(IF_ERR_ARG, "null batch");
+    b->eng->use_graphs = enable != 0;
+    return IF_OK;
 }
 
 if_status if_case_sweep_timing(if_case *c, int reset, double *ms, long long *launches)

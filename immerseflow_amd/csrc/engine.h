@@ -68,6 +68,8 @@ public:
     int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3/4: register-tile (4/8/16 waves)
     int sweep_ty = 32, sweep_m = 4, check_k = 10;
     bool timing = false;
+    // replay each chunk of Poisson sweep launches from a captured hipGraph
+    bool use_graphs = true;
     double sweep_ms = 0.0;
     long long sweep_launches = 0;
     double sweep_cells = 0.0;  // interior cell-sweeps inside the timed chunks
@@ -83,6 +85,10 @@ private:
     bool all_done(const Ctrl *snap, int which) const;
     if_status read_ctrl_sync(std::vector<Ctrl> &out);
     if_status update_moving();
+    // the K sweeps of one Poisson chunk, as kernel launches on st_
+    void enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line);
+    if_status capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line);
+    void drop_graph();
 
     int prec_;
     size_t budget_, used_ = 0;
@@ -105,6 +111,10 @@ private:
     Ctrl *h_snap_ = nullptr;           // pinned, kRing * cap_ctrl_
     hipEvent_t ev_[kRing] = {};
     hipEvent_t tev_[2 * 64] = {};      // timing events
+    // captured sweep chunk, valid for the resident set of the last
+    // refresh() and the launch shape in g_key_
+    hipGraphExec_t g_sweeps_ = nullptr;
+    int g_key_[7] = {};
 };
 
 }  // namespace iff

@@ -45,6 +45,7 @@ Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget)
 Engine::~Engine()
 {
     if (st_) hipStreamSynchronize(st_);
+    drop_graph();
     for (auto &s : slots_)
         if (s && s->arena) hipFree(s->arena);
     for (auto &s : slots_)
@@ -559,8 +560,63 @@ if_status Engine::sync()
 }
 
 // Re-upload the list of running cases and the tile tables.
+void Engine::drop_graph()
+{
+    if (g_sweeps_) hipGraphExecDestroy(g_sweeps_);
+    g_sweeps_ = nullptr;
+}
+
+// Kernel launches of one Poisson chunk (K sweeps; the red-black kernels do
+// M sweeps per launch).  Every argument of every launch is fixed for a
+// given resident set and shape: the kernels read the loop state (buffer
+// parity, sweep counts, done flags) from the control blocks on the device,
+// so a captured chunk replays unchanged.
+void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line)
+{
+    const int n = (int)order_.size();
+    for (int s = 0; s < K; s++) {
+        if (any_rb && s % M == 0) {
+            if (sweep_kind >= 2)
+                launch_ppe_sweep_reg(st_, prec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
+                                     nt_sweep_, s, K);
+            else if (sweep_kind == 1)
+                launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+            else
+                launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
+                                 s, K);
+        }
+        if (any_jac)  // point Jacobi runs on the base tiling
+            launch_ppe_sweep(st_, prec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
+        if (any_line) launch_line_sor(st_, prec_, d_cases_, n, max_ring_, s, K);
+    }
+}
+
+// Capture the chunk into a graph (kernel nodes only) unless the cached one
+// matches the shape.  refresh() drops the graph, since it re-uploads the
+// case list and tile tables the launches point at.
+if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line)
+{
+    const int key[7] = {K, M, sweep_kind, tiles_sweep_ty_, (int)any_rb, (int)any_jac, (int)any_line};
+    if (g_sweeps_ && std::equal(key, key + 7, g_key_)) return IF_OK;
+    drop_graph();
+    hipGraph_t g = nullptr;
+    IFF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    enqueue_sweeps(K, M, any_rb, any_jac, any_line);
+    hipError_t e = hipStreamEndCapture(st_, &g);
+    if (e != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g_sweeps_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        g_sweeps_ = nullptr;
+        return fail(IF_ERR_HIP, std::string("sweep graph instantiate: ") + hipGetErrorString(e));
+    }
+    std::copy(key, key + 7, g_key_);
+    return IF_OK;
+}
+
 if_status Engine::refresh()
 {
+    drop_graph();
     order_.clear();
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
@@ -720,6 +776,11 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;
         int timed = 0;
+        if (use_graphs) {
+            rc = capture_sweeps(K, M, any_rb, any_jac, any_line);
+            if (rc != IF_OK) return rc;
+        }
+        hipError_t glaunch = hipSuccess;
         rc = spec(nunits, 1, [&](int u) {
             if (u == 0) {
                 launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
@@ -727,21 +788,11 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             }
             const bool tm = timing && (u - 1) < 64;
             if (tm) hipEventRecord(tev_[2 * (u - 1)], st_);
-            for (int s = 0; s < K; s++) {
-                if (any_rb && s % M == 0) {
-                    if (sweep_kind >= 2)
-                        launch_ppe_sweep_reg(st_, prec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_,
-                                             d_tiles_sweep_, nt_sweep_, s, K);
-                    else if (sweep_kind == 1)
-                        launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
-                                                nt_sweep_, s, K);
-                    else
-                        launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
-                                         nt_sweep_, s, K);
-                }
-                if (any_jac)  // point Jacobi runs on the base tiling
-                    launch_ppe_sweep(st_, prec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
-                if (any_line) launch_line_sor(st_, prec_, d_cases_, n, max_ring_, s, K);
+            if (use_graphs) {
+                const hipError_t e = hipGraphLaunch(g_sweeps_, st_);
+                if (e != hipSuccess && glaunch == hipSuccess) glaunch = e;
+            } else {
+                enqueue_sweeps(K, M, any_rb, any_jac, any_line);
             }
             if (tm) {
                 hipEventRecord(tev_[2 * (u - 1) + 1], st_);
@@ -750,6 +801,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, K);
         });
         if (rc != IF_OK) return rc;
+        if (glaunch != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph launch: ") + hipGetErrorString(glaunch));
 
         // ---- pressure BC, projection, divergence, ghost cells
         launch_pressure_bc(st_, d_cases_, n, max_ring_, max_ng_);

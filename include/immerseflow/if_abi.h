@@ -227,6 +227,13 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * (mixed); kernel 3, 32 rows, 4 sweeps (FP64). */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
+/* Launch mode of the Poisson sweeps (no counterpart in the reference,
+ * whose PPESolver launches each Jacobi sweep from the host,
+ * PPESolver.cu:172-189).  enable = 1 (the default): each chunk of
+ * check_every sweeps is captured once into a hipGraph of kernel nodes and
+ * replayed, recaptured when the resident set or the launch shape changes;
+ * 0: the same kernels are launched one by one.  Results are identical. */
+if_status if_case_graphs(if_case *c, int enable);
 /* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
  * body-force diagnostic with no counterpart there): after every step the
  * device records t, the force (Fx, Fy) the fluid exerts on the bodies and
@@ -297,6 +304,8 @@ if_status if_batch_info(if_batch *b, int *resident, size_t *bytes_in_use,
 /* As if_case_tune, for every case of the batch. */
 if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int sweeps_per_launch,
                         int check_every);
+/* As if_case_graphs, for the batch. */
+if_status if_batch_graphs(if_batch *b, int enable);
 /* As if_case_autotune, over every running case of the batch at once (the
  * batched launches are what is timed; S is the median of the cases' last
  * sweep counts). */
