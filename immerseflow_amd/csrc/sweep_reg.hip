@@ -339,26 +339,70 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     }
 
     const int Z = (c0 + jt0) & 1;  // parity of slot a in the wave's first row
-    for (int h = 0; h < nst; h++) {
+    // Half-sweep h has colour h & 1, and slot a of row k has parity Z + k,
+    // so rows with (k & 1) == Q update slot a where Q = (Z ^ h) & 1.  A
+    // wave of fast-path rows (FP32: every wave) runs branch-free.
+    // FP32: each half-sweep first updates the rows that do not touch the
+    // neighbouring waves (hiding the LDS read of the neighbour rows), then
+    // the first and last row, and ends by publishing those two rows for
+    // the next half-sweep.  Within a half-sweep every updated cell reads
+    // only cells of the other colour, so the row order changes no result.
+    auto half32 = [&]<int Q>(int h) {
         const int b = h & 1;
-        xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
-        xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
-        __syncthreads();
         V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
         if (w > 0) sv = xch[b][w - 1][1][lane];
         if (w < NW - 1) nv = xch[b][w + 1][0][lane];
-        // half-sweep h has colour h & 1: slot a of row k has parity Z + k.
-        // A wave of interior fast-path rows (the common case) runs
-        // branch-free.
-#define IFF_HALF(Q_, F_)                                                                             \
-    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, \
-                             cxpb, cxmb, cxb, om)
-        if (sizeof(T) == 4 || smask == 0) {
-            if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+        [&]<int... K>(std::integer_sequence<int, K...>) {
+            (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxa, cxpb, cxmb, cxb, om),
+             ...);
+        }(std::make_integer_sequence<int, NRW - 2>{});
+        reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, cxpb, cxmb, cxb, om);
+        reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, cxpb, cxmb, cxb, om);
+        xch[b ^ 1][w][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[b ^ 1][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        __syncthreads();
+    };
+    // nst is even: the half-sweeps run in pairs whose colour order is fixed
+    // per wave, so each loop body is straight-line code
+    auto pairs = [&]() {
+        xch[0][w][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[0][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        __syncthreads();
+        if (Z == 0) {
+            for (int h = 0; h < nst; h += 2) {
+                half32.template operator()<0>(h);
+                half32.template operator()<1>(h + 1);
+            }
         } else {
-            if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+            for (int h = 0; h < nst; h += 2) {
+                half32.template operator()<1>(h);
+                half32.template operator()<0>(h + 1);
+            }
         }
+    };
+    if constexpr (sizeof(T) == 4) {
+        pairs();
+    } else {
+        // FP64: one loop body picks the colour order per half-sweep (two
+        // copies of the unrolled pair would exceed the register budget)
+        for (int h = 0; h < nst; h++) {
+            const int b = h & 1;
+            xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
+            xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+            __syncthreads();
+            V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
+            if (w > 0) sv = xch[b][w - 1][1][lane];
+            if (w < NW - 1) nv = xch[b][w + 1][0][lane];
+#define IFF_HALF(Q_, F_)                                                                              \
+    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
+                             cxa, cxpb, cxmb, cxb, om)
+            if (smask == 0) {
+                if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+            } else {
+                if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+            }
 #undef IFF_HALF
+        }
     }
 
     // store the interior of the tile: rows H .. H+TY-1, columns H .. H+TW-1
