@@ -377,7 +377,7 @@ template <typename T, int TY, int M, int h>
 __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
                                               const uint8_t (*sf)[kTileX + 4 * M],
                                               const T *mxp, const T *mxm, const T *myp, const T *mym,
-                                              int gpar, T om)
+                                              int gpar, int cpar, T om)
 {
     constexpr int H = 2 * M;
     constexpr int R = H - 1 - h;
@@ -403,22 +403,22 @@ __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (
         const T sa = (aE + aW) + (aN + aS);
         if (sa == (T)0) continue;
         sp[ly][lx] = sor_cell<T>(sp[ly][lx], sr[ly][lx], aE, sp[ly][lx + 1], aW, sp[ly][lx - 1], aN,
-                                 sp[ly + 1][lx], aS, sp[ly - 1][lx], sa, om);
+                                 sp[ly + 1][lx], aS, sp[ly - 1][lx], sa, om, cpar + lx);
     }
 }
 
 template <typename T, int TY, int M, int h>
 __device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
                                                const uint8_t (*sf)[kTileX + 4 * M], const T *mxp,
-                                               const T *mxm, const T *myp, const T *mym, int gpar, T om,
-                                               int nst)
+                                               const T *mxm, const T *myp, const T *mym, int gpar, int cpar,
+                                               T om, int nst)
 {
     if constexpr (h < 2 * M) {
         if (h < nst) {
-            rb_half_sweep<T, TY, M, h>(sp, sr, sf, mxp, mxm, myp, mym, gpar, om);
+            rb_half_sweep<T, TY, M, h>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om);
             __syncthreads();
         }
-        rb_half_sweeps<T, TY, M, h + 1>(sp, sr, sf, mxp, mxm, myp, mym, gpar, om, nst);
+        rb_half_sweeps<T, TY, M, h + 1>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om, nst);
     }
 }
 
@@ -534,8 +534,10 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     __syncthreads();
 
     // 2*meff half-sweeps on shrinking regions
+    // parity of the tile's first cell, and of its first column (LDS column
+    // lx is global column i0 - H + lx)
     const int gpar = (i0 - H) + (j0 - H);
-    rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, om, 2 * meff);
+    rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, (i0 - H) & 1, om, 2 * meff);
 
     // write the tile (all interior cells: non-fluid values pass through)
     constexpr int VW = 16 / sizeof(T);  // 16-byte stores; i0 is 256-byte aligned

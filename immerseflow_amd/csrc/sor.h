@@ -60,36 +60,43 @@ __device__ __forceinline__ void gstore_vec(void *p, VT v)
 
 // SOR update of one cell of the 5-point Poisson operator.  aE..aS are the
 // face coefficients (zero on closed faces), sa = (aE + aW) + (aN + aS) =
-// -ap, r the right side, om the relaxation factor.
+// -ap, r the right side, om the relaxation factor, i the cell's column.
 // FP64: the oracle's expression and order, no contraction, IEEE division:
 //   sum = (aE pE + aW pW) + (aN pN + aS pS);  gs = (r - sum) / ap;
 //   p' = pc + om (gs - pc)
 // so the FP64 sweeps are bit-identical to it.  FP32 (the error equation
 // of the mixed path, refined in FP64 afterwards): the same update
-// rearranged for fused multiply-adds and the hardware reciprocal,
-//   s = aS pS + (aN pN + (aW pW + (aE pE - r)));  p' = (om / sa) s + (1 - om) pc,
-// evaluated identically by every FP32 sweep kernel.
+// rearranged for fused multiply-adds and the hardware reciprocal, with the
+// weight w = om / sa and wr = w r:
+//   s = aS pS + (aN pN + (a2 p2 + a1 p1));  p' = w s + ((1 - om) pc - wr)
+// where (a1, p1) is the west neighbour in an odd column and the east
+// neighbour in an even one: the neighbour the register-tile kernel takes
+// from the adjacent lane, so that the lane shift folds into the product.
+// Every FP32 sweep kernel evaluates exactly this.
 // FP32 relaxation weight om / sa of a cell (sa != 0).
 __device__ __forceinline__ float sor_weight(float sa, float om) { return om * __builtin_amdgcn_rcpf(sa); }
 
-// FP32 update with the cell's weight w = sor_weight(sa, om) already formed.
-__device__ __forceinline__ float sor_cell_w(float pc, float r, float aE, float pE, float aW, float pW,
-                                            float aN, float pN, float aS, float pS, float w, float om)
+// FP32 update with w = sor_weight(sa, om), wr = w * r and c1 = 1 - om formed.
+__device__ __forceinline__ float sor_cell_w(float pc, float wr, float a1, float p1, float a2, float p2, float aN,
+                                            float pN, float aS, float pS, float w, float c1)
 {
-    const float s = __builtin_fmaf(aS, pS, __builtin_fmaf(aN, pN, __builtin_fmaf(aW, pW, __builtin_fmaf(aE, pE, -r))));
-    return __builtin_fmaf(w, s, (1.0f - om) * pc);
+    const float s = __builtin_fmaf(aS, pS, __builtin_fmaf(aN, pN, __builtin_fmaf(a2, p2, a1 * p1)));
+    return __builtin_fmaf(w, s, __builtin_fmaf(c1, pc, -wr));
 }
 
 template <typename T>
 __device__ __forceinline__ T sor_cell(T pc, T r, T aE, T pE, T aW, T pW, T aN, T pN, T aS, T pS, T sa,
-                                      T om)
+                                      T om, int i)
 {
     if constexpr (sizeof(T) == 8) {
         const T sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
         const T gs = (r - sum) / (-sa);
         return pc + om * (gs - pc);
     } else {
-        return sor_cell_w(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sor_weight(sa, om), om);
+        const float w = sor_weight(sa, om);
+        const bool odd = (i & 1) != 0;
+        return sor_cell_w(pc, w * r, odd ? aW : aE, odd ? pW : pE, odd ? aE : aW, odd ? pE : pW, aN, pN, aS, pS,
+                          w, 1.0f - om);
     }
 }
 

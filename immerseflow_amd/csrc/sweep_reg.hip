@@ -38,7 +38,7 @@ namespace {
 template <typename T, int NRW>
 struct RegTile {
     T pa[NRW], pb[NRW];  // unknown, slots a (even column) and b (odd column)
-    T ra[NRW], rb[NRW];  // right side
+    T ra[NRW], rb[NRW];  // right side (FP32: w r once the weights are formed)
     T cyp[NRW], cym[NRW];
     T cy[NRW];           // FP64: cyp + cym of the fast path
     T wa[NRW], wb[NRW];  // FP32: the cell's weight om/sa, 0 for a cell not updated
@@ -63,11 +63,13 @@ __device__ __forceinline__ T face_coef(int f, T a)
 // non-fluid cell, so the metric coefficients (zeroed on boundary faces)
 // give the same neighbour sum as the flagged ones.  The closed faces enter
 // through the cell's weight w = om / sa (sa over the open faces), formed
-// once per launch; w = 0 leaves a non-fluid cell at zero.
-__device__ __forceinline__ float reg_cell_w(float pc, float r, float pE, float pW, float pN, float pS,
-                                            float cxp, float cxm, float cyp, float cym, float w, float om)
+// once per launch with wr = w r; w = 0 leaves a non-fluid cell at zero.
+// (a1, p1) is the neighbour held by the adjacent lane: the lane shift
+// folds into that product (v_mul_f32 with DPP).
+__device__ __forceinline__ float reg_cell_w(float pc, float wr, float a1, float p1, float a2, float p2, float pN,
+                                            float pS, float cyp, float cym, float w, float c1)
 {
-    return sor_cell_w(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, w, om);
+    return sor_cell_w(pc, wr, a1, p1, a2, p2, cyp, pN, cym, pS, w, c1);
 }
 
 // One cell of a half-sweep, FP64.  Fast path (FAST): a fluid cell whose faces
@@ -80,14 +82,14 @@ __device__ __forceinline__ T reg_cell(int f, T pc, T r, T pE, T pW, T pN, T pS, 
                                       T cyp, T cym, T cy, T om)
 {
     if constexpr (FAST) {
-        return sor_cell<T>(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, cx + cy, om);
+        return sor_cell<T>(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, cx + cy, om, 0);
     } else {
         const T aE = face_coef<T, F_OPEN_E>(f, cxp);
         const T aW = face_coef<T, F_OPEN_W>(f, cxm);
         const T aN = face_coef<T, F_OPEN_N>(f, cyp);
         const T aS = face_coef<T, F_OPEN_S>(f, cym);
         const T sa = (aE + aW) + (aN + aS);
-        const T nv = sor_cell<T>(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sa, om);
+        const T nv = sor_cell<T>(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sa, om, 0);
         return ((f & F_FLUID) && sa != (T)0) ? nv : pc;
     }
 }
@@ -101,8 +103,9 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
         const T pS = k == 0 ? sva : R.pa[ks];
         const T pN = k == NRW - 1 ? nva : R.pa[kn];
         const T pE = R.pb[k], pW = from_west(R.pb[k]);
-        if constexpr (sizeof(T) == 4)
-            R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, R.cyp[k], R.cym[k], R.wa[k], om);
+        if constexpr (sizeof(T) == 4)  // slot a: odd column, west neighbour in lane l-1
+            R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], cxma, pW, cxpa, pE, pN, pS, R.cyp[k], R.cym[k], R.wa[k],
+                                 1.0f - om);
         else
             R.pa[k] = reg_cell<T, FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, cxa, R.cyp[k],
                                         R.cym[k], R.cy[k], om);
@@ -110,8 +113,9 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
         const T pS = k == 0 ? svb : R.pb[ks];
         const T pN = k == NRW - 1 ? nvb : R.pb[kn];
         const T pE = from_east(R.pa[k]), pW = R.pa[k];
-        if constexpr (sizeof(T) == 4)
-            R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, R.cyp[k], R.cym[k], R.wb[k], om);
+        if constexpr (sizeof(T) == 4)  // slot b: even column, east neighbour in lane l+1
+            R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], cxpb, pE, cxmb, pW, pN, pS, R.cyp[k], R.cym[k], R.wb[k],
+                                 1.0f - om);
         else
             R.pb[k] = reg_cell<T, FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, cxb, R.cyp[k],
                                         R.cym[k], R.cy[k], om);
@@ -333,6 +337,8 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 R.wa[k] = ((fa & F_FLUID) && sa_a != 0.0f) ? sor_weight(sa_a, om) : 0.0f;
                 R.wb[k] = ((fb & F_FLUID) && sa_b != 0.0f) ? sor_weight(sa_b, om) : 0.0f;
             }
+            R.ra[k] = R.wa[k] * R.ra[k];
+            R.rb[k] = R.wb[k] * R.rb[k];
         } else {
             if (offfast) smask |= 1u << k;
         }

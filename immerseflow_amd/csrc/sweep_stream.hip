@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
                         const T aS = (f & F_OPEN_S) ? MY[sj] : (T)0;
                         const T sa = (aE + aW) + (aN + aS);
                         if (sa != (T)0) {
-                            Pr[sj] = sor_cell<T>(pc, Rr[sj], aE, pE, aW, pW, aN, Pr[sn], aS, Pr[ss], sa, om);
+                            Pr[sj] = sor_cell<T>(pc, Rr[sj], aE, pE, aW, pW, aN, Pr[sn], aS, Pr[ss], sa, om, gi);
                         }
                     }
                 }
