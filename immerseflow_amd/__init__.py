@@ -343,6 +343,12 @@ class Case:
         default) or launch the kernels one by one; results are identical."""
         _check(_lib().if_case_graphs(self._h, 1 if enable else 0), "graphs")
 
+    def persistent(self, enable=True):
+        """Run each chunk of Poisson sweeps as one persistent launch when the
+        shape allows it and the grid fits on the device (the default), or
+        as one launch per stage; results are identical."""
+        _check(_lib().if_case_persistent(self._h, 1 if enable else 0), "persistent")
+
     def sweep_timing(self, reset=False):
         ms, n = C.c_double(), C.c_longlong()
         _check(_lib().if_case_sweep_timing(self._h, 1 if reset else 0, C.byref(ms), C.byref(n)))
@@ -422,6 +428,10 @@ class Batch:
     def graphs(self, enable=True):
         """As Case.graphs."""
         _check(_lib().if_batch_graphs(self._h, 1 if enable else 0), "graphs")
+
+    def persistent(self, enable=True):
+        """As Case.persistent."""
+        _check(_lib().if_batch_persistent(self._h, 1 if enable else 0), "persistent")
 
     def autotune(self):
         """As Case.autotune, over every running case of the batch."""

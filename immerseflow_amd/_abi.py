@@ -98,6 +98,8 @@ SIGNATURES = [
     ("if_batch_tune", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("if_case_graphs", C.c_int, [C.c_void_p, C.c_int]),
     ("if_batch_graphs", C.c_int, [C.c_void_p, C.c_int]),
+    ("if_case_persistent", C.c_int, [C.c_void_p, C.c_int]),
+    ("if_batch_persistent", C.c_int, [C.c_void_p, C.c_int]),
     ("if_case_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
     ("if_case_monitor", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("if_case_monitor_read", C.c_int, [C.c_void_p, _dp, C.c_size_t, C.POINTER(C.c_size_t),

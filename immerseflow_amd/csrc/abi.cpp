@@ -235,6 +235,14 @@ if_status if_case_graphs(if_case *c, int enable)
     return IF_OK;
 }
 
+if_status if_case_persistent(if_case *c, int enable)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "persistent: physics mode only");
+    c->eng->use_persistent = enable != 0;
+    c->eng->mark_dirty();
+    return IF_OK;
+}
+
 static if_status case_autotune(if_case *c, if_tune_result *out)
 {
     // a single case is resident but idle between calls: open it for one
@@ -287,6 +295,14 @@ if_status if_batch_graphs(if_batch *b, int enable)
 {
     if (!b) return fail(IF_ERR_ARG, "null batch");
     b->eng->use_graphs = enable != 0;
+    return IF_OK;
+}
+
+if_status if_batch_persistent(if_batch *b, int enable)
+{
+    if (!b) return fail(IF_ERR_ARG, "null batch");
+    b->eng->use_persistent = enable != 0;
+    b->eng->mark_dirty();
     return IF_OK;
 }
 

@@ -112,6 +112,12 @@ struct Ctrl {
     double max_div;
 };
 
+// Synchronisation state of the persistent Poisson sweep (one per engine,
+// zeroed at creation): a sticky error flag set when a wait times out.
+struct GridBar {
+    unsigned err, pad[3];
+};
+
 // Everything a kernel needs to know about one case.
 // Per-step monitors of a case (probes and body force), in device memory.
 constexpr int kMaxProbes = 32;

@@ -70,6 +70,9 @@ public:
     bool timing = false;
     // replay each chunk of Poisson sweep launches from a captured hipGraph
     bool use_graphs = true;
+    // mixed precision, register-tile shapes: run each chunk as one
+    // persistent launch when all its blocks fit on the device at once
+    bool use_persistent = true;
     double sweep_ms = 0.0;
     long long sweep_launches = 0;
     double sweep_cells = 0.0;  // interior cell-sweeps inside the timed chunks
@@ -86,8 +89,9 @@ private:
     if_status read_ctrl_sync(std::vector<Ctrl> &out);
     if_status update_moving();
     // the K sweeps of one Poisson chunk, as kernel launches on st_
-    void enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line);
-    if_status capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line);
+    void enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers);
+    if_status capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers);
+    if_status check_barrier();
     void drop_graph();
 
     int prec_;
@@ -114,7 +118,10 @@ private:
     // captured sweep chunk, valid for the resident set of the last
     // refresh() and the launch shape in g_key_
     hipGraphExec_t g_sweeps_ = nullptr;
-    int g_key_[7] = {};
+    int g_key_[8] = {};
+    GridBar *d_bar_ = nullptr;     // error flag of the persistent sweep
+    unsigned *d_flags_ = nullptr;  // its per-tile stage counters
+    int cap_flags_ = 0;
 };
 
 }  // namespace iff

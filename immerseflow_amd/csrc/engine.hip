@@ -54,6 +54,8 @@ Engine::~Engine()
         if (s && s->mon_mem) hipFree(s->mon_mem);
     if (d_cases_) hipFree(d_cases_);
     if (d_ctrl_) hipFree(d_ctrl_);
+    if (d_bar_) hipFree(d_bar_);
+    if (d_flags_) hipFree(d_flags_);
     if (d_tiles_base_) hipFree(d_tiles_base_);
     if (d_tiles_sweep_) hipFree(d_tiles_sweep_);
     if (h_snap_) hipHostFree(h_snap_);
@@ -72,6 +74,8 @@ if_status Engine::init()
     IFF_HIP(hipStreamCreate(&st_));  // default priority
     IFF_HIP(hipMalloc(&d_ctrl_, sizeof(Ctrl) * cap_ctrl_));
     IFF_HIP(hipMemsetAsync(d_ctrl_, 0, sizeof(Ctrl) * cap_ctrl_, st_));
+    IFF_HIP(hipMalloc(&d_bar_, sizeof(GridBar)));
+    IFF_HIP(hipMemsetAsync(d_bar_, 0, sizeof(GridBar), st_));
     IFF_HIP(hipHostMalloc((void **)&h_snap_, sizeof(Ctrl) * cap_ctrl_ * kRing, hipHostMallocDefault));
     for (auto &e : ev_) IFF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : tev_) IFF_HIP(hipEventCreate(&e));
@@ -571,11 +575,14 @@ void Engine::drop_graph()
 // given resident set and shape: the kernels read the loop state (buffer
 // parity, sweep counts, done flags) from the control blocks on the device,
 // so a captured chunk replays unchanged.
-void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line)
+void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers)
 {
     const int n = (int)order_.size();
+    if (any_rb && pers)  // the whole chunk in one launch
+        launch_ppe_sweep_pers(st_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
+                              K, d_bar_, d_flags_);
     for (int s = 0; s < K; s++) {
-        if (any_rb && s % M == 0) {
+        if (any_rb && !pers && s % M == 0) {
             if (sweep_kind >= 2)
                 launch_ppe_sweep_reg(st_, prec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
                                      nt_sweep_, s, K);
@@ -594,14 +601,14 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
 // Capture the chunk into a graph (kernel nodes only) unless the cached one
 // matches the shape.  refresh() drops the graph, since it re-uploads the
 // case list and tile tables the launches point at.
-if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line)
+if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers)
 {
-    const int key[7] = {K, M, sweep_kind, tiles_sweep_ty_, (int)any_rb, (int)any_jac, (int)any_line};
-    if (g_sweeps_ && std::equal(key, key + 7, g_key_)) return IF_OK;
+    const int key[8] = {K, M, sweep_kind, tiles_sweep_ty_, (int)any_rb, (int)any_jac, (int)any_line, (int)pers};
+    if (g_sweeps_ && std::equal(key, key + 8, g_key_)) return IF_OK;
     drop_graph();
     hipGraph_t g = nullptr;
     IFF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-    enqueue_sweeps(K, M, any_rb, any_jac, any_line);
+    enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers);
     hipError_t e = hipStreamEndCapture(st_, &g);
     if (e != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph capture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&g_sweeps_, g, nullptr, nullptr, 0);
@@ -610,8 +617,26 @@ if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool a
         g_sweeps_ = nullptr;
         return fail(IF_ERR_HIP, std::string("sweep graph instantiate: ") + hipGetErrorString(e));
     }
-    std::copy(key, key + 7, g_key_);
+    std::copy(key, key + 8, g_key_);
     return IF_OK;
+}
+
+// After a step that ran persistent sweeps: a neighbour wait that timed out
+// (blocks not all resident) leaves the results invalid.  Reset the state,
+// stop using the persistent form and report the failure.
+if_status Engine::check_barrier()
+{
+    GridBar b{};
+    IFF_HIP(hipMemcpyAsync(&b, d_bar_, sizeof(GridBar), hipMemcpyDeviceToHost, st_));
+    IFF_HIP(hipStreamSynchronize(st_));
+    if (b.err == 0) return IF_OK;
+    use_persistent = false;
+    drop_graph();
+    IFF_HIP(hipMemsetAsync(d_bar_, 0, sizeof(GridBar), st_));
+    if (cap_flags_ > 0) IFF_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned) * cap_flags_, st_));
+    IFF_HIP(hipStreamSynchronize(st_));
+    return fail(IF_ERR_HIP, "persistent Poisson sweep: a neighbour wait timed out (blocks not co-resident); "
+                            "this step's result is invalid and the persistent form is now off");
 }
 
 if_status Engine::refresh()
@@ -664,6 +689,14 @@ if_status Engine::refresh()
     if (rc != IF_OK) return rc;
     rc = up(d_tiles_sweep_, cap_tiles_sweep_, nt_sweep_, tsw);
     if (rc != IF_OK) return rc;
+    // stage counters of the persistent sweep: one per sweep tile, all equal
+    if (nt_sweep_ > cap_flags_) {
+        if (d_flags_) hipFree(d_flags_);
+        d_flags_ = nullptr;
+        cap_flags_ = std::max(nt_sweep_, 2 * cap_flags_);
+        IFF_HIP(hipMalloc(&d_flags_, sizeof(unsigned) * cap_flags_));
+    }
+    if (cap_flags_ > 0) IFF_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned) * cap_flags_, st_));
     tiles_sweep_ty_ = sty;
     IFF_HIP(hipStreamSynchronize(st_));  // host vectors go out of scope
     dirty_ = false;
@@ -716,6 +749,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
     if (agg) std::memset(agg, 0, sizeof(*agg));
     const int M = std::max(1, sweep_m);
     for (int it = 0; it < nsteps; it++) {
+        bool ran_pers = false;
         if (dirty_) {
             if_status rc = refresh();
             if (rc != IF_OK) return rc;
@@ -776,8 +810,12 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;
         int timed = 0;
+        // persistent chunk: mixed precision, register tiles, grid fits
+        const bool pers = any_rb && use_persistent && prec_ != IF_FP64 && sweep_kind >= 2 &&
+                          pers_fits(reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
+        ran_pers = ran_pers || pers;
         if (use_graphs) {
-            rc = capture_sweeps(K, M, any_rb, any_jac, any_line);
+            rc = capture_sweeps(K, M, any_rb, any_jac, any_line, pers);
             if (rc != IF_OK) return rc;
         }
         hipError_t glaunch = hipSuccess;
@@ -792,7 +830,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
                 const hipError_t e = hipGraphLaunch(g_sweeps_, st_);
                 if (e != hipSuccess && glaunch == hipSuccess) glaunch = e;
             } else {
-                enqueue_sweeps(K, M, any_rb, any_jac, any_line);
+                enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers);
             }
             if (tm) {
                 hipEventRecord(tev_[2 * (u - 1) + 1], st_);
@@ -814,6 +852,10 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         std::vector<Ctrl> all;
         rc = read_ctrl_sync(all);
         if (rc != IF_OK) return rc;
+        if (ran_pers) {
+            rc = check_barrier();
+            if (rc != IF_OK) return rc;
+        }
         if (timing && timed > 0) {
             int maxsw = 0;
             for (int id : order_) maxsw = std::max(maxsw, all[id].ppe_sweeps);

@@ -153,12 +153,54 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
     }
 }
 
+// Neighbour synchronisation of the persistent sweep.  Every tile has a
+// stage counter (flags[t], advanced by the number of stages per launch by
+// every tile, so all counters agree when a launch starts).  A block
+// publishes "stage l done" after its border strips are written, then waits
+// until the 8 neighbouring tiles of its case have published the same stage.
+// Strips are written and halos read at device scope (sc1: write-through,
+// and loads that miss in the per-XCD L2), so no cache flush is needed; the
+// waits are bounded: past kBarTimeout ticks of the 100 MHz real-time
+// counter bar->err is set and every later wait returns at once, so the
+// kernel always ends; the host checks err after the step.
+constexpr unsigned long long kBarTimeout = 2000000;  // 20 ms
+constexpr int kDev = 16;                             // cache policy: sc1 (device scope)
+constexpr unsigned kRsrc3 = 0x00020000;              // buffer resource word 3 (gfx9)
+
+// wait (lanes 0-7 of wave 0, one neighbour each; nb < 0: none) until
+// flags[nb] has reached target
+__device__ __forceinline__ void wait_neighbours(const unsigned *flags, int nb, unsigned target, GridBar *bar)
+{
+    if (nb >= 0) {
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while ((int)(__hip_atomic_load(&flags[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+            if (__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kBarTimeout) {
+                __hip_atomic_store(&bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
 }  // namespace
 
-template <typename T, int M, int NW, int NRW>
+// PERS (FP32 only): one launch runs the chunk's K sweeps as stages of M
+// sweeps.  The tile stays in registers; after each stage a block writes the
+// border strips (depth 2M) of its stored region, waits for its neighbouring
+// tiles to have written theirs, and reloads its halo ring from them.  Stage
+// L leaves its output where launch L of the chunked form does
+// (e[(L + 1) & 1]), the last stage of a case the whole stored region, so
+// the checks read the same buffer either way.  A block is at most one
+// stage ahead of its neighbours, so the two buffers never hold data a
+// neighbour still needs when they are overwritten.
+template <typename T, int M, int NW, int NRW, bool PERS>
 __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
-                                                       int ntiles_pad, int s, int K)
+                                                       int ntiles_pad, int s, int K, GridBar *bar,
+                                                       unsigned *flags)
 {
+    static_assert(!PERS || sizeof(T) == 4, "persistent sweeps are FP32");
     constexpr int H = 2 * M;
     constexpr int TW = 128 - 2 * H;   // stored columns per tile
     constexpr int TY = NW * NRW - 2 * H;  // stored rows per tile
@@ -171,14 +213,31 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     __shared__ uint16_t sfl[kFlw][NRW][64];
 
     const int t = tile_of_block(ntiles_pad);
-    if (t >= ntiles) return;  // whole workgroup
+    const int nstage = (K + M - 1) / M;  // PERS: stages of the launch
+    if constexpr (PERS) {
+        bool run = false;
+        if (t < ntiles) {
+            const CaseDev &cr = cs[tiles[t].x];
+            const Ctrl *cc = cr.ctrl;
+            run = cr.ppe_solver == IF_PPE_RBSOR && cc->active && !cc->ppe_done && cr.ppe_itermax > cc->ppe_sweeps;
+        }
+        if (!run) {  // whole workgroup: nobody waits on this tile, advance its counter
+            if (t < ntiles && threadIdx.x == 0) {
+                const unsigned f0 = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&flags[t], f0 + (unsigned)nstage, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+    } else {
+        if (t >= ntiles) return;  // whole workgroup
+    }
     const int2 tl = tiles[t];
     const CaseDev &c = cs[tl.x];
     const Ctrl *ct = c.ctrl;
-    if (c.ppe_solver != IF_PPE_RBSOR || !ct->active || ct->ppe_done) return;
+    if (c.ppe_solver != IF_PPE_RBSOR || !ct->active || ct->ppe_done) return;  // (PERS: never taken)
     int kexec = c.ppe_itermax - ct->ppe_sweeps;
     if (kexec > K) kexec = K;
-    if (s >= kexec) return;
+    if (s >= kexec) return;  // (PERS: s = 0 < kexec)
     int meff = kexec - s;
     if (meff > M) meff = M;
     const int nst = 2 * meff;  // half-sweeps of this launch
@@ -370,24 +429,117 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     };
     // nst is even: the half-sweeps run in pairs whose colour order is fixed
     // per wave, so each loop body is straight-line code
-    auto pairs = [&]() {
+    auto pairs = [&](int n) {
         xch[0][w][0][lane] = V2{R.pa[0], R.pb[0]};
         xch[0][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         __syncthreads();
         if (Z == 0) {
-            for (int h = 0; h < nst; h += 2) {
+            for (int h = 0; h < n; h += 2) {
                 half32.template operator()<0>(h);
                 half32.template operator()<1>(h + 1);
             }
         } else {
-            for (int h = 0; h < nst; h += 2) {
+            for (int h = 0; h < n; h += 2) {
                 half32.template operator()<1>(h);
                 half32.template operator()<0>(h + 1);
             }
         }
     };
+    // store the stored region of the tile (rows H .. H+TY-1, columns H ..
+    // H+TW-1), or only its border strips of depth H (the halo of the
+    // neighbouring tiles)
+    auto store = [&](T *out, bool border) {
+        if (lane >= H / 2 && lane < 64 - H / 2) {
+            const bool lb = lane < H || lane >= 64 - H;
+#pragma unroll
+            for (int k = 0; k < NRW; k++) {
+                const int rr = w * NRW + k;
+                const int gj = jt0 + k;
+                if (rr < H || rr >= H + TY || gj > ny - 2) continue;
+                if (border && !lb && rr >= 2 * H && rr < TY) continue;
+                T *row = out + (size_t)gj * P;  // wave-uniform
+                if (ga + 1 <= nx - 2)
+                    gstore_vec<V2>(row + ga, V2{R.pa[k], R.pb[k]});
+                else if (ga <= nx - 2)
+                    GW(row)[ga] = R.pa[k];
+            }
+        }
+    };
+    if constexpr (PERS) {
+        using U2 = unsigned __attribute__((ext_vector_type(2)));
+        const int nexec = (kexec + M - 1) / M;  // stages with sweeps for this case
+        const unsigned f0 = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the neighbouring tiles of the case (lanes 0-7 of wave 0)
+        int nb = -1;
+        if (w == 0 && lane < 8) {
+            const int d = lane < 4 ? lane : lane + 1;  // 3 x 3 without the centre
+            const int ntys = (ny - 2 + TY - 1) / TY;
+            const int qx = tx + d % 3 - 1, qy = ty + d / 3 - 1;
+            if (qx >= 0 && qx < ntxs && qy >= 0 && qy < ntys) nb = t - tl.y + qy * ntxs + qx;
+        }
+        const int nrec = P * ny * (int)sizeof(float);  // bytes of a field
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(c.e[0], 0, nrec, kRsrc3);
+        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(c.e[1], 0, nrec, kRsrc3);
+        for (int l = 0; l < nexec; l++) {
+            int me = kexec - l * M;
+            if (me > M) me = M;
+            pairs(2 * me);
+            const bool last = l + 1 == nexec;
+            const __amdgpu_buffer_rsrc_t ro = ((l + 1) & 1) ? rs1 : rs0;
+            // stage output at device scope: the border strips, or (last
+            // stage of the case) the whole stored region
+            if (lane >= H / 2 && lane < 64 - H / 2) {
+                const bool lb = lane < H || lane >= 64 - H;
+#pragma unroll
+                for (int k = 0; k < NRW; k++) {
+                    const int rr = w * NRW + k;
+                    const int gj = jt0 + k;
+                    if (rr < H || rr >= H + TY || gj > ny - 2) continue;
+                    if (!last && !lb && rr >= 2 * H && rr < TY) continue;
+                    const int off = (gj * P + ga) * (int)sizeof(float);
+                    if (ga + 1 <= nx - 2)
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro, off,
+                                                              0, kDev);
+                    else if (ga <= nx - 2)
+                        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, off, 0, kDev);
+                }
+            }
+            if (last) break;
+            // publish stage l once every wave's stores are complete, wait
+            // for the neighbours' stage l
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&flags[t], f0 + (unsigned)(l + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wait_neighbours(flags, nb, f0 + (unsigned)(l + 1), bar);
+            __syncthreads();
+            // halo ring: the cells of the tile outside its stored region
+            // that lie in the domain's interior (device-scope loads)
+            const bool lh = lane < H / 2 || lane >= 64 - H / 2;
+#pragma unroll
+            for (int k = 0; k < NRW; k++) {
+                const int rr = w * NRW + k;
+                const int gj = jt0 + k;
+                if (gj < 1 || gj > ny - 2) continue;
+                if (!lh && rr >= H && rr < H + TY) continue;
+                const int off = (gj * P + ga) * (int)sizeof(float);
+                if (ina && inb) {
+                    const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
+                    R.pa[k] = v[0];
+                    R.pb[k] = v[1];
+                } else {
+                    if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off, 0, kDev));
+                    if (inb) R.pb[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off + 4, 0, kDev));
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&flags[t], f0 + (unsigned)nstage, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     if constexpr (sizeof(T) == 4) {
-        pairs();
+        pairs(nst);
     } else {
         // FP64: one loop body picks the colour order per half-sweep (two
         // copies of the unrolled pair would exceed the register budget)
@@ -411,28 +563,67 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
         }
     }
 
-    // store the interior of the tile: rows H .. H+TY-1, columns H .. H+TW-1
-    if (lane >= H / 2 && lane < 64 - H / 2) {
-#pragma unroll
-        for (int k = 0; k < NRW; k++) {
-            const int rr = w * NRW + k;
-            const int gj = jt0 + k;
-            if (rr < H || rr >= H + TY || gj > ny - 2) continue;
-            const size_t o = (size_t)gj * P + ga;
-            if (ga + 1 <= nx - 2)
-                gstore_vec<V2>(dst + o, V2{R.pa[k], R.pb[k]});
-            else if (ga <= nx - 2)
-                GW(dst)[o] = R.pa[k];
-        }
-    }
+    store(dst, false);
 }
 
 template <typename T, int M, int NW, int NRW>
 static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt, pad, s,
-                       K);
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt, pad,
+                       s, K, (GridBar *)nullptr, (unsigned *)nullptr);
+}
+
+// Blocks of the persistent form that can be resident at once: resident
+// blocks per CU (occupancy calculator) times the CUs, per shape.
+template <int M, int NW, int NRW>
+static int pers_capacity()
+{
+    static int capacity = -1;  // one device type per process
+    if (capacity < 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbsor_reg<float, M, NW, NRW, true>, 64 * NW,
+                                                         0) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        capacity = per_cu * cus;
+    }
+    return capacity;
+}
+
+template <int M, int NW, int NRW>
+static void pers_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int K, GridBar *bar,
+                     unsigned *flags)
+{
+    const int pad = ((nt + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_rbsor_reg<float, M, NW, NRW, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
+                       pad, 0, K, bar, flags);
+}
+
+// the shapes with a persistent form
+#define IFF_PERS_SHAPES(X) X(16, 8, 96, 8) X(16, 8, 64, 6) X(16, 4, 48, 4) X(8, 8, 32, 8) X(8, 4, 32, 6)
+
+bool pers_fits(int waves, int ty, int m, int nt)
+{
+    const int pad = ((nt + 7) / 8) * 8;
+#define IFF_PF(W_, M_, TY_, NRW_) \
+    if (waves == W_ && m == M_ && ty == TY_) return pad <= pers_capacity<M_, W_, NRW_>();
+    IFF_PERS_SHAPES(IFF_PF)
+#undef IFF_PF
+    return false;
+}
+
+void launch_ppe_sweep_pers(hipStream_t st, int waves, int ty, int m, const CaseDev *cs, const int2 *tiles,
+                           int nt, int K, GridBar *bar, unsigned *flags)
+{
+#define IFF_PL(W_, M_, TY_, NRW_)                                  \
+    if (waves == W_ && m == M_ && ty == TY_) {                    \
+        pers_one<M_, W_, NRW_>(st, cs, tiles, nt, K, bar, flags); \
+        return;                                                   \
+    }
+    IFF_PERS_SHAPES(IFF_PL)
+#undef IFF_PL
 }
 
 // Stored tile: 128 - 4M columns by tile_y rows; tile_y + 4M rows per

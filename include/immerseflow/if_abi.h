@@ -234,6 +234,16 @@ if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch
  * replayed, recaptured when the resident set or the launch shape changes;
  * 0: the same kernels are launched one by one.  Results are identical. */
 if_status if_case_graphs(if_case *c, int enable);
+/* Persistent Poisson chunks (no counterpart in the reference).  enable = 1
+ * (the default): in mixed precision with a register-tile kernel (2, 3 or
+ * 4; shapes 16 waves x 96/64 rows x 8 sweeps, 16 x 48 x 4, 8 x 32 x 8 and
+ * 8 x 32 x 4), when every block of the launch can be resident on the
+ * device at once, each chunk of check_every sweeps runs as one launch
+ * whose tiles stay in registers, in stages of sweeps_per_launch sweeps
+ * separated by a grid-wide barrier; otherwise one launch per stage.
+ * Results are identical.  A barrier wait that times out makes the step
+ * return IF_ERR_HIP and turns the form off. */
+if_status if_case_persistent(if_case *c, int enable);
 /* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
  * body-force diagnostic with no counterpart there): after every step the
  * device records t, the force (Fx, Fy) the fluid exerts on the bodies and
@@ -306,6 +316,8 @@ if_status if_batch_tune(if_batch *b, int kernel, int tile_y, int sweeps_per_laun
                         int check_every);
 /* As if_case_graphs, for the batch. */
 if_status if_batch_graphs(if_batch *b, int enable);
+/* As if_case_persistent, for the batch. */
+if_status if_batch_persistent(if_batch *b, int enable);
 /* As if_case_autotune, over every running case of the batch at once (the
  * batched launches are what is timed; S is the median of the cases' last
  * sweep counts). */

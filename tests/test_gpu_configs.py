@@ -88,6 +88,26 @@ def test_c2_mixed_within_1e4_of_fp64(require_gpu):
     _mixed_vs_fp64(cfg, _prm(cfg, 1.998, cfg["ppe_tol"], 160), 2)
 
 
+def test_c2_persistent_chunks_equal_launch_per_stage(require_gpu):
+    """The 2048x1024 case at the default shape (242 tiles, one per CU, halo
+    strips crossing every XCD boundary): persistent chunks give the same
+    bits and sweep counts as one launch per stage."""
+    cfg = configs.c2()
+    prm = _prm(cfg, 1.998, cfg["ppe_tol"], 240)
+    _say("C2 persistent vs per-stage, 2 steps")
+    out = []
+    for pers in (True, False):
+        c = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_MIXED, bodies=cfg["bodies"], **prm)
+        c.tune(4, 96, 8, 240)
+        c.persistent(pers)
+        st = c.step(2)
+        out.append((st["total_ppe_sweeps"], {f: c.get(f) for f in ("u", "v", "p", "uf", "vf")}))
+        c.close()
+    assert out[0][0] == out[1][0]
+    for f in out[0][1]:
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
+
+
 def test_c3_stars_4096_geometry_and_mixed(require_gpu):
     cfg = configs.c3()
     prm = _prm(cfg, 0.0, cfg["ppe_tol"], 160)

@@ -313,6 +313,38 @@ def test_graph_replay_equals_direct_launches(require_gpu, prec):
             assert np.array_equal(runs[0][k], runs[1][k]), k
 
 
+@pytest.mark.parametrize("kind,ty,m,K,graphs", [(4, 96, 8, 40, True), (4, 64, 8, 13, True), (3, 32, 8, 40, False),
+                                                 (4, 48, 4, 12, True), (3, 32, 4, 10, True)])
+def test_persistent_chunks_equal_launch_per_stage(require_gpu, kind, ty, m, K, graphs):
+    """A chunk run as one persistent launch (tiles kept in registers, halo
+    strips exchanged through a grid barrier between stages) gives the same
+    bits as one launch per stage: in a batch with a point-Jacobi case (its
+    blocks only take part in the barriers), a case whose sweep budget ends
+    inside a chunk, and chunks that are not a whole number of stages."""
+    xf, yf, bodies = CASES["circle-stretched"]
+    xf2, yf2, bodies2 = CASES["ellipse"]
+    _say("persistent vs per-stage launches", kind, ty, m, K, graphs)
+    kw = dict(dt=0.01, re=100.0, ppe_omega=1.8, ppe_tol=1e-8, check_every=K)
+    runs = []
+    for pers in (True, False):
+        b = iff.Batch(iff.IF_MIXED)
+        b.tune(kind, ty, m, K)
+        b.graphs(graphs)
+        b.persistent(pers)
+        sl = [b.add(xf, yf, 3, bodies=bodies, **kw),
+              b.add(xf2, yf2, 3, bodies=bodies2, **kw),
+              b.add(xf2, yf2, 3, bodies=bodies2, **dict(kw, ppe_itermax=3 * K + 5)),
+              b.add(xf, yf, 3, bodies=bodies, **dict(kw, ppe_solver=0, ppe_omega=0.8))]
+        b.step(3)
+        runs.append({(k, f): b.get(s, f) for k, s in enumerate(sl) for f in ("u", "v", "p", "uf", "vf")})
+        runs[-1]["sweeps"] = [b.state(s)[2]["total_ppe_sweeps"] for s in sl]
+        b.close()
+    assert runs[0]["sweeps"] == runs[1]["sweeps"]
+    for k in runs[0]:
+        if k != "sweeps":
+            assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
 def test_mixed_precision_against_fp64_oracle(require_gpu):
     xf, yf, bodies = CASES["circle-stretched"]
     _say("mixed precision")
