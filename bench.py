@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--omega", type=float, default=1.998)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the Poisson sweep kernels one by one instead of replaying a hipGraph")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="one Poisson launch per sweeps-per-launch stage instead of one per check interval")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
@@ -54,14 +56,14 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_name(kind, ty, m, t):
+def kernel_name(kind, ty, m, t, pers=False):
     """The Poisson sweep kernel a tune() choice launches (as rocprofv3 names it)."""
     if kind == 0:
         return "k_rbsor<%s, %d, %d>" % (t, ty, m)
     if kind == 1:
         return "k_rbsor_stream<%s, %d, %d>" % (t, m, ty)
     nw = {2: 4, 3: 8, 4: 16}[kind]
-    return "k_rbsor_reg<%s, %d, %d, %d>" % (t, m, nw, (ty + 4 * m) // nw)
+    return "k_rbsor_reg<%s, %d, %d, %d, %s>" % (t, m, nw, (ty + 4 * m) // nw, "true" if pers else "false")
 
 
 def log(*a):
@@ -79,6 +81,7 @@ def make_case(iff, cfg, precision, a):
                  check_every=shape[3])
     c.tune(*shape)
     c.graphs(not a.no_graphs)
+    c.persistent(not a.no_persistent)
     return c
 
 
@@ -137,6 +140,7 @@ def batched_c4(iff, configs, a):
     b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
     b.tune(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
     b.graphs(not a.no_graphs)
+    b.persistent(not a.no_persistent)
     t0 = time.perf_counter()
     slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
                    ppe_tol=1e-6, check_every=a.check_every) for c in cases]
@@ -165,6 +169,7 @@ def batched_c4(iff, configs, a):
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-6, check_every=a.fp64_check_every)
         s.graphs(not a.no_graphs)
+        s.persistent(not a.no_persistent)
         s.step(1)
         t0 = time.perf_counter()
         s.step(a.batch_steps)
@@ -233,11 +238,13 @@ def main():
     case = make_case(iff, cfg, iff.IF_MIXED, a)
     mixed = timed_run(case, a.warmup, a.steps, cells, "mixed")
     sweep_bpc, step_bpc = case.byte_model()
+    sweeps_per_launch = int(round(sweep_bpc / 13.0))  # 13 B per cell-sweep (mixed)
+    pers = sweeps_per_launch > a.sweeps_per_launch    # a whole check interval per launch
     avg_ms = mixed["sweep_ms"] / max(1, mixed["sweep_launches"])
     achieved = sweep_bpc * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c2.json")
-    kname = kernel_name(a.kernel, a.tile_y, a.sweeps_per_launch, "float")
+    kname = kernel_name(a.kernel, a.tile_y, a.sweeps_per_launch, "float", pers)
     if os.path.exists(tp):
         with open(tp) as f:
             tj = json.load(f)
@@ -246,8 +253,8 @@ def main():
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel=kname,
-                bytes_per_launch=sweep_bpc * cells, mean_launch_us=round(avg_ms * 1e3, 3),
-                launches_timed=mixed["sweep_launches"])
+                bytes_per_launch=sweep_bpc * cells, sweeps_per_launch=sweeps_per_launch,
+                mean_launch_us=round(avg_ms * 1e3, 3), launches_timed=mixed["sweep_launches"])
 
     fp64 = None
     if not a.no_fp64 and a.fp64_steps > 0:
@@ -288,7 +295,9 @@ def main():
                    "poisson": "red-black SOR, stop max|div|<=%g" % cfg["ppe_tol"],
                    "sweeps_last_step": st["ppe_sweeps"], "ad_iters_last_step": st["ad_iters"],
                    "max_div": st["max_div"], "parallelism": "single GPU",
-                   "poisson_launch": "kernel launches" if a.no_graphs else "hipGraph replay per chunk"},
+                   "poisson_launch": ("persistent launch per check interval" if pers else
+                                      "launch per %d sweeps" % a.sweeps_per_launch) +
+                                     ("" if a.no_graphs else ", hipGraph replay")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64": fp64,

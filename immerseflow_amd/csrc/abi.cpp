@@ -330,7 +330,9 @@ if_status if_case_byte_model(if_case *c, double *sweep_bpc, double *step_bpc)
     Engine &e = *c->eng;
     Slot *s = e.slot(c->slot);
     const bool mixed = s->prec == IF_MIXED;
-    const int M = s->dev.ppe_solver == IF_PPE_RBSOR ? e.sweep_m : 1;
+    // sweeps per launch: M, or the whole chunk when the last step ran
+    // persistent chunks
+    const int M = s->dev.ppe_solver == IF_PPE_RBSOR ? (e.launch_sweeps > 0 ? e.launch_sweeps : e.sweep_m) : 1;
     const double ts = mixed ? 4 : 8;
     const double sweep = mixed ? 13.0 : 25.0;
     if (sweep_bpc) *sweep_bpc = sweep * M;

@@ -76,6 +76,7 @@ public:
     double sweep_ms = 0.0;
     long long sweep_launches = 0;
     double sweep_cells = 0.0;  // interior cell-sweeps inside the timed chunks
+    int launch_sweeps = 0;     // red-black sweeps per launch in the last step (M; K when persistent)
     int spec_depth = 3;
 
     static size_t footprint(const if_case_desc &d, int prec);

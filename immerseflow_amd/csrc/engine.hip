@@ -814,6 +814,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         const bool pers = any_rb && use_persistent && prec_ != IF_FP64 && sweep_kind >= 2 &&
                           pers_fits(reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
         ran_pers = ran_pers || pers;
+        if (any_rb) launch_sweeps = pers ? K : M;
         if (use_graphs) {
             rc = capture_sweeps(K, M, any_rb, any_jac, any_line, pers);
             if (rc != IF_OK) return rc;
@@ -864,7 +865,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, tev_[2 * u], tev_[2 * u + 1]) == hipSuccess) {
                     sweep_ms += ms;
-                    sweep_launches += (K + m_eff - 1) / m_eff;
+                    sweep_launches += ran_pers ? 1 : (K + m_eff - 1) / m_eff;
                 }
             }
             for (int id : order_)

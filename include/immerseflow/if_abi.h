@@ -274,11 +274,14 @@ if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size
  * solve stops (still at the same tolerance). */
 if_status if_case_autotune(if_case *c, if_tune_result *out);
 /* Accumulated event timing of the Poisson sweep kernel (milliseconds,
- * launches) over the steps since the last reset. */
+ * launches) over the steps since the last reset.  A persistent chunk
+ * counts as one launch. */
 if_status if_case_sweep_timing(if_case *c, int reset, double *ms,
                                long long *launches);
-/* Algorithmic HBM bytes per interior cell of one Poisson sweep launch and
- * of one step at the given sweep counts (the roofline byte model). */
+/* Algorithmic HBM bytes per interior cell of one Poisson sweep launch (13
+ * or 25 bytes per sweep times the sweeps per launch of the last step:
+ * sweeps_per_launch, or check_every for a persistent chunk) and of one
+ * step at the given sweep counts (the roofline byte model). */
 if_status if_case_byte_model(if_case *c, double *sweep_bytes_per_cell,
                              double *step_bytes_per_cell);
 
