@@ -991,19 +991,32 @@ if_status Engine::autotune(if_tune_result *out)
         sweep_m = c.m;
         rc = refresh();
         if (rc != IF_OK) break;
+        // a shape that will run as persistent chunks is timed that way, over
+        // 32 stages (a typical check interval); it writes only the scratch
+        // error buffers and advances the stage counters of every tile alike
+        const bool pc = prec_ != IF_FP64 && c.kind >= 2 && use_persistent &&
+                        pers_fits(reg_waves(c.kind), c.ty, c.m, nt_sweep_);
+        const int kp = 32 * c.m;
         const int s0 = prec_ == IF_FP64 ? 0 : c.m;
+        auto timed = [&]() {
+            if (pc)
+                launch_ppe_sweep_pers(st_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
+                                      d_bar_, d_flags_);
+            else
+                sweep_launch(c, s0, 2 * c.m);
+        };
         launch_tune_arm(st_, d_cases_, n, d_ctrl_, saved);
-        sweep_launch(c, s0, 2 * c.m);  // warm-up
-        sweep_launch(c, s0, 2 * c.m);
+        timed();  // warm-up
+        timed();
         (void)hipEventRecord(e0, st_);
-        for (int r = 0; r < kReps; r++) sweep_launch(c, s0, 2 * c.m);
+        for (int r = 0; r < kReps; r++) timed();
         (void)hipEventRecord(e1, st_);
         float ms = 0.f;
         if (!elapsed(&ms)) {
             rc = fail(IF_ERR_HIP, "autotune: timing failed");
             break;
         }
-        const double ns = 1e6 * (double)ms / (kReps * (double)c.m * cells);
+        const double ns = 1e6 * (double)ms / (kReps * (double)(pc ? kp : c.m) * cells);
         ncand++;
         if (ns < best_ns) {
             best_ns = ns;
@@ -1034,6 +1047,7 @@ if_status Engine::autotune(if_tune_result *out)
         hipStreamSynchronize(st_) == hipSuccess;
     cleanup();
     if (!restored) return fail(IF_ERR_HIP, "autotune: control-block restore failed");
+    if (rc == IF_OK && prec_ != IF_FP64) rc = check_barrier();
     if (rc != IF_OK) {
         sweep_kind = keep.kind;
         sweep_ty = keep.ty;

@@ -264,7 +264,8 @@ if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size
 /* Autotuner (no counterpart in the reference, whose loop is fixed):
  * times every Poisson sweep launch shape of if_case_tune on the case's
  * current fields (a few launches each, on the solver's scratch buffers;
- * the solution and the loop state are left as they were) and keeps the
+ * the solution and the loop state are left as they were; a shape that
+ * would run as persistent chunks is timed as one) and keeps the
  * fastest.  The check interval is then chosen from the measured cost of a
  * convergence check c (in sweeps) and the sweep count S of the last solve:
  * checking every K sweeps costs about S c / K checks plus K / 2 sweeps of
