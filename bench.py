@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
 
@@ -68,6 +70,23 @@ def kernel_name(kind, ty, m, t, pers=False):
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_err(a, b):
+    """max |a - b| / max(1, max |b|): the error of a mixed-precision field
+    against the FP64 field of the same case and step."""
+    import numpy as np
+    return float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b))))
+
+
+def fields_err(get_mixed, get_fp64):
+    """u, v, p errors; p also after removing the difference of the two
+    fields' means (the pressure of the all-Neumann problem is fixed only up
+    to a constant, which neither solve pins)."""
+    out = {f: rel_err(get_mixed(f), get_fp64(f)) for f in ("u", "v", "p")}
+    pm, pf = get_mixed("p"), get_fp64("p")
+    out["p_mod_const"] = rel_err(pm - pm.mean(), pf - pf.mean())
+    return out
 
 
 def make_case(iff, cfg, precision, a):
@@ -157,15 +176,18 @@ def batched_c4(iff, configs, a):
     info = b.info()
     divs = [b.state(s)[2]["max_div"] for s in slots]
     cells = sum((len(c["xf"]) - 1) * (len(c["yf"]) - 1) for c in cases)
-    b.close()
     per_step = t_b / a.batch_steps
     cps = len(cases) / (200.0 * per_step)
     ms = info["sweep_ms"] - i0["sweep_ms"]
     gbs = (info["cell_sweeps"] - i0["cell_sweeps"]) * 13.0 / (ms * 1e-3) / 1e9 if ms > 0 else None
     # plain FP64 single-case path on every 32nd case, run one after another
+    # (same steps as the batch: the sampled cases' mixed results are compared
+    # with these FP64 results field by field)
     tot = 0.0
-    sample = cases[::32]
-    for c in sample:
+    sample = list(range(0, len(cases), 32))
+    err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
+    for k in sample:
+        c = cases[k]
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-6, check_every=a.fp64_check_every)
         s.graphs(not a.no_graphs)
@@ -174,7 +196,10 @@ def batched_c4(iff, configs, a):
         t0 = time.perf_counter()
         s.step(a.batch_steps)
         tot += time.perf_counter() - t0
+        e = fields_err(lambda f: b.get(slots[k], f), s.get)
+        err = {f: max(err[f], e[f]) for f in err}
         s.close()
+    b.close()
     single_cps = len(sample) / (200.0 * tot / a.batch_steps)
     log("batched: %.3f s/step for 256 cases, %.2f cases/s (FP64 single-case %.3f cases/s)"
         % (per_step, cps, single_cps))
@@ -182,6 +207,7 @@ def batched_c4(iff, configs, a):
                 ms_per_batched_step=round(1e3 * per_step, 2), mcell_steps_per_s=round(cells / per_step / 1e6, 2),
                 sweep_gbs=round(gbs, 1) if gbs else None, max_div=max(divs),
                 fp64_single_case_cases_per_s=round(single_cps, 4),
+                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
                 speedup_vs_fp64_single=round(cps / single_cps, 2), setup_s=round(t_setup, 2),
                 autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
                               check_every=tr["check_every"], sweep_ns_per_cell=round(tr["sweep_ns"], 4),
@@ -189,7 +215,44 @@ def batched_c4(iff, configs, a):
                               candidates=tr["candidates"], seconds=round(t_tune, 3)),
                 sample="all 256 cases resident; step 1 untimed, then autotune, then steps 2..%d of 200 timed; "
                        "FP64 single-case (library defaults) on 8 of them (every 32nd), same steps; rates per "
-                       "200-step case" % (1 + a.batch_steps))
+                       "200-step case; error_vs_fp64: max over those 8 of max|mixed - FP64| / max(1, max|FP64|) "
+                       "after step %d" % (1 + a.batch_steps, 1 + a.batch_steps))
+
+
+def complex_c3(iff, configs, a):
+    """The complex-geometry workload (4096^2, eight star bodies, Re 1000),
+    mixed precision: step 1 untimed, autotune, then --c3-steps timed; the
+    FP64 path (library defaults) runs the same steps for the error."""
+    cfg = configs.c3()
+    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+    kw = dict(dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6,
+              ppe_itermax=400000, ppe_tol=cfg["ppe_tol"], check_every=a.check_every)
+    c = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_MIXED, **kw)
+    c.graphs(not a.no_graphs)
+    c.persistent(not a.no_persistent)
+    c.step(1)
+    tr = c.autotune()
+    t0 = time.perf_counter()
+    st = c.step(a.c3_steps)
+    dt = time.perf_counter() - t0
+    c64 = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_FP64, **dict(kw, check_every=a.fp64_check_every))
+    t1 = time.perf_counter()
+    s64 = c64.step(1 + a.c3_steps)
+    dt64 = time.perf_counter() - t1
+    err = fields_err(c.get, c64.get)
+    c.close()
+    c64.close()
+    log("c3: %.3f s/step mixed, %.3f s/step FP64" % (dt / a.c3_steps, dt64 / (1 + a.c3_steps)))
+    return dict(value=round(NX * NY * a.c3_steps / dt / 1e6, 3), unit="Mcell-steps/s", NX=NX, NY=NY,
+                ms_per_step=round(1e3 * dt / a.c3_steps, 2), sweeps_last_step=st["ppe_sweeps"],
+                max_div=st["max_div"], ppe_tol=cfg["ppe_tol"],
+                fp64_mcell_steps_per_s=round(NX * NY * (1 + a.c3_steps) / dt64 / 1e6, 3),
+                fp64_max_div=s64["max_div"],
+                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
+                autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
+                              check_every=tr["check_every"]),
+                sample="step 1 untimed, autotune, steps 2..%d timed; FP64 from the same start, %d steps "
+                       "(the FP64 rate includes its first step)" % (1 + a.c3_steps, 1 + a.c3_steps))
 
 
 def serving_c5(iff, configs, a):
@@ -203,14 +266,23 @@ def serving_c5(iff, configs, a):
         c = dict(c, xf=xf, yf=xf.copy(), dt=0.5 * 20.0 / n, steps=max(5, c["steps"] // 20), ppe_tol=1e-6,
                  check_every=a.check_every)
         trace.append((t, c))
-    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune="auto")
+    sample = set(range(0, len(trace), 8))
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune="auto", keep_fields=sample)
     log("serving:", summ)
+    # the error of sampled results against the FP64 path (after the loop)
+    err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
+    for r in recs:
+        if r["id"] in sample and r["state"] == 1:
+            e = serve.error_vs_fp64(r, trace[r["id"]][1])
+            err = {f: max(err[f], e[f]) for f in err}
     return dict(value=round(summ["cases_per_s"], 3), unit="cases/s", requests=summ["requests"],
                 completed=summ["completed"], p50_s=round(summ["p50_s"], 4), p99_s=round(summ["p99_s"], 4),
                 mcell_steps_per_s=round(summ["mcell_steps_per_s"], 2), max_div=summ["max_div"],
                 max_resident=summ["max_resident"], budget_bytes=16 << 30,
+                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
                 sample="first %d requests of the seeded trace (mean 50 req/s), grids clipped to <= 512^2, "
-                       "steps/20 (5..50), 20%% heaving cylinders" % a.serve_requests)
+                       "steps/20 (5..50), 20%% heaving cylinders; error_vs_fp64: max over every 8th request of "
+                       "max|served - FP64 alone| / max(1, max|FP64|)" % a.serve_requests)
 
 
 def gpu_omega(cfg):
@@ -262,10 +334,18 @@ def main():
         r64 = timed_run(c64, max(1, a.warmup), a.fp64_steps, cells, "fp64")
         b64, _ = c64.byte_model()
         avg64 = r64["sweep_ms"] / max(1, r64["sweep_launches"])
+        # the mixed path's error: a fresh mixed case run the same steps
+        nst = max(1, a.warmup) + a.fp64_steps
+        cm = make_case(iff, cfg, iff.IF_MIXED, a)
+        cm.step(nst)
+        err = fields_err(cm.get, c64.get)
+        cm.close()
         fp64 = dict(value=round(r64["mcs"], 3), ms_per_step=round(1e3 * r64["seconds"] / a.fp64_steps, 3),
                     sweeps_last_step=r64["stats"]["ppe_sweeps"],
                     max_div=r64["stats"]["max_div"],
-                    sweep_gbs=round(b64 * cells / (avg64 * 1e-3) / 1e9, 1) if avg64 > 0 else None)
+                    sweep_gbs=round(b64 * cells / (avg64 * 1e-3) / 1e9, 1) if avg64 > 0 else None,
+                    mixed_error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
+                    error_after_steps=nst)
         c64.close()
 
     cpu = None
@@ -274,6 +354,7 @@ def main():
         log("cpu baseline", cpu)
 
     batched = None if a.no_batch else batched_c4(iff, configs, a)
+    c3 = None if a.no_c3 else complex_c3(iff, configs, a)
     serving = None if a.no_serve else serving_c5(iff, configs, a)
 
     st = mixed["stats"]
@@ -302,6 +383,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64": fp64,
         "batched": batched,
+        "complex": c3,
         "serving": serving,
         "step_bytes_per_cell": round(step_bpc, 1),
     }

@@ -295,10 +295,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     if (tile_in) {
 #pragma unroll
         for (int k = 0; k < NRW; k++) {
-            const size_t o = (size_t)(jt0 + k) * P + ga;
-            const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + o);
-            const V2 rv = gload_vec<V2>(rhs + o);
-            const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+            const size_t ro = (size_t)(jt0 + k) * P;  // wave-uniform row offset
+            const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + ro + ga);
+            const V2 rv = gload_vec<V2>(rhs + ro + ga);
+            const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
             R.pa[k] = pv[0];
             R.pb[k] = pv[1];
             R.ra[k] = rv[0];

@@ -13,14 +13,38 @@ import time
 
 import numpy as np
 
-from . import IF_FP64, IF_MIXED, Batch
+from . import IF_FP64, IF_MIXED, Batch, Case
+
+
+def _kwargs(c):
+    return dict(dt=c["dt"], re=c["re"], bodies=c.get("bodies", ()), ppe_tol=c.get("ppe_tol", 1e-6),
+                ppe_omega=c.get("ppe_omega", 0.0), ad_itermax=c.get("ad_itermax", 25),
+                check_every=c.get("check_every", 160), t0=c.get("t0", 0.0))
 
 
 def _add(batch, c):
-    return batch.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"],
-                     bodies=c.get("bodies", ()), ppe_tol=c.get("ppe_tol", 1e-6),
-                     ppe_omega=c.get("ppe_omega", 0.0), ad_itermax=c.get("ad_itermax", 25),
-                     check_every=c.get("check_every", 160), t0=c.get("t0", 0.0))
+    return batch.add(c["xf"], c["yf"], c["steps"], **_kwargs(c))
+
+
+def error_vs_fp64(rec, c):
+    """The error of a served (mixed-precision) result against the FP64 path:
+    runs request c alone in FP64 for the same number of steps and returns
+    max |served - FP64| / max(1, max |FP64|) for u, v and p, and for p with
+    each field's mean removed (p_mod_const).  rec must hold the served
+    fields (serve(..., keep_fields=...))."""
+    s = Case(c["xf"], c["yf"], precision=IF_FP64, **_kwargs(c))
+    try:
+        s.step(rec["steps"])
+        out = {}
+        rel = lambda a, b: float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b))))
+        for f in ("u", "v", "p"):
+            out[f] = rel(rec[f], s.get(f))
+        # pressure of the all-Neumann problem: fixed up to a constant
+        pf = s.get("p")
+        out["p_mod_const"] = rel(rec["p"] - rec["p"].mean(), pf - pf.mean())
+        return out
+    finally:
+        s.close()
 
 
 def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
@@ -31,7 +55,8 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
     the autotuner once on the resident batch after its first batched step.
     Returns (records, summary).  records: one dict per request with
     arrival / admitted / finished times (s, trace clock), latency, state,
-    steps, cells and the final max |div|."""
+    steps, cells and the final max |div|; with keep_fields (True, or a
+    collection of request ids) also the final u, v, p of those requests."""
     batch = Batch(precision, hbm_budget)
     auto = tune == "auto"
     if not auto:
@@ -81,7 +106,7 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
                 rec = dict(id=k, arrival=arr, admitted=adm, finished=now, latency=now - arr,
                            queue=adm - arr, state=state, steps=steps_done,
                            cells=(len(c["xf"]) - 1) * (len(c["yf"]) - 1), max_div=st["max_div"])
-                if keep_fields:
+                if keep_fields is True or (keep_fields and k in keep_fields):
                     rec["u"] = batch.get(slot, "u")
                     rec["v"] = batch.get(slot, "v")
                     rec["p"] = batch.get(slot, "p")

@@ -44,3 +44,19 @@ def test_served_results_equal_alone(require_gpu):
         for f in ("u", "v", "p"):
             assert np.array_equal(b.get(s, f), r[f]), (r["id"], f)
         b.close()
+
+
+def test_served_results_state_their_fp64_error(require_gpu):
+    """Sampled served results against the FP64 path run alone for the same
+    steps: within the mixed path's 1e-4 bound, with fields kept only for the
+    requested ids."""
+    print("[gpu-test] serving error vs fp64", flush=True)
+    trace = _trace()
+    keep = {0, 2, 4}
+    records, summary = serve.serve(trace, precision=iff.IF_MIXED, tune="auto", keep_fields=keep)
+    assert summary["completed"] == len(trace)
+    for r in records:
+        assert ("u" in r) == (r["id"] in keep)
+        if r["id"] in keep:
+            e = serve.error_vs_fp64(r, trace[r["id"]][1])
+            assert max(e["u"], e["v"], e["p"]) <= 1e-4, (r["id"], e)
