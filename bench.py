@@ -8,9 +8,10 @@ step is one full fractional step (predictor, divergence RHS, Poisson to
 max|div| <= 1e-6, projection, divergence check) on device-resident fields.
 
 value = interior cells x steps / wall seconds (Mcell-steps/s) of the
-mixed-precision path (FP32 stencils and Poisson sweeps, FP64 refinement,
-face velocities, residuals and divergence); the FP64 path is timed beside
-it.  roofline: the Poisson sweep kernel (the dominant kernel), algorithmic
+mixed-precision path (FP32 Poisson error sweeps, FP64 residual and defect
+correction, FP64 velocity passes, faces and divergence); the FP64 path and
+the FP32-velocity variant (IF_MIXED_V32) are timed beside it, each with its
+error against the FP64 path.  roofline: the Poisson sweep kernel (the dominant kernel), algorithmic
 bytes per launch / mean launch time from HIP events recorded by the library
 on its stream during the timed steps.  cpu_baseline: the C oracle (the same
 per-cell routines, OpenMP) timed for one step from the GPU's state.
@@ -53,6 +54,7 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-v32", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
@@ -361,6 +363,23 @@ def main():
                                "by the stopping error; at a 1e-9 stop (3 steps) only precision differs")
         c64.close()
 
+    # the FP32-velocity variant of the mixed path (IF_MIXED_V32), same case
+    v32 = None
+    if not a.no_v32:
+        cv = make_case(iff, cfg, iff.IF_MIXED_V32, a)
+        rv = timed_run(cv, a.warmup, a.steps, cells, "mixed_v32")
+        v32 = dict(value=round(rv["mcs"], 3), ms_per_step=round(1e3 * rv["seconds"] / a.steps, 3),
+                   max_div=rv["stats"]["max_div"])
+        cv.close()
+        if fp64 is not None:  # error after the FP64 run's step count, as for the mixed path
+            cv = make_case(iff, cfg, iff.IF_MIXED_V32, a)
+            cv.step(fp64["error_after_steps"])
+            c64 = make_case(iff, cfg, iff.IF_FP64, a)
+            c64.step(fp64["error_after_steps"])
+            v32["error_vs_fp64"] = {f: float("%.3g" % v) for f, v in fields_err(cv.get, c64.get).items()}
+            cv.close()
+            c64.close()
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(cfg, case, mixed["stats"], a.omega if a.omega > 0 else gpu_omega(cfg))
@@ -385,7 +404,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": cfg["name"], "NX": NX, "NY": NY, "dt": cfg["dt"], "Re": cfg["re"],
-                   "precision": "mixed (FP32 stencils/sweeps, FP64 refinement)",
+                   "precision": "mixed (FP32 Poisson error sweeps in registers, FP64 residual/defect "
+                                 "correction, FP64 velocity passes)",
                    "poisson": "red-black SOR, stop max|div|<=%g" % cfg["ppe_tol"],
                    "sweeps_last_step": st["ppe_sweeps"], "ad_iters_last_step": st["ad_iters"],
                    "max_div": st["max_div"], "parallelism": "single GPU",
@@ -395,6 +415,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64": fp64,
+        "mixed_fp32_velocity": v32,
         "batched": batched,
         "complex": c3,
         "serving": serving,

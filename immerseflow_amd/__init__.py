@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (IF_FIELD_FLAGS, IF_FIELD_IBLANK, IF_FIELD_P, IF_FIELD_U,  # noqa: F401
-                   IF_FIELD_UF, IF_FIELD_V, IF_FIELD_VF, IF_FP64, IF_MIXED,
+                   IF_FIELD_UF, IF_FIELD_V, IF_FIELD_VF, IF_FP64, IF_MIXED, IF_MIXED_V32,
                    IF_MODE_COMPAT, IF_MODE_PHYSICS, IF_PPE_JACOBI, IF_PPE_RBSOR, IF_PPE_LINE,
                    IF_BODY_CIRCLE, IF_BODY_ELLIPSE, IF_BODY_POLYGON, IF_MOTION_HEAVE)
 

@@ -9,7 +9,7 @@ IF_OK = 0
 IF_ERR_ARG, IF_ERR_IO, IF_ERR_PARSE, IF_ERR_HIP = -1, -2, -3, -4
 IF_ERR_NOMEM, IF_ERR_STATE, IF_ERR_NODEV, IF_ERR_BUDGET, IF_ERR_DIVERGED = -5, -6, -7, -8, -9
 IF_MODE_PHYSICS, IF_MODE_COMPAT = 0, 1
-IF_FP64, IF_MIXED = 0, 1
+IF_FP64, IF_MIXED, IF_MIXED_V32 = 0, 1, 2
 IF_PPE_JACOBI, IF_PPE_RBSOR, IF_PPE_LINE = 0, 1, 2
 IF_BODY_CIRCLE, IF_BODY_ELLIPSE, IF_BODY_POLYGON = 0, 1, 2
 IF_MOTION_NONE, IF_MOTION_HEAVE = 0, 1

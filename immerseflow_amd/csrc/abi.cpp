@@ -89,7 +89,7 @@ static if_status case_create(const if_case_desc *d, if_case **out)
         if_status rc = c->cc->create(*d);
         if (rc != IF_OK) return rc;
     } else if (d->mode == IF_MODE_PHYSICS) {
-        if (d->precision != IF_FP64 && d->precision != IF_MIXED)
+        if (d->precision != IF_FP64 && d->precision != IF_MIXED && d->precision != IF_MIXED_V32)
             return fail(IF_ERR_ARG, "if_case_create: unknown precision");
         c->eng = std::make_unique<Engine>(d->precision, 0);
         if_status rc = c->eng->init();
@@ -203,7 +203,7 @@ static if_status tune(Engine &e, int kind, int ty, int m, int k)
         return fail(IF_ERR_ARG, "tune: LDS sweep needs tile_y 8/16/32 and 1/2/4 sweeps per launch");
     if (kind == 1 && !stream_variant_ok(ty, m))
         return fail(IF_ERR_ARG, "tune: streaming sweep needs strip height 16/32/64/128 and 1/2/4 sweeps");
-    const int pr = e.precision();
+    const int pr = e.poisson_precision();
     if (kind == 2 && !reg_variant_ok(pr, 4, ty, m))
         return fail(IF_ERR_ARG, "tune: register-tile sweep needs tile_y 16/32/48 and 1/2/4 sweeps");
     if (kind == 3 && !reg_variant_ok(pr, 8, ty, m))
@@ -329,11 +329,11 @@ if_status if_case_byte_model(if_case *c, double *sweep_bpc, double *step_bpc)
     if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "byte model: physics mode only");
     Engine &e = *c->eng;
     Slot *s = e.slot(c->slot);
-    const bool mixed = s->prec == IF_MIXED;
+    const bool mixed = s->prec != IF_FP64;  // FP32 Poisson error sweeps
     // sweeps per launch: M, or the whole chunk when the last step ran
     // persistent chunks
     const int M = s->dev.ppe_solver == IF_PPE_RBSOR ? (e.launch_sweeps > 0 ? e.launch_sweeps : e.sweep_m) : 1;
-    const double ts = mixed ? 4 : 8;
+    const double ts = s->prec == IF_MIXED_V32 ? 4 : 8;  // velocity fields
     const double sweep = mixed ? 13.0 : 25.0;
     if (sweep_bpc) *sweep_bpc = sweep * M;
     if (step_bpc) {
@@ -355,7 +355,7 @@ static if_status batch_create(int precision, size_t budget, if_batch **out)
 {
     if (!out) return fail(IF_ERR_ARG, "if_batch_create: null output");
     *out = nullptr;
-    if (precision != IF_FP64 && precision != IF_MIXED)
+    if (precision != IF_FP64 && precision != IF_MIXED && precision != IF_MIXED_V32)
         return fail(IF_ERR_ARG, "if_batch_create: unknown precision");
     auto b = std::make_unique<if_batch>();
     b->eng = std::make_unique<Engine>(precision, budget);

@@ -135,7 +135,7 @@ struct CaseDev {
     int tile_y;
     int gcap, nbody;       // ghost-record capacity, bodies
     int moving;            // bodies move: reclassify every step
-    int prec;              // IF_FP64 / IF_MIXED
+    int prec;              // velocity storage: IF_FP64 (double u, v) or IF_MIXED (float)
     int ppe_solver;
     int ppe_m;             // sweeps per Poisson launch of this case's solver
     int ad_itermax, ppe_itermax, check_every;

@@ -61,6 +61,7 @@ public:
     size_t in_use() const { return used_; }
     size_t budget() const { return budget_; }
     int precision() const { return prec_; }
+    int poisson_precision() const { return pprec_; }
     Slot *slot(int k) { return (k >= 0 && k < (int)slots_.size()) ? slots_[k].get() : nullptr; }
 
     // Poisson sweep variant and check cadence (autotuned or user-set)
@@ -95,7 +96,9 @@ private:
     if_status check_barrier();
     void drop_graph();
 
-    int prec_;
+    int prec_;   // IF_FP64 / IF_MIXED / IF_MIXED_V32
+    int vprec_;  // velocity passes: IF_FP64, or IF_MIXED (FP32) for IF_MIXED_V32
+    int pprec_;  // Poisson: IF_FP64, or IF_MIXED (FP32 error sweeps) for both mixed modes
     size_t budget_, used_ = 0;
     hipStream_t st_ = nullptr;
     std::vector<std::unique_ptr<Slot>> slots_;

@@ -30,7 +30,9 @@ static int reg_waves(int kind) { return kind == 4 ? 16 : kind == 3 ? 8 : 4; }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Engine::Engine(int prec, size_t budget) : prec_(prec), budget_(budget)
+Engine::Engine(int prec, size_t budget)
+    : prec_(prec), vprec_(prec == IF_MIXED_V32 ? IF_MIXED : IF_FP64), pprec_(prec == IF_FP64 ? IF_FP64 : IF_MIXED),
+      budget_(budget)
 {
     // default Poisson launch shape, by measurement on the 2048 x 1024
     // ellipse: FP32 sweeps are fastest on 16-wave tiles at 8 sweeps per
@@ -109,7 +111,7 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
 {
     ArenaPlan a{};
     size_t o = 0;
-    const size_t ts = prec == IF_FP64 ? 8 : 4;
+    const size_t ts = prec == IF_MIXED_V32 ? 4 : 8;  // u, v, sx, sy
     auto fieldT = [&](size_t es) {
         const size_t lead = 256 / es - 1;
         const size_t n = (size_t)P * ny + lead + 64;
@@ -125,7 +127,7 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
     for (int b = 0; b < 2; b++) a.off_vf[b] = fieldT(8);
     for (int b = 0; b < 2; b++) a.off_p[b] = fieldT(8);
     a.off_rhs = fieldT(8);
-    if (prec == IF_MIXED) {
+    if (prec != IF_FP64) {
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
     }
@@ -318,7 +320,7 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.tile_y = kTileY;
     c.nbody = d.nbody;
     c.moving = s.moving ? 1 : 0;
-    c.prec = prec_;
+    c.prec = vprec_;
     c.ppe_solver = d.ppe_solver;
     c.ad_itermax = d.ad_itermax;
     c.ppe_itermax = d.ppe_itermax;
@@ -343,12 +345,12 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
         c.uf[b] = (double *)(A + a.off_uf[b]);
         c.vf[b] = (double *)(A + a.off_vf[b]);
         c.p[b] = (double *)(A + a.off_p[b]);
-        if (prec_ == IF_MIXED) c.e[b] = (float *)(A + a.off_e[b]);
+        if (pprec_ == IF_MIXED) c.e[b] = (float *)(A + a.off_e[b]);
     }
     c.sx = A + a.off_sx;
     c.sy = A + a.off_sy;
     c.rhs = (double *)(A + a.off_rhs);
-    if (prec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
+    if (pprec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
     c.flags = (uint8_t *)(A + a.off_flags);
     c.bodyid = (int8_t *)(A + a.off_bodyid);
     c.part = (double *)(A + a.off_part);
@@ -445,7 +447,7 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     launch_ghost_weights(st_, tmp, 1, s.gcap, 0);
 
     // initial state (u = ubc, v = vbc in fluid; ghost values; faces; p = 0)
-    launch_init_fields(st_, prec_, tmp, 1, s.nx * s.ny);
+    launch_init_fields(st_, vprec_, tmp, 1, s.nx * s.ny);
     {
         // ghost kernels need an active case
         Ctrl on = ct;
@@ -453,13 +455,13 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
         on.ng = s.ng;
         on.ng_total = s.ng;
         IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
-        launch_ghost_vel(st_, prec_, tmp, 1, s.gcap, 1);
+        launch_ghost_vel(st_, vprec_, tmp, 1, s.gcap, 1);
         std::vector<int2> tl(s.ntx * s.nty);
         for (int t = 0; t < (int)tl.size(); t++) tl[t] = make_int2(0, t);
         int2 *dtl = nullptr;
         IFF_HIP(hipMalloc(&dtl, sizeof(int2) * tl.size()));
         IFF_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_));
-        launch_faces(st_, prec_, tmp, dtl, (int)tl.size(), 0, 0);
+        launch_faces(st_, vprec_, tmp, dtl, (int)tl.size(), 0, 0);
         on.active = 0;
         IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
         const hipError_t e = hipStreamSynchronize(st_);
@@ -584,17 +586,17 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
     for (int s = 0; s < K; s++) {
         if (any_rb && !pers && s % M == 0) {
             if (sweep_kind >= 2)
-                launch_ppe_sweep_reg(st_, prec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
+                launch_ppe_sweep_reg(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
                                      nt_sweep_, s, K);
             else if (sweep_kind == 1)
-                launch_ppe_sweep_stream(st_, prec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+                launch_ppe_sweep_stream(st_, pprec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
             else
-                launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
+                launch_ppe_sweep(st_, pprec_, IF_PPE_RBSOR, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
                                  s, K);
         }
         if (any_jac)  // point Jacobi runs on the base tiling
-            launch_ppe_sweep(st_, prec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
-        if (any_line) launch_line_sor(st_, prec_, d_cases_, n, max_ring_, s, K);
+            launch_ppe_sweep(st_, pprec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
+        if (any_line) launch_line_sor(st_, pprec_, d_cases_, n, max_ring_, s, K);
     }
 }
 
@@ -771,7 +773,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             if (rc != IF_OK) return rc;
         }
         launch_step_begin(st_, d_cases_, n);
-        launch_source(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
+        launch_source(st_, vprec_, d_cases_, d_tiles_base_, nt_base_);
 
         // ---- P2: viscous Jacobi sweeps, ended per case on the device
         auto spec = [&](int maxn, int which, auto &&unit) -> if_status {
@@ -798,20 +800,20 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             return IF_OK;
         };
         if_status rc = spec(max_ad_, 0, [&](int) {
-            launch_ad_sweep(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
-            launch_ghost_vel(st_, prec_, d_cases_, n, max_ng_, 0);
+            launch_ad_sweep(st_, vprec_, d_cases_, d_tiles_base_, nt_base_);
+            launch_ghost_vel(st_, vprec_, d_cases_, n, max_ng_, 0);
             launch_ad_finalize(st_, d_cases_, n);
         });
         if (rc != IF_OK) return rc;
 
         // ---- P3: faces + RHS
-        launch_faces(st_, prec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
+        launch_faces(st_, vprec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
 
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;
         int timed = 0;
         // persistent chunk: mixed precision, register tiles, grid fits
-        const bool pers = any_rb && use_persistent && prec_ != IF_FP64 && sweep_kind >= 2 &&
+        const bool pers = any_rb && use_persistent && pprec_ != IF_FP64 && sweep_kind >= 2 &&
                           pers_fits(reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
         ran_pers = ran_pers || pers;
         if (any_rb) launch_sweeps = pers ? K : M;
@@ -822,7 +824,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         hipError_t glaunch = hipSuccess;
         rc = spec(nunits, 1, [&](int u) {
             if (u == 0) {
-                launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
+                launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
                 return;
             }
             const bool tm = timing && (u - 1) < 64;
@@ -837,16 +839,16 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
                 hipEventRecord(tev_[2 * (u - 1) + 1], st_);
                 timed = u;
             }
-            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, K);
+            launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, K);
         });
         if (rc != IF_OK) return rc;
         if (glaunch != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph launch: ") + hipGetErrorString(glaunch));
 
         // ---- pressure BC, projection, divergence, ghost cells
         launch_pressure_bc(st_, d_cases_, n, max_ring_, max_ng_);
-        launch_project(st_, prec_, d_cases_, d_tiles_base_, nt_base_);
-        launch_ghost_vel(st_, prec_, d_cases_, n, max_ng_, 1);
-        if (any_monitor_) launch_monitor(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_);
+        launch_project(st_, vprec_, d_cases_, d_tiles_base_, nt_base_);
+        launch_ghost_vel(st_, vprec_, d_cases_, n, max_ng_, 1);
+        if (any_monitor_) launch_monitor(st_, vprec_, d_cases_, n, d_tiles_base_, nt_base_);
         launch_step_end(st_, d_cases_, n);
         for (int id : order_) slots_[id]->steps_host++;
 
@@ -970,12 +972,12 @@ if_status Engine::autotune(if_tune_result *out)
     constexpr int kReps = 6;
     auto sweep_launch = [&](const Shape &c, int s, int K) {
         if (c.kind >= 2)
-            launch_ppe_sweep_reg(st_, prec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
+            launch_ppe_sweep_reg(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
                                  s, K);
         else if (c.kind == 1)
-            launch_ppe_sweep_stream(st_, prec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+            launch_ppe_sweep_stream(st_, pprec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
         else
-            launch_ppe_sweep(st_, prec_, IF_PPE_RBSOR, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
+            launch_ppe_sweep(st_, pprec_, IF_PPE_RBSOR, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
     };
     auto elapsed = [&](float *ms) -> bool {
         return hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(ms, e0, e1) == hipSuccess;
@@ -985,7 +987,7 @@ if_status Engine::autotune(if_tune_result *out)
     double best_ns = 1e300;
     int ncand = 0;
     for (const Shape &c : cands) {
-        if (c.kind >= 2 && !reg_variant_ok(prec_, reg_waves(c.kind), c.ty, c.m)) continue;
+        if (c.kind >= 2 && !reg_variant_ok(pprec_, reg_waves(c.kind), c.ty, c.m)) continue;
         sweep_kind = c.kind;
         sweep_ty = c.ty;
         sweep_m = c.m;
@@ -994,10 +996,10 @@ if_status Engine::autotune(if_tune_result *out)
         // a shape that will run as persistent chunks is timed that way, over
         // 32 stages (a typical check interval); it writes only the scratch
         // error buffers and advances the stage counters of every tile alike
-        const bool pc = prec_ != IF_FP64 && c.kind >= 2 && use_persistent &&
+        const bool pc = pprec_ != IF_FP64 && c.kind >= 2 && use_persistent &&
                         pers_fits(reg_waves(c.kind), c.ty, c.m, nt_sweep_);
         const int kp = 32 * c.m;
-        const int s0 = prec_ == IF_FP64 ? 0 : c.m;
+        const int s0 = pprec_ == IF_FP64 ? 0 : c.m;
         auto timed = [&]() {
             if (pc)
                 launch_ppe_sweep_pers(st_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
@@ -1034,7 +1036,7 @@ if_status Engine::autotune(if_tune_result *out)
         (void)hipEventRecord(e0, st_);
         for (int r = 0; r < kReps; r++) {
             launch_tune_arm(st_, d_cases_, n, d_ctrl_, saved);
-            launch_ppe_check(st_, prec_, d_cases_, n, d_tiles_base_, nt_base_, 2 * best.m);
+            launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, 2 * best.m);
         }
         (void)hipEventRecord(e1, st_);
         float ms = 0.f;
@@ -1047,7 +1049,7 @@ if_status Engine::autotune(if_tune_result *out)
         hipStreamSynchronize(st_) == hipSuccess;
     cleanup();
     if (!restored) return fail(IF_ERR_HIP, "autotune: control-block restore failed");
-    if (rc == IF_OK && prec_ != IF_FP64) rc = check_barrier();
+    if (rc == IF_OK && pprec_ != IF_FP64) rc = check_barrier();
     if (rc != IF_OK) {
         sweep_kind = keep.kind;
         sweep_ty = keep.ty;
@@ -1115,7 +1117,7 @@ if_status Engine::get_field(int k, int which, double *out, size_t n)
     std::vector<uint8_t> hb;
     const CaseDev &c = s->dev;
     auto fetch_ts = [&](void *p) -> if_status {
-        if (prec_ == IF_FP64) {
+        if (vprec_ == IF_FP64) {
             hd.resize(np);
             IFF_HIP(hipMemcpy(hd.data(), p, np * 8, hipMemcpyDeviceToHost));
         } else {
@@ -1190,7 +1192,7 @@ if_status Engine::set_field(int k, int which, const double *in, size_t n)
     case IF_FIELD_VF: dst = c.vf[0]; break;
     }
     std::vector<double> hd(np, 0.0);
-    if (ts && prec_ == IF_MIXED) {
+    if (ts && vprec_ == IF_MIXED) {
         std::vector<float> hf(np);
         IFF_HIP(hipMemcpy(hf.data(), dst, np * 4, hipMemcpyDeviceToHost));
         hd.assign(hf.begin(), hf.end());
@@ -1207,7 +1209,7 @@ if_status Engine::set_field(int k, int which, const double *in, size_t n)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) hd[i + (size_t)j * P] = in[i + (size_t)j * nx];
     }
-    if (ts && prec_ == IF_MIXED) {
+    if (ts && vprec_ == IF_MIXED) {
         std::vector<float> hf(hd.begin(), hd.end());
         IFF_HIP(hipMemcpy(dst, hf.data(), np * 4, hipMemcpyHostToDevice));
     } else {
@@ -1349,8 +1351,8 @@ if_status Engine::restart(int k, const double *u, const double *v, const double 
               hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_) == hipSuccess &&
               hipMemcpyAsync(s->dev.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_) == hipSuccess;
     if (ok) {
-        launch_ghost_vel(st_, prec_, tmp, 1, s->gcap, 1);
-        if (!uf) launch_faces(st_, prec_, tmp, dtl, (int)tl.size(), 0, 0);
+        launch_ghost_vel(st_, vprec_, tmp, 1, s->gcap, 1);
+        if (!uf) launch_faces(st_, vprec_, tmp, dtl, (int)tl.size(), 0, 0);
         ok = hipMemcpyAsync(s->dev.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_) == hipSuccess &&
              hipStreamSynchronize(st_) == hipSuccess;
     }

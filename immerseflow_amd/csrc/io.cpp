@@ -140,6 +140,7 @@ if_status parse_inputs(const char *path, if_inputs *out)
             else return fail(IF_ERR_PARSE, "unknown mode '" + mode + "'");
             if (prec == "fp64") out->precision = IF_FP64;
             else if (prec == "mixed") out->precision = IF_MIXED;
+            else if (prec == "mixed_v32") out->precision = IF_MIXED_V32;
             else return fail(IF_ERR_PARSE, "unknown precision '" + prec + "'");
         } else if (line.find("Body") != std::string::npos) {
             std::getline(f, line);

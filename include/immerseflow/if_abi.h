@@ -45,7 +45,15 @@ enum {
 };
 
 enum { IF_MODE_PHYSICS = 0, IF_MODE_COMPAT = 1 };
-enum { IF_FP64 = 0, IF_MIXED = 1 };
+/* Precision.  IF_FP64: every pass in FP64, the oracle's expressions.
+ * IF_MIXED: the Poisson error sweeps in FP32 (the FP64 residual and defect
+ * correction around them), everything else FP64.  IF_MIXED_V32: as
+ * IF_MIXED, and the velocity stencils (convective source, viscous sweeps,
+ * projection) with u, v stored in FP32 as well; faster on the velocity
+ * passes, but the pressure then answers the FP32 rounding of the predicted
+ * velocity (dp ~ eps32 L / dt), about 1e-4 relative on the workloads here
+ * against under 1e-5 for IF_MIXED. */
+enum { IF_FP64 = 0, IF_MIXED = 1, IF_MIXED_V32 = 2 };
 enum { IF_PPE_JACOBI = 0, IF_PPE_RBSOR = 1, IF_PPE_LINE = 2 };
 enum { IF_BODY_CIRCLE = 0, IF_BODY_ELLIPSE = 1, IF_BODY_POLYGON = 2 };
 enum { IF_MOTION_NONE = 0, IF_MOTION_HEAVE = 1 };
@@ -83,7 +91,7 @@ typedef struct {
     int write_interval;
     /* extension ("Fractional step" block) */
     int mode;               /* IF_MODE_*; default compat */
-    int precision;          /* IF_FP64 / IF_MIXED */
+    int precision;          /* IF_FP64 / IF_MIXED / IF_MIXED_V32 */
     double conv_factor;     /* default 0.5 (ADSolver.cu:66-74) */
     double ppe_omega;       /* SOR relaxation; <= 0: from the grid */
     double ppe_tol;         /* dt*||b - A p||_inf stop level */
@@ -97,7 +105,7 @@ typedef struct {
  * readGridData + initializeData, preSim.cu:147-240,242-372). */
 typedef struct {
     int mode;               /* IF_MODE_* */
-    int precision;          /* IF_FP64 / IF_MIXED (physics mode) */
+    int precision;          /* IF_FP64 / IF_MIXED / IF_MIXED_V32 (physics mode) */
     int nxf, nyf;           /* face counts (NX+1, NY+1) */
     const double *xf, *yf;  /* face coordinates (grid-file contents) */
     double dt, re;

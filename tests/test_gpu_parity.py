@@ -345,10 +345,11 @@ def test_persistent_chunks_equal_launch_per_stage(require_gpu, kind, ty, m, K, g
             assert np.array_equal(runs[0][k], runs[1][k]), k
 
 
-def test_mixed_precision_against_fp64_oracle(require_gpu):
+@pytest.mark.parametrize("prec", [iff.IF_MIXED, iff.IF_MIXED_V32])
+def test_mixed_precision_against_fp64_oracle(require_gpu, prec):
     xf, yf, bodies = CASES["circle-stretched"]
-    _say("mixed precision")
-    oc, gc = _pair(xf, yf, bodies, prec=iff.IF_MIXED, dt=0.01, ppe_tol=1e-9, check_every=10)
+    _say("mixed precision", prec)
+    oc, gc = _pair(xf, yf, bodies, prec=prec, dt=0.01, ppe_tol=1e-9, check_every=10)
     for _ in range(4):
         oc.step(1)
         g = gc.step(1)
