@@ -166,6 +166,9 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
 constexpr unsigned long long kBarTimeout = 2000000;  // 20 ms
 constexpr int kDev = 16;                             // cache policy: sc1 (device scope)
 constexpr unsigned kRsrc3 = 0x00020000;              // buffer resource word 3 (gfx9)
+// s_waitcnt operand (gfx9 encoding) waiting for vmcnt = 0 only: vmcnt
+// bits 3:0 and 15:14 zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
+constexpr int kWaitVmcnt0 = 0x0F70;
 
 // wait (lanes 0-7 of wave 0, one neighbour each; nb < 0: none) until
 // flags[nb] has reached target
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             if (last) break;
             // publish stage l once every wave's stores are complete, wait
             // for the neighbours' stage l
-            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             __syncthreads();
             if (threadIdx.x == 0)
                 __hip_atomic_store(&flags[t], f0 + (unsigned)(l + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
