@@ -25,6 +25,9 @@ namespace iff {
 
 constexpr int kTileY = 16;  // base tiling (reductions, stencil passes)
 
+// viscous iteration budgets up to this are enqueued without polling
+constexpr int kAdEnqueueAll = 32;
+
 // waves per workgroup of the register-tile sweep kinds (2, 3, 4)
 static int reg_waves(int kind) { return kind == 4 ? 16 : kind == 3 ? 8 : 4; }
 
@@ -799,11 +802,19 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             }
             return IF_OK;
         };
-        if_status rc = spec(max_ad_, 0, [&](int) {
+        auto ad_unit = [&](int) {
             launch_ad_sweep(st_, vprec_, d_cases_, d_tiles_base_, nt_base_);
             launch_ghost_vel(st_, vprec_, d_cases_, n, max_ng_, 0);
             launch_ad_finalize(st_, d_cases_, n);
-        });
+        };
+        // A short viscous loop (the usual 25-iteration budget) is enqueued
+        // whole: the kernels end each case's iterations on the device, and
+        // polling snapshots would cost a copy per iteration.
+        if_status rc = IF_OK;
+        if (max_ad_ <= kAdEnqueueAll)
+            for (int u = 0; u < max_ad_; u++) ad_unit(u);
+        else
+            rc = spec(max_ad_, 0, ad_unit);
         if (rc != IF_OK) return rc;
 
         // ---- P3: faces + RHS
