@@ -112,10 +112,12 @@ struct Ctrl {
     double max_div;
 };
 
-// Synchronisation state of the persistent Poisson sweep (one per engine,
-// zeroed at creation): a sticky error flag set when a wait times out.
+// Synchronisation state of the persistent Poisson sweep (one per engine):
+// a sticky error flag set when a wait times out, and the wait limit in
+// ticks of the 100 MHz real-time counter.
 struct GridBar {
-    unsigned err, pad[3];
+    unsigned err, pad;
+    unsigned long long timeout;
 };
 
 // Everything a kernel needs to know about one case.

@@ -14,6 +14,7 @@
 // case converged instead of waiting for a result after every unit.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -80,7 +81,13 @@ if_status Engine::init()
     IFF_HIP(hipMalloc(&d_ctrl_, sizeof(Ctrl) * cap_ctrl_));
     IFF_HIP(hipMemsetAsync(d_ctrl_, 0, sizeof(Ctrl) * cap_ctrl_, st_));
     IFF_HIP(hipMalloc(&d_bar_, sizeof(GridBar)));
-    IFF_HIP(hipMemsetAsync(d_bar_, 0, sizeof(GridBar), st_));
+    // neighbour waits of the persistent sweep give up after 20 ms
+    // (IFF_PERSISTENT_TIMEOUT_TICKS overrides it, for tests of the time-out
+    // path); b lives until the synchronisation below
+    GridBar b{};
+    b.timeout = 2000000;
+    if (const char *e = std::getenv("IFF_PERSISTENT_TIMEOUT_TICKS")) b.timeout = std::strtoull(e, nullptr, 10);
+    IFF_HIP(hipMemcpyAsync(d_bar_, &b, sizeof(GridBar), hipMemcpyHostToDevice, st_));
     IFF_HIP(hipHostMalloc((void **)&h_snap_, sizeof(Ctrl) * cap_ctrl_ * kRing, hipHostMallocDefault));
     for (auto &e : ev_) IFF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &e : tev_) IFF_HIP(hipEventCreate(&e));
@@ -637,7 +644,7 @@ if_status Engine::check_barrier()
     if (b.err == 0) return IF_OK;
     use_persistent = false;
     drop_graph();
-    IFF_HIP(hipMemsetAsync(d_bar_, 0, sizeof(GridBar), st_));
+    IFF_HIP(hipMemsetAsync(&d_bar_->err, 0, sizeof(unsigned), st_));
     if (cap_flags_ > 0) IFF_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned) * cap_flags_, st_));
     IFF_HIP(hipStreamSynchronize(st_));
     return fail(IF_ERR_HIP, "persistent Poisson sweep: a neighbour wait timed out (blocks not co-resident); "

@@ -160,10 +160,10 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
 // until the 8 neighbouring tiles of its case have published the same stage.
 // Strips are written and halos read at device scope (sc1: write-through,
 // and loads that miss in the per-XCD L2), so no cache flush is needed; the
-// waits are bounded: past kBarTimeout ticks of the 100 MHz real-time
-// counter bar->err is set and every later wait returns at once, so the
-// kernel always ends; the host checks err after the step.
-constexpr unsigned long long kBarTimeout = 2000000;  // 20 ms
+// waits are bounded: past bar->timeout ticks of the 100 MHz real-time
+// counter bar->err is set, every later wait returns at once and every
+// block stops at its next stage boundary, so the kernel always ends; the
+// host checks err after the step.
 constexpr int kDev = 16;                             // cache policy: sc1 (device scope)
 constexpr unsigned kRsrc3 = 0x00020000;              // buffer resource word 3 (gfx9)
 // s_waitcnt operand (gfx9 encoding) waiting for vmcnt = 0 only: vmcnt
@@ -176,9 +176,10 @@ __device__ __forceinline__ void wait_neighbours(const unsigned *flags, int nb, u
 {
     if (nb >= 0) {
         const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        const unsigned long long tmo = bar->timeout;
         while ((int)(__hip_atomic_load(&flags[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
             if (__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > kBarTimeout) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > tmo) {
                 __hip_atomic_store(&bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -515,7 +516,12 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             if (threadIdx.x == 0)
                 __hip_atomic_store(&flags[t], f0 + (unsigned)(l + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             wait_neighbours(flags, nb, f0 + (unsigned)(l + 1), bar);
-            __syncthreads();
+            // after a timed-out wait anywhere in the launch, stop sweeping:
+            // the error buffers keep finite values, the step reports the
+            // failure and the solve resumes from the current pressure
+            const int bad =
+                threadIdx.x == 0 ? (int)__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            if (__syncthreads_or(bad)) break;
             // halo ring: the cells of the tile outside its stored region
             // that lie in the domain's interior (device-scope loads)
             const bool lh = lane < H / 2 || lane >= 64 - H / 2;
