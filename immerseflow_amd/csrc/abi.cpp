@@ -418,6 +418,8 @@ if_status if_batch_info(if_batch *b, int *resident, size_t *bytes, double *ms, l
 if_status if_case_footprint(const if_case_desc *d, int precision, size_t *bytes)
 {
     if (!d || !bytes) return fail(IF_ERR_ARG, "if_case_footprint: null argument");
+    if (precision != IF_FP64 && precision != IF_MIXED && precision != IF_MIXED_V32)
+        return fail(IF_ERR_ARG, "if_case_footprint: unknown precision");
     *bytes = Engine::footprint(*d, precision);
     return IF_OK;
 }

@@ -189,3 +189,18 @@ def test_case_descriptor_validation_without_device():
     rc = lib.if_case_create(C.byref(d), C.byref(h))
     assert rc in (_abi.IF_ERR_ARG, _abi.IF_ERR_NODEV)
     assert lib.if_case_step(None, 1, None) == _abi.IF_ERR_ARG
+
+
+def test_footprint_by_precision():
+    """Device bytes a case needs (host-side plan, no device): the mixed mode
+    adds the FP32 error and residual fields to the FP64 fields; the
+    FP32-velocity mode stores u, v and the sources in 4 bytes."""
+    xf = np.linspace(0.0, 16.0, 257)
+    fp = {p: iff.Batch.footprint(xf, xf, precision=p, dt=0.01, re=100.0, bodies=[iff.circle(5.0, 8.0, 0.5)])
+          for p in (iff.IF_FP64, iff.IF_MIXED, iff.IF_MIXED_V32)}
+    cells = 258 * 258
+    assert fp[iff.IF_MIXED] > fp[iff.IF_FP64] > fp[iff.IF_MIXED_V32]
+    # three FP32 fields more than FP64 (e twice, r32), to within the padding
+    assert 10 * cells <= fp[iff.IF_MIXED] - fp[iff.IF_FP64] <= 16 * cells
+    with pytest.raises(iff.ImmerseFlowError):
+        iff.Batch.footprint(xf, xf, precision=7, dt=0.01, re=100.0)
