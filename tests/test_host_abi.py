@@ -71,6 +71,14 @@ heave 5 8 0.5 0.5 0.2
     assert d["bodies"][3]["motion"] == iff.IF_MOTION_HEAVE
     # the reference-era keys are unchanged by the extension
     assert (d["nx"], d["ny"], d["ad_itermax"], d["dt"]) == (52, 52, 25, 0.001)
+    # the other precision words
+    for word, prec in (("fp64", iff.IF_FP64), ("mixed_v32", iff.IF_MIXED_V32)):
+        p.write_text(deck.replace("physics  mixed ", "physics  %s " % word))
+        assert iff.read_input_file(str(p))["precision"] == prec
+    p.write_text(deck.replace("physics  mixed ", "physics  half "))
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.read_input_file(str(p))
+    assert e.value.code == _abi.IF_ERR_PARSE
 
 
 def test_parse_errors(tmp_path):
