@@ -207,7 +207,8 @@ def batched_c4(iff, configs, a):
         % (per_step, cps, single_cps))
     return dict(value=round(cps, 3), unit="cases/s", cases=len(cases), steps_timed=a.batch_steps,
                 ms_per_batched_step=round(1e3 * per_step, 2), mcell_steps_per_s=round(cells / per_step / 1e6, 2),
-                sweep_gbs=round(gbs, 1) if gbs else None, max_div=max(divs),
+                sweep_gbs=round(gbs, 1) if gbs else None,
+                sweep_roofline_frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None, max_div=max(divs),
                 fp64_single_case_cases_per_s=round(single_cps, 4),
                 error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
                 speedup_vs_fp64_single=round(cps / single_cps, 2), setup_s=round(t_setup, 2),
