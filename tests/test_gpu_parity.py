@@ -317,10 +317,12 @@ def test_graph_replay_equals_direct_launches(require_gpu, prec):
                                                  (4, 48, 4, 12, True), (3, 32, 4, 10, True)])
 def test_persistent_chunks_equal_launch_per_stage(require_gpu, kind, ty, m, K, graphs):
     """A chunk run as one persistent launch (tiles kept in registers, halo
-    strips exchanged through a grid barrier between stages) gives the same
-    bits as one launch per stage: in a batch with a point-Jacobi case (its
-    blocks only take part in the barriers), a case whose sweep budget ends
-    inside a chunk, and chunks that are not a whole number of stages."""
+    strips exchanged between neighbouring tiles between stages) gives the
+    same bits as one launch per stage: in a batch with a point-Jacobi case
+    (its blocks only advance their stage counters), a case whose sweep
+    budget ends inside a chunk, a heaving body (flags, ghosts and weights
+    rebuilt every step), and chunks that are not a whole number of
+    stages."""
     xf, yf, bodies = CASES["circle-stretched"]
     xf2, yf2, bodies2 = CASES["ellipse"]
     _say("persistent vs per-stage launches", kind, ty, m, K, graphs)
@@ -334,7 +336,8 @@ def test_persistent_chunks_equal_launch_per_stage(require_gpu, kind, ty, m, K, g
         sl = [b.add(xf, yf, 3, bodies=bodies, **kw),
               b.add(xf2, yf2, 3, bodies=bodies2, **kw),
               b.add(xf2, yf2, 3, bodies=bodies2, **dict(kw, ppe_itermax=3 * K + 5)),
-              b.add(xf, yf, 3, bodies=bodies, **dict(kw, ppe_solver=0, ppe_omega=0.8))]
+              b.add(xf, yf, 3, bodies=bodies, **dict(kw, ppe_solver=0, ppe_omega=0.8)),
+              b.add(xf2, yf2, 3, bodies=[iff.heaving_circle(2.0, 1.0, 0.3, 0.2, 0.5)], **kw)]
         b.step(3)
         runs.append({(k, f): b.get(s, f) for k, s in enumerate(sl) for f in ("u", "v", "p", "uf", "vf")})
         runs[-1]["sweeps"] = [b.state(s)[2]["total_ppe_sweeps"] for s in sl]
