@@ -624,16 +624,38 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 
 // Zebra line SOR (the deck's PPE_Solver 2, inputs/inputs.txt:14-15), one
 // colour of rows per launch: rows j odd (colour 1) first, then rows j even.
-// One thread per row solves the row's tridiagonal system (Thomas algorithm)
-// -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs for each fluid
-// cell with sa != 0 (identity rows elsewhere), then relaxes, p + om (x - p).
-// The forward-elimination coefficients and x go to the case's predictor
-// source arrays (free during the Poisson phase).  Reads the unknown from
+// Each row's tridiagonal system
+//     -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs
+// (fluid cells with sa != 0; identity rows elsewhere) is solved by the
+// Thomas algorithm and relaxed, p + om (x - p).  Reads the unknown from
 // buffer src (own row, and for colour 1 the rows above and below), writes
 // dst; colour 0 reads its neighbour rows from dst (just updated).  FP64 is
 // the oracle's expression and order (line_sor_sweep, oracle.c).
+//
+// The elimination is a serial recurrence along a row, so a row costs the
+// latency of its dependence chain.  A block owns 64 rows of one colour;
+// wave 0 runs the recurrences (one lane per row, c' and d' carried in
+// registers) on a chunk of kLineW columns staged in LDS, while the other
+// waves ("movers") write the previous chunk's results back and stage the
+// next chunk with coalesced row loads, double-buffered, one barrier per
+// chunk.  Forward elimination leaves c', d' in the case's predictor source
+// arrays (free during the Poisson phase); the backward pass stages c', d'
+// right to left, the recurrence turns d' into x in place, and the movers
+// relax and store.
+constexpr int kLineW = 32;                          // columns per chunk (a multiple of 8)
+constexpr int kLineMovers = 4;                      // mover waves per block
+constexpr int kLineThreads = 64 * (1 + kLineMovers);
+constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 fluid cell
+
+template <typename T> struct LineStage {
+    T a[2][64][kLineW + 1];           // forward: b, then c'; backward: c'
+    T d[2][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
+    uint8_t f[2][64][kLineW + 4];     // forward: cell flags | kLineSolved
+    T yp[64], ym[64];                 // the rows' y coefficients
+};
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
+__global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
 {
     const CaseDev &c = cs[blockIdx.y];
     const Ctrl *ct = c.ctrl;
@@ -642,8 +664,9 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
     if (kexec > K) kexec = K;
     if (s >= kexec) return;
     const int nx = c.nx, ny = c.ny, P = c.P;
-    const int j = 2 - colour + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j > ny - 2) return;
+    const int jlast = ny - 2;
+    const int j0 = 2 - colour + 128 * (int)blockIdx.x;  // local row r is j0 + 2 r
+    if (j0 > jlast) return;
     const T *src;
     T *dst;
     const T *rhs;
@@ -664,53 +687,208 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
         cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
         om = c.omega_f;
     }
-    const T *nbr = colour == 1 ? src : dst;  // rows above and below
     const bool zero_nbr = colour == 1 && zero_src;
-    T *cpv = fld<T>(c.sx), *dpv = fld<T>(c.sy);
-    const size_t row = (size_t)j * P;
-    const T yp = cyp[j], ym = cym[j];
-    for (int i = 1; i <= nx - 2; i++) {
-        const size_t id = row + i;
-        const uint8_t f = c.flags[id];
-        const T pold = zero_src ? (T)0 : src[id];
-        T l = (T)0, b = (T)1, u = (T)0, d = pold;
-        if (f & F_FLUID) {
-            const T aE = (f & F_OPEN_E) ? cxp[i] : (T)0;
-            const T aW = (f & F_OPEN_W) ? cxm[i] : (T)0;
-            const T aN = (f & F_OPEN_N) ? yp : (T)0;
-            const T aS = (f & F_OPEN_S) ? ym : (T)0;
-            const T sa = (aE + aW) + (aN + aS);
-            if (sa != (T)0) {
-                const T pN = zero_nbr ? (T)0 : nbr[id + P], pS = zero_nbr ? (T)0 : nbr[id - P];
-                l = -aW;
-                b = sa;
-                u = -aE;
-                d = (aN * pN + aS * pS) - rhs[id];
+    // global address space (the case descriptor's pointers are generic)
+    const auto gsrc = G(src), gnbr = G(colour == 1 ? src : dst), grhs = G(rhs);  // nbr: rows above and below
+    const auto gcxp = G(cxp), gcxm = G(cxm), gcyp = G(cyp), gcym = G(cym);
+    const auto gflags = G(c.flags);
+    const auto gdst = GW(dst), gcpv = GW(fld<T>(c.sx)), gdpv = GW(fld<T>(c.sy));
+
+    __shared__ LineStage<T> st;
+    const int n = nx - 2;  // a row's cells are i = 1..n
+    const int nch = (n + kLineW - 1) / kLineW;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int kMv = 64 * kLineMovers;
+    constexpr int kPer = 64 * kLineW / kMv;  // cells per mover thread and chunk
+    static_assert(kMv % kLineW == 0 && kLineW % 8 == 0, "line chunk shape");
+    // mover thread mt owns column mt % kLineW of every chunk and local rows
+    // mt / kLineW + q kMv / kLineW (q < kPer)
+    const int mt = (int)threadIdx.x - 64;
+    const int mcol = mt % kLineW, mrow = mt / kLineW;
+    if (threadIdx.x < 64) {
+        const int j = min(j0 + 2 * (int)threadIdx.x, jlast);
+        st.yp[threadIdx.x] = gcyp[j];
+        st.ym[threadIdx.x] = gcym[j];
+    }
+    __syncthreads();
+
+    // ---- forward elimination ------------------------------------------
+    T cprev = (T)0, dprev = (T)0;
+    for (int t = 0; t < nch + 2; t++) {
+        if (wave == 0) {
+            if (t >= 1 && t <= nch) {
+                const int b = (t - 1) & 1;
+                const int i0 = 1 + (t - 1) * kLineW;
+                const int kn = min(kLineW, n - i0 + 1);
+                // groups of 8 cells: the group's LDS and coefficient loads
+                // are issued ahead of its recurrence.  Cells past the row's
+                // end (last chunk only) run on clamped inputs and are never
+                // stored.
+                for (int k8 = 0; k8 < kn; k8 += 8) {
+                    T bb[8], dd[8], xp[8], xm[8];
+                    unsigned ff[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int i = min(i0 + k8 + q, n);
+                        xp[q] = gcxp[i];
+                        xm[q] = gcxm[i];
+                        ff[q] = st.f[b][lane][k8 + q];
+                        bb[q] = st.a[b][lane][k8 + q];
+                        dd[q] = st.d[b][lane][k8 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const unsigned f = ff[q];
+                        const bool solved = f & kLineSolved;
+                        const T aE = (f & F_OPEN_E) ? xp[q] : (T)0;
+                        const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
+                        const T l = solved ? -aW : (T)0, u = solved ? -aE : (T)0;
+                        T cpi, dpi;
+                        if (i0 + k8 + q == 1) {
+                            cpi = u / bb[q];
+                            dpi = dd[q] / bb[q];
+                        } else {
+                            const T m = bb[q] - l * cprev;
+                            cpi = u / m;
+                            dpi = (dd[q] - l * dprev) / m;
+                        }
+                        cprev = cpi;
+                        dprev = dpi;
+                        st.a[b][lane][k8 + q] = cpi;
+                        st.d[b][lane][k8 + q] = dpi;
+                    }
+                }
+            }
+        } else {
+            const int b = t & 1;
+            if (t >= 2) {  // results of chunk t-2
+                const int i = 1 + (t - 2) * kLineW + mcol;
+#pragma unroll
+                for (int q = 0; q < kPer; q++) {
+                    const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
+                    if (i <= n && j <= jlast) {
+                        const size_t id = (size_t)j * P + i;
+                        gcpv[id] = st.a[b][rl][mcol];
+                        gdpv[id] = st.d[b][rl][mcol];
+                    }
+                }
+            }
+            if (t < nch) {  // stage chunk t (indices clamped into the row)
+                const int i = min(1 + t * kLineW + mcol, n);
+                const T xp = gcxp[i], xm = gcxm[i];
+                unsigned fv[kPer];
+                T po[kPer], pn[kPer], ps[kPer], rh[kPer];
+#pragma unroll
+                for (int q = 0; q < kPer; q++) {  // every load first
+                    const int j = min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast);
+                    const size_t id = (size_t)j * P + i;
+                    fv[q] = gflags[id];
+                    po[q] = gsrc[id];
+                    pn[q] = gnbr[id + P];
+                    ps[q] = gnbr[id - P];
+                    rh[q] = grhs[id];
+                }
+#pragma unroll
+                for (int q = 0; q < kPer; q++) {
+                    const int rl = mrow + q * (kMv / kLineW);
+                    const unsigned f = fv[q];
+                    T bb = (T)1, d = zero_src ? (T)0 : po[q];
+                    unsigned fs = f;
+                    if (f & F_FLUID) {
+                        const T aE = (f & F_OPEN_E) ? xp : (T)0;
+                        const T aW = (f & F_OPEN_W) ? xm : (T)0;
+                        const T aN = (f & F_OPEN_N) ? st.yp[rl] : (T)0;
+                        const T aS = (f & F_OPEN_S) ? st.ym[rl] : (T)0;
+                        const T sa = (aE + aW) + (aN + aS);
+                        if (sa != (T)0) {
+                            const T vn = zero_nbr ? (T)0 : pn[q], vs = zero_nbr ? (T)0 : ps[q];
+                            bb = sa;
+                            d = (aN * vn + aS * vs) - rh[q];
+                            fs |= kLineSolved;
+                        }
+                    }
+                    st.f[b][rl][mcol] = (uint8_t)fs;
+                    st.a[b][rl][mcol] = bb;
+                    st.d[b][rl][mcol] = d;
+                }
             }
         }
-        if (i == 1) {
-            cpv[id] = u / b;
-            dpv[id] = d / b;
-        } else {
-            const T m = b - l * cpv[id - 1];
-            cpv[id] = u / m;
-            dpv[id] = (d - l * dpv[id - 1]) / m;
-        }
+        __syncthreads();
     }
-    for (int i = nx - 3; i >= 1; i--) dpv[row + i] = dpv[row + i] - cpv[row + i] * dpv[row + i + 1];
-    for (int i = 1; i <= nx - 2; i++) {
-        const size_t id = row + i;
-        const uint8_t f = c.flags[id];
-        const T pold = zero_src ? (T)0 : src[id];
-        T val = pold;
-        if (f & F_FLUID) {
-            const T aE = (f & F_OPEN_E) ? cxp[i] : (T)0;
-            const T aW = (f & F_OPEN_W) ? cxm[i] : (T)0;
-            const T aN = (f & F_OPEN_N) ? yp : (T)0;
-            const T aS = (f & F_OPEN_S) ? ym : (T)0;
-            if ((aE + aW) + (aN + aS) != (T)0) val = pold + om * (dpv[id] - pold);
+
+    // ---- back substitution (right to left), relaxation, store -----------
+    T x = (T)0;
+    for (int t = 0; t < nch + 2; t++) {
+        if (wave == 0) {
+            if (t >= 1 && t <= nch) {
+                const int b = (t - 1) & 1;
+                const int i0 = 1 + (nch - t) * kLineW;
+                const int kn = min(kLineW, n - i0 + 1);
+                // groups of 8 cells from the right; in the row's last
+                // chunk the cells past its end come first and x restarts
+                // at i = n
+                for (int k8 = ((kn - 1) & ~7); k8 >= 0; k8 -= 8) {
+                    T ca[8], da[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        ca[q] = st.a[b][lane][k8 + q];
+                        da[q] = st.d[b][lane][k8 + q];
+                    }
+#pragma unroll
+                    for (int q = 7; q >= 0; q--) {
+                        x = (i0 + k8 + q == n) ? da[q] : da[q] - ca[q] * x;
+                        st.d[b][lane][k8 + q] = x;
+                    }
+                }
+            }
+        } else {
+            const int b = t & 1;
+            const int is = 1 + (nch + 1 - t) * kLineW + mcol;    // chunk staged at step t-2
+            const int isc = min(is, n);
+            const int il = max(1, min(1 + (nch - 1 - t) * kLineW + mcol, n));  // chunk staged at step t
+            const T xp = gcxp[isc], xm = gcxm[isc];
+            unsigned fv[kPer] = {};
+            T po[kPer] = {}, ca[kPer] = {}, da[kPer] = {};
+            if (t >= 2) {  // every load first
+#pragma unroll
+                for (int q = 0; q < kPer; q++) {
+                    const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + isc;
+                    fv[q] = gflags[id];
+                    po[q] = gsrc[id];
+                }
+            }
+            if (t < nch) {
+#pragma unroll
+                for (int q = 0; q < kPer; q++) {
+                    const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + il;
+                    ca[q] = gcpv[id];
+                    da[q] = gdpv[id];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
+                if (t >= 2) {
+                    const unsigned f = fv[q];
+                    const T pold = zero_src ? (T)0 : po[q];
+                    const T xv = st.d[b][rl][mcol];
+                    T val = pold;
+                    if (f & F_FLUID) {
+                        const T aE = (f & F_OPEN_E) ? xp : (T)0;
+                        const T aW = (f & F_OPEN_W) ? xm : (T)0;
+                        const T aN = (f & F_OPEN_N) ? st.yp[rl] : (T)0;
+                        const T aS = (f & F_OPEN_S) ? st.ym[rl] : (T)0;
+                        if ((aE + aW) + (aN + aS) != (T)0) val = pold + om * (xv - pold);
+                    }
+                    if (is <= n && j <= jlast) gdst[(size_t)j * P + is] = val;
+                }
+                if (t < nch) {
+                    st.a[b][rl][mcol] = ca[q];
+                    st.d[b][rl][mcol] = da[q];
+                }
+            }
         }
-        dst[id] = val;
+        __syncthreads();
     }
 }
 
@@ -720,9 +898,9 @@ void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int
     const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)ncase);
     for (int colour = 1; colour >= 0; colour--) {
         if (prec == IF_FP64)
-            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(64), 0, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(kLineThreads), 0, st, cs, s, K, colour);
         else
-            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(64), 0, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(kLineThreads), 0, st, cs, s, K, colour);
     }
 }
 
