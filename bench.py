@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-v32", action="store_true")
+    ap.add_argument("--no-line", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
@@ -258,6 +259,38 @@ def complex_c3(iff, configs, a):
                        "(the FP64 rate includes its first step)" % (1 + a.c3_steps, 1 + a.c3_steps))
 
 
+def line_sor_f2(iff, configs, a):
+    """The deck's PPE_Solver 2 (zebra line SOR, one Thomas solve per row):
+    whole steps of the 256^2 cylinder to its 1e-8 stop, FP64 and mixed; and
+    the time per sweep on the 2048x1024 workload, from the sweep events of a
+    solve capped at 400 sweeps (a full solve there takes ~10^4 sweeps)."""
+    out = {}
+    for cname, cfg, steps, cap in (("c1", configs.c1(), 2, 200000), ("c2", configs.c2(), 1, 400)):
+        NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+        for pname, prec in (("fp64", iff.IF_FP64), ("mixed", iff.IF_MIXED)):
+            c = iff.Case(cfg["xf"], cfg["yf"], precision=prec, dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"],
+                         ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_LINE, ppe_itermax=cap,
+                         ppe_tol=cfg["ppe_tol"], check_every=40)
+            c.step(1)
+            c.sweep_timing(reset=True)
+            t0 = time.perf_counter()
+            st = c.step(steps)
+            dt = time.perf_counter() - t0
+            ms, nsw = c.sweep_timing(reset=True)
+            c.close()
+            us = 1e3 * ms / max(1, nsw)
+            r = dict(us_per_sweep=round(us, 2), mcell_sweeps_per_s=round(NX * NY / us, 1) if us > 0 else None)
+            if cname == "c1":
+                r.update(ms_per_step=round(1e3 * dt / steps, 2), sweeps_last_step=st["ppe_sweeps"],
+                         max_div=st["max_div"])
+            out["%s_%s" % (cname, pname)] = r
+            log("line sor", cname, pname, r)
+    out["sample"] = ("c1: 256^2 cylinder, step 1 untimed, steps 2..3 timed to max|div| <= 1e-8; c2: 2048x1024, "
+                     "one step with the solve capped at 400 sweeps; us_per_sweep from HIP events around the "
+                     "sweep chunks (both colours, forward elimination and back substitution)")
+    return out
+
+
 def serving_c5(iff, configs, a):
     """A bounded sample of the serving workload: the seeded Poisson-arrival
     trace (50 req/s) with grids clipped to <= 512^2 and step counts / 20."""
@@ -389,6 +422,7 @@ def main():
     batched = None if a.no_batch else batched_c4(iff, configs, a)
     c3 = None if a.no_c3 else complex_c3(iff, configs, a)
     serving = None if a.no_serve else serving_c5(iff, configs, a)
+    line = None if a.no_line else line_sor_f2(iff, configs, a)
 
     st = mixed["stats"]
     out = {
@@ -420,6 +454,7 @@ def main():
         "batched": batched,
         "complex": c3,
         "serving": serving,
+        "line_sor": line,
         "step_bytes_per_cell": round(step_bpc, 1),
     }
     case.close()

@@ -633,24 +633,28 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 // the oracle's expression and order (line_sor_sweep, oracle.c).
 //
 // The elimination is a serial recurrence along a row, so a row costs the
-// latency of its dependence chain.  A block owns 64 rows of one colour;
-// wave 0 runs the recurrences (one lane per row, c' and d' carried in
-// registers) on a chunk of kLineW columns staged in LDS, while the other
-// waves ("movers") write the previous chunk's results back and stage the
-// next chunk with coalesced row loads, double-buffered, one barrier per
-// chunk.  Forward elimination leaves c', d' in the case's predictor source
-// arrays (free during the Poisson phase); the backward pass stages c', d'
-// right to left, the recurrence turns d' into x in place, and the movers
-// relax and store.
+// latency of its dependence chain.  A block owns 64 rows of one colour and
+// walks them in chunks of kLineW columns staged in LDS, one lane per row.
+// Forward elimination is a pipeline over three LDS buffers with one
+// barrier per stage: the "mover" waves stage chunk t with coalesced row
+// loads (and write chunk t-3's results back), wave 0 runs the c' chain
+// (m = b - l c'[i-1], c' = u / m) on chunk t-1 and wave 1 the d' chain
+// (d' = (d - l d'[i-1]) / m) on chunk t-2; c' depends on the coefficients
+// only, so the two divisions per cell run on different SIMDs.  c' and d'
+// go to the case's predictor source arrays (free during the Poisson
+// phase).  The back substitution stages c', d' right to left (two
+// buffers), wave 0 turns d' into x in place, and the movers relax and
+// store.
 constexpr int kLineW = 32;                          // columns per chunk (a multiple of 8)
 constexpr int kLineMovers = 4;                      // mover waves per block
-constexpr int kLineThreads = 64 * (1 + kLineMovers);
+constexpr int kLineThreads = 64 * (2 + kLineMovers);
 constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 fluid cell
 
 template <typename T> struct LineStage {
-    T a[2][64][kLineW + 1];           // forward: b, then c'; backward: c'
-    T d[2][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
-    uint8_t f[2][64][kLineW + 4];     // forward: cell flags | kLineSolved
+    T a[3][64][kLineW + 1];           // forward: b, then c'; backward: c'
+    T d[3][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
+    T m[3][64][kLineW + 1];           // forward: the pivots m
+    uint8_t f[3][64][kLineW + 4];     // forward: cell flags | kLineSolved
     T yp[64], ym[64];                 // the rows' y coefficients
 };
 
@@ -700,10 +704,10 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int kMv = 64 * kLineMovers;
     constexpr int kPer = 64 * kLineW / kMv;  // cells per mover thread and chunk
-    static_assert(kMv % kLineW == 0 && kLineW % 8 == 0, "line chunk shape");
+    static_assert(kMv % kLineW == 0 && (64 * kLineW) % kMv == 0 && kLineW % 8 == 0, "line chunk shape");
     // mover thread mt owns column mt % kLineW of every chunk and local rows
     // mt / kLineW + q kMv / kLineW (q < kPer)
-    const int mt = (int)threadIdx.x - 64;
+    const int mt = (int)threadIdx.x - 128;
     const int mcol = mt % kLineW, mrow = mt / kLineW;
     if (threadIdx.x < 64) {
         const int j = min(j0 + 2 * (int)threadIdx.x, jlast);
@@ -714,10 +718,10 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
 
     // ---- forward elimination ------------------------------------------
     T cprev = (T)0, dprev = (T)0;
-    for (int t = 0; t < nch + 2; t++) {
-        if (wave == 0) {
+    for (int t = 0; t < nch + 3; t++) {
+        if (wave == 0) {  // c' chain on chunk t-1
             if (t >= 1 && t <= nch) {
-                const int b = (t - 1) & 1;
+                const int b = (t - 1) % 3;
                 const int i0 = 1 + (t - 1) * kLineW;
                 const int kn = min(kLineW, n - i0 + 1);
                 // groups of 8 cells: the group's LDS and coefficient loads
@@ -725,7 +729,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                 // end (last chunk only) run on clamped inputs and are never
                 // stored.
                 for (int k8 = 0; k8 < kn; k8 += 8) {
-                    T bb[8], dd[8], xp[8], xm[8];
+                    T bb[8], xp[8], xm[8];
                     unsigned ff[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
@@ -734,7 +738,6 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                         xm[q] = gcxm[i];
                         ff[q] = st.f[b][lane][k8 + q];
                         bb[q] = st.a[b][lane][k8 + q];
-                        dd[q] = st.d[b][lane][k8 + q];
                     }
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
@@ -743,26 +746,51 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                         const T aE = (f & F_OPEN_E) ? xp[q] : (T)0;
                         const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
                         const T l = solved ? -aW : (T)0, u = solved ? -aE : (T)0;
-                        T cpi, dpi;
+                        T m, cpi;
                         if (i0 + k8 + q == 1) {
+                            m = bb[q];
                             cpi = u / bb[q];
-                            dpi = dd[q] / bb[q];
                         } else {
-                            const T m = bb[q] - l * cprev;
+                            m = bb[q] - l * cprev;
                             cpi = u / m;
-                            dpi = (dd[q] - l * dprev) / m;
                         }
                         cprev = cpi;
-                        dprev = dpi;
                         st.a[b][lane][k8 + q] = cpi;
+                        st.m[b][lane][k8 + q] = m;
+                    }
+                }
+            }
+        } else if (wave == 1) {  // d' chain on chunk t-2
+            if (t >= 2 && t <= nch + 1) {
+                const int b = (t - 2) % 3;
+                const int i0 = 1 + (t - 2) * kLineW;
+                const int kn = min(kLineW, n - i0 + 1);
+                for (int k8 = 0; k8 < kn; k8 += 8) {
+                    T dd[8], mm[8], xm[8];
+                    unsigned ff[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        xm[q] = gcxm[min(i0 + k8 + q, n)];
+                        ff[q] = st.f[b][lane][k8 + q];
+                        dd[q] = st.d[b][lane][k8 + q];
+                        mm[q] = st.m[b][lane][k8 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const unsigned f = ff[q];
+                        const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
+                        const T l = (f & kLineSolved) ? -aW : (T)0;
+                        // the first cell: m is b, and d' = d / b
+                        const T dpi = (i0 + k8 + q == 1) ? dd[q] / mm[q] : (dd[q] - l * dprev) / mm[q];
+                        dprev = dpi;
                         st.d[b][lane][k8 + q] = dpi;
                     }
                 }
             }
         } else {
-            const int b = t & 1;
-            if (t >= 2) {  // results of chunk t-2
-                const int i = 1 + (t - 2) * kLineW + mcol;
+            const int b = t % 3;
+            if (t >= 3) {  // results of chunk t-3
+                const int i = 1 + (t - 3) * kLineW + mcol;
 #pragma unroll
                 for (int q = 0; q < kPer; q++) {
                     const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
@@ -841,7 +869,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                     }
                 }
             }
-        } else {
+        } else if (wave >= 2) {
             const int b = t & 1;
             const int is = 1 + (nch + 1 - t) * kLineW + mcol;    // chunk staged at step t-2
             const int isc = min(is, n);
