@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-v32", action="store_true")
     ap.add_argument("--no-line", action="store_true")
+    ap.add_argument("--line-cases", default="c1,c2", help="line-SOR workloads to time (c1, c2)")
     ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--serve-requests", type=int, default=32)
     return ap.parse_args()
@@ -265,7 +266,9 @@ def line_sor_f2(iff, configs, a):
     the time per sweep on the 2048x1024 workload, from the sweep events of a
     solve capped at 400 sweeps (a full solve there takes ~10^4 sweeps)."""
     out = {}
-    for cname, cfg, steps, cap in (("c1", configs.c1(), 2, 200000), ("c2", configs.c2(), 1, 400)):
+    runs = [("c1", configs.c1, 2, 200000), ("c2", configs.c2, 1, 400)]
+    for cname, mk, steps, cap in (r for r in runs if r[0] in a.line_cases.split(",")):
+        cfg = mk()
         NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
         for pname, prec in (("fp64", iff.IF_FP64), ("mixed", iff.IF_MIXED)):
             c = iff.Case(cfg["xf"], cfg["yf"], precision=prec, dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"],

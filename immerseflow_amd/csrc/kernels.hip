@@ -717,6 +717,27 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     __syncthreads();
 
     // ---- forward elimination ------------------------------------------
+    // The movers load a chunk's inputs one stage before they stage it, so
+    // the loads are in flight across a barrier (indices clamped into the
+    // row).
+    unsigned fv[kPer];
+    T po[kPer], pn[kPer], ps[kPer], rh[kPer], xp = (T)0, xm = (T)0;
+    auto fwd_load = [&](int ch) {
+        const int i = min(1 + ch * kLineW + mcol, n);
+        xp = gcxp[i];
+        xm = gcxm[i];
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int j = min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast);
+            const size_t id = (size_t)j * P + i;
+            fv[q] = gflags[id];
+            po[q] = gsrc[id];
+            pn[q] = gnbr[id + P];
+            ps[q] = gnbr[id - P];
+            rh[q] = grhs[id];
+        }
+    };
+    if (wave >= 2) fwd_load(0);
     T cprev = (T)0, dprev = (T)0;
     for (int t = 0; t < nch + 3; t++) {
         if (wave == 0) {  // c' chain on chunk t-1
@@ -801,21 +822,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                     }
                 }
             }
-            if (t < nch) {  // stage chunk t (indices clamped into the row)
-                const int i = min(1 + t * kLineW + mcol, n);
-                const T xp = gcxp[i], xm = gcxm[i];
-                unsigned fv[kPer];
-                T po[kPer], pn[kPer], ps[kPer], rh[kPer];
-#pragma unroll
-                for (int q = 0; q < kPer; q++) {  // every load first
-                    const int j = min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast);
-                    const size_t id = (size_t)j * P + i;
-                    fv[q] = gflags[id];
-                    po[q] = gsrc[id];
-                    pn[q] = gnbr[id + P];
-                    ps[q] = gnbr[id - P];
-                    rh[q] = grhs[id];
-                }
+            if (t < nch) {  // stage chunk t from the registers loaded a stage ago
 #pragma unroll
                 for (int q = 0; q < kPer; q++) {
                     const int rl = mrow + q * (kMv / kLineW);
@@ -840,11 +847,38 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                     st.d[b][rl][mcol] = d;
                 }
             }
+            if (t + 1 < nch) fwd_load(t + 1);
         }
         __syncthreads();
     }
 
     // ---- back substitution (right to left), relaxation, store -----------
+    // Mover registers for step t, loaded at step t-1: the cell flags and old
+    // values of the chunk staged at step t-2 (relaxed and stored at step t:
+    // fv, po, xp, xm) and c', d' of the chunk staged at step t (pn, ps).
+    auto bwd_load = [&](int t) {
+        if (t >= 2 && t <= nch + 1) {
+            const int i = min(1 + (nch + 1 - t) * kLineW + mcol, n);
+            xp = gcxp[i];
+            xm = gcxm[i];
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + i;
+                fv[q] = gflags[id];
+                po[q] = gsrc[id];
+            }
+        }
+        if (t < nch) {
+            const int i = max(1, min(1 + (nch - 1 - t) * kLineW + mcol, n));
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + i;
+                pn[q] = gcpv[id];
+                ps[q] = gdpv[id];
+            }
+        }
+    };
+    if (wave >= 2) bwd_load(0);
     T x = (T)0;
     for (int t = 0; t < nch + 2; t++) {
         if (wave == 0) {
@@ -871,28 +905,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
             }
         } else if (wave >= 2) {
             const int b = t & 1;
-            const int is = 1 + (nch + 1 - t) * kLineW + mcol;    // chunk staged at step t-2
-            const int isc = min(is, n);
-            const int il = max(1, min(1 + (nch - 1 - t) * kLineW + mcol, n));  // chunk staged at step t
-            const T xp = gcxp[isc], xm = gcxm[isc];
-            unsigned fv[kPer] = {};
-            T po[kPer] = {}, ca[kPer] = {}, da[kPer] = {};
-            if (t >= 2) {  // every load first
-#pragma unroll
-                for (int q = 0; q < kPer; q++) {
-                    const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + isc;
-                    fv[q] = gflags[id];
-                    po[q] = gsrc[id];
-                }
-            }
-            if (t < nch) {
-#pragma unroll
-                for (int q = 0; q < kPer; q++) {
-                    const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + il;
-                    ca[q] = gcpv[id];
-                    da[q] = gdpv[id];
-                }
-            }
+            const int is = 1 + (nch + 1 - t) * kLineW + mcol;  // chunk staged at step t-2
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
                 const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
@@ -911,10 +924,11 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                     if (is <= n && j <= jlast) gdst[(size_t)j * P + is] = val;
                 }
                 if (t < nch) {
-                    st.a[b][rl][mcol] = ca[q];
-                    st.d[b][rl][mcol] = da[q];
+                    st.a[b][rl][mcol] = pn[q];
+                    st.d[b][rl][mcol] = ps[q];
                 }
             }
+            bwd_load(t + 1);
         }
         __syncthreads();
     }
