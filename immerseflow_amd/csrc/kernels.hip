@@ -640,7 +640,9 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 // loads (and write chunk t-3's results back), wave 0 runs the c' chain
 // (m = b - l c'[i-1], c' = u / m) on chunk t-1 and wave 1 the d' chain
 // (d' = (d - l d'[i-1]) / m) on chunk t-2; c' depends on the coefficients
-// only, so the two divisions per cell run on different SIMDs.  c' and d'
+// only, so the two divisions per cell run on different SIMDs (FP64 divides
+// as the oracle does; the FP32 error sweeps form 1/m once with the hardware
+// reciprocal and both chains multiply by it).  c' and d'
 // go to the case's predictor source arrays (free during the Poisson
 // phase).  The back substitution stages c', d' right to left (two
 // buffers), wave 0 turns d' into x in place, and the movers relax and
@@ -653,7 +655,7 @@ constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 flui
 template <typename T> struct LineStage {
     T a[3][64][kLineW + 1];           // forward: b, then c'; backward: c'
     T d[3][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
-    T m[3][64][kLineW + 1];           // forward: the pivots m
+    T m[3][64][kLineW + 1];           // forward: the pivots m (FP32: 1/m)
     uint8_t f[3][64][kLineW + 4];     // forward: cell flags | kLineSolved
     T yp[64], ym[64];                 // the rows' y coefficients
 };
@@ -767,17 +769,18 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                         const T aE = (f & F_OPEN_E) ? xp[q] : (T)0;
                         const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
                         const T l = solved ? -aW : (T)0, u = solved ? -aE : (T)0;
-                        T m, cpi;
-                        if (i0 + k8 + q == 1) {
-                            m = bb[q];
-                            cpi = u / bb[q];
-                        } else {
-                            m = bb[q] - l * cprev;
+                        const T m = (i0 + k8 + q == 1) ? bb[q] : bb[q] - l * cprev;
+                        T cpi, mq;
+                        if constexpr (sizeof(T) == 8) {  // IEEE division, as the oracle
                             cpi = u / m;
+                            mq = m;
+                        } else {  // FP32 error sweeps: the hardware reciprocal
+                            mq = __builtin_amdgcn_rcpf(m);
+                            cpi = u * mq;
                         }
                         cprev = cpi;
                         st.a[b][lane][k8 + q] = cpi;
-                        st.m[b][lane][k8 + q] = m;
+                        st.m[b][lane][k8 + q] = mq;
                     }
                 }
             }
@@ -801,8 +804,14 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                         const unsigned f = ff[q];
                         const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
                         const T l = (f & kLineSolved) ? -aW : (T)0;
-                        // the first cell: m is b, and d' = d / b
-                        const T dpi = (i0 + k8 + q == 1) ? dd[q] / mm[q] : (dd[q] - l * dprev) / mm[q];
+                        // the first cell: m is b, and d' = d / b.  FP32
+                        // holds 1/m (the hardware reciprocal) in place of m.
+                        const T num = (i0 + k8 + q == 1) ? dd[q] : dd[q] - l * dprev;
+                        T dpi;
+                        if constexpr (sizeof(T) == 8)
+                            dpi = num / mm[q];
+                        else
+                            dpi = num * mm[q];
                         dprev = dpi;
                         st.d[b][lane][k8 + q] = dpi;
                     }
