@@ -423,17 +423,28 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     ct.t = s.t0;
     IFF_HIP(hipMemcpyAsync(c.ctrl, &ct, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
 
-    // classification and ghost list: count first, then size the records
-    CaseDev *tmp = nullptr;
-    IFF_HIP(hipMalloc(&tmp, sizeof(CaseDev)));
-    auto done_tmp = [&]() { hipFree(tmp); };
+    // classification and ghost list: count first, then size the records.
+    // The scratch descriptor and tile list are freed on every return, after
+    // the stream has drained (an early error return leaves work queued).
+    struct Scratch {
+        hipStream_t st;
+        CaseDev *tmp = nullptr;
+        int2 *dtl = nullptr;
+        ~Scratch()
+        {
+            if (tmp || dtl) (void)hipStreamSynchronize(st);
+            if (tmp) (void)hipFree(tmp);
+            if (dtl) (void)hipFree(dtl);
+        }
+    } scr{st_};
+    IFF_HIP(hipMalloc(&scr.tmp, sizeof(CaseDev)));
+    CaseDev *tmp = scr.tmp;
     IFF_HIP(hipMemcpyAsync(tmp, &c, sizeof(CaseDev), hipMemcpyHostToDevice, st_));
     launch_classify(st_, tmp, 1, s.nx * s.ny, 0);
     launch_ghost_build(st_, tmp, 1, s.nx * s.ny, 0);
     Ctrl back{};
     if (hipMemcpyAsync(&back, c.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st_) != hipSuccess ||
         hipStreamSynchronize(st_) != hipSuccess) {
-        done_tmp();
         return fail(IF_ERR_HIP, std::string("case setup failed: ") + hipGetErrorString(hipGetLastError()));
     }
     const int total = back.ng_total;
@@ -443,7 +454,6 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     if (s.gcap > 0) {
         if (hipMalloc(&c.gh, sizeof(GhostDev) * (size_t)s.gcap) != hipSuccess) {
             c.gh = nullptr;
-            done_tmp();
             return fail(IF_ERR_NOMEM, "ghost record allocation failed");
         }
         s.bytes += sizeof(GhostDev) * (size_t)s.gcap;
@@ -459,29 +469,23 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     // initial state (u = ubc, v = vbc in fluid; ghost values; faces; p = 0)
     launch_init_fields(st_, vprec_, tmp, 1, s.nx * s.ny);
     {
-        // ghost kernels need an active case
-        Ctrl on = ct;
+        // ghost kernels need an active case; the two control blocks are
+        // separate host objects, both alive until the synchronisation
+        Ctrl on = ct, off = ct;
         on.active = 1;
-        on.ng = s.ng;
-        on.ng_total = s.ng;
+        on.ng = off.ng = s.ng;
+        on.ng_total = off.ng_total = s.ng;
         IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
         launch_ghost_vel(st_, vprec_, tmp, 1, s.gcap, 1);
         std::vector<int2> tl(s.ntx * s.nty);
         for (int t = 0; t < (int)tl.size(); t++) tl[t] = make_int2(0, t);
-        int2 *dtl = nullptr;
-        IFF_HIP(hipMalloc(&dtl, sizeof(int2) * tl.size()));
-        IFF_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_));
-        launch_faces(st_, vprec_, tmp, dtl, (int)tl.size(), 0, 0);
-        on.active = 0;
-        IFF_HIP(hipMemcpyAsync(c.ctrl, &on, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+        IFF_HIP(hipMalloc(&scr.dtl, sizeof(int2) * tl.size()));
+        IFF_HIP(hipMemcpyAsync(scr.dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_));
+        launch_faces(st_, vprec_, tmp, scr.dtl, (int)tl.size(), 0, 0);
+        IFF_HIP(hipMemcpyAsync(c.ctrl, &off, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
         const hipError_t e = hipStreamSynchronize(st_);
-        hipFree(dtl);
-        if (e != hipSuccess) {
-            done_tmp();
-            return fail(IF_ERR_HIP, std::string("case init failed: ") + hipGetErrorString(e));
-        }
+        if (e != hipSuccess) return fail(IF_ERR_HIP, std::string("case init failed: ") + hipGetErrorString(e));
     }
-    done_tmp();
     return IF_OK;
 }
 
@@ -535,6 +539,7 @@ if_status Engine::remove(int k)
     used_ -= s->bytes;
     Ctrl z{};
     IFF_HIP(hipMemcpyAsync(d_ctrl_ + k, &z, sizeof(Ctrl), hipMemcpyHostToDevice, st_));
+    IFF_HIP(hipStreamSynchronize(st_));  // z is a local
     slots_[k].reset();
     dirty_ = true;
     return IF_OK;
