@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--tile-y", type=int, default=96)
     ap.add_argument("--sweeps-per-launch", type=int, default=8)
     ap.add_argument("--check-every", type=int, default=240)
-    ap.add_argument("--fp64-kernel", type=int, default=3)
-    ap.add_argument("--fp64-tile-y", type=int, default=32)
+    ap.add_argument("--fp64-kernel", type=int, default=4)
+    ap.add_argument("--fp64-tile-y", type=int, default=80)
     ap.add_argument("--fp64-sweeps-per-launch", type=int, default=4)
     ap.add_argument("--fp64-check-every", type=int, default=160)
     ap.add_argument("--omega", type=float, default=1.998)

@@ -210,13 +210,14 @@ static if_status tune(Engine &e, int kind, int ty, int m, int k)
         return fail(IF_ERR_ARG, "tune: 8-wave register-tile sweep needs 2 sweeps with tile_y 24/40/56, "
                                 "4 sweeps with tile_y 32/48/64 or 8 sweeps with tile_y 32/48/64");
     if (kind == 4 && !reg_variant_ok(pr, 16, ty, m))
-        return fail(IF_ERR_ARG, "tune: 16-wave register-tile sweep needs 4 sweeps with tile_y 48 or 8 sweeps "
+        return fail(IF_ERR_ARG, "tune: 16-wave register-tile sweep needs 4 sweeps with tile_y 48/80 or 8 sweeps "
                                 "with tile_y 32/64 (96: mixed precision only)");
     if (kind < 0 || kind > 4) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
     if (k < 1) return fail(IF_ERR_ARG, "tune: check_every must be >= 1");
     e.sweep_kind = kind;
     e.sweep_ty = ty;
     e.sweep_m = m;
+    e.shape_auto = false;
     e.check_k = k;
     e.mark_dirty();
     return IF_OK;

@@ -68,6 +68,7 @@ public:
     // (FP64 defaults; the constructor sets 4, 64, 8 for mixed precision)
     int sweep_kind = 3;  // 0: LDS-tiled, 1: register-streaming, 2/3/4: register-tile (4/8/16 waves)
     int sweep_ty = 32, sweep_m = 4, check_k = 10;
+    bool shape_auto = true;  // FP64: the shape follows the resident set (refresh) until tuned
     bool timing = false;
     // replay each chunk of Poisson sweep launches from a captured hipGraph
     bool use_graphs = true;

@@ -40,7 +40,8 @@ Engine::Engine(int prec, size_t budget)
 {
     // default Poisson launch shape, by measurement on the 2048 x 1024
     // ellipse: FP32 sweeps are fastest on 16-wave tiles at 8 sweeps per
-    // launch; FP64 sweeps (IEEE division per cell) on 8-wave tiles at 4
+    // launch; FP64 sweeps (IEEE division per cell) on 8-wave tiles at 4,
+    // or as persistent 16-wave chunks when those fill the device (refresh)
     if (prec != IF_FP64) {
         sweep_kind = 4;
         sweep_ty = 64;
@@ -596,7 +597,7 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
 {
     const int n = (int)order_.size();
     if (any_rb && pers)  // the whole chunk in one launch
-        launch_ppe_sweep_pers(st_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
+        launch_ppe_sweep_pers(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
                               K, d_bar_, d_flags_);
     for (int s = 0; s < K; s++) {
         if (any_rb && !pers && s % M == 0) {
@@ -663,6 +664,21 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
+    // FP64 default shape: 16-wave tiles of 80 rows as persistent chunks
+    // when the running cases' tiles at that shape fill at least 3/4 of the
+    // device in one launch (the 2048 x 1024 case: 247 tiles, 1.42x faster
+    // per sweep); otherwise 8-wave tiles of 32 rows, which keep more
+    // blocks busy on smaller grids
+    if (shape_auto && pprec_ == IF_FP64) {
+        int nt80 = 0;
+        for (int id : order_)
+            nt80 += ((slots_[id]->NX + reg_out_cols(4) - 1) / reg_out_cols(4)) * ((slots_[id]->NY + 79) / 80);
+        const int cap = pers_blocks(IF_FP64, 16, 80, 4);
+        const bool big = use_persistent && cap > 0 && pers_fits(IF_FP64, 16, 80, 4, nt80) && 4 * nt80 >= 3 * cap;
+        sweep_kind = big ? 4 : 3;
+        sweep_ty = big ? 80 : 32;
+        sweep_m = 4;
+    }
     if (n > cap_cases_) {
         if (d_cases_) hipFree(d_cases_);
         cap_cases_ = std::max(n, 2 * cap_cases_);
@@ -835,9 +851,10 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;
         int timed = 0;
-        // persistent chunk: mixed precision, register tiles, grid fits
-        const bool pers = any_rb && use_persistent && pprec_ != IF_FP64 && sweep_kind >= 2 &&
-                          pers_fits(reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
+        // persistent chunk: a register-tile shape with a persistent form in
+        // this precision, and the grid fits on the device
+        const bool pers = any_rb && use_persistent && sweep_kind >= 2 &&
+                          pers_fits(pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
         ran_pers = ran_pers || pers;
         if (any_rb) launch_sweeps = pers ? K : M;
         if (use_graphs) {
@@ -990,7 +1007,7 @@ if_status Engine::autotune(if_tune_result *out)
     };
     // register tiles of 16, 8 and 4 waves at 8, 4 and 2 sweeps per launch,
     // and the LDS-tiled kernel; shapes a precision lacks are skipped
-    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {3, 32, 8}, {3, 48, 8},
+    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {4, 80, 4}, {3, 32, 8}, {3, 48, 8},
                                   {3, 64, 8}, {3, 32, 4}, {3, 48, 4}, {3, 24, 2}, {2, 16, 4}, {0, 16, 2}};
     constexpr int kReps = 6;
     auto sweep_launch = [&](const Shape &c, int s, int K) {
@@ -1018,14 +1035,16 @@ if_status Engine::autotune(if_tune_result *out)
         if (rc != IF_OK) break;
         // a shape that will run as persistent chunks is timed that way, over
         // 32 stages (a typical check interval); it writes only the scratch
-        // error buffers and advances the stage counters of every tile alike
+        // error buffers and advances the stage counters of every tile alike.
+        // FP64 shapes are timed per stage: an FP64 persistent launch's
+        // second stage would overwrite the solve's current pressure.
         const bool pc = pprec_ != IF_FP64 && c.kind >= 2 && use_persistent &&
-                        pers_fits(reg_waves(c.kind), c.ty, c.m, nt_sweep_);
+                        pers_fits(pprec_, reg_waves(c.kind), c.ty, c.m, nt_sweep_);
         const int kp = 32 * c.m;
         const int s0 = pprec_ == IF_FP64 ? 0 : c.m;
         auto timed = [&]() {
             if (pc)
-                launch_ppe_sweep_pers(st_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
+                launch_ppe_sweep_pers(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
                                       d_bar_, d_flags_);
             else
                 sweep_launch(c, s0, 2 * c.m);
@@ -1053,6 +1072,7 @@ if_status Engine::autotune(if_tune_result *out)
         sweep_kind = best.kind;
         sweep_ty = best.ty;
         sweep_m = best.m;
+        shape_auto = false;
         rc = refresh();
     }
     if (rc == IF_OK) {

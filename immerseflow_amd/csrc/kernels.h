@@ -23,15 +23,17 @@ bool reg_variant_ok(int prec, int waves, int ty, int m);
 int reg_out_cols(int m);
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                           const int2 *tiles, int nt, int s, int K);
-// The K sweeps of a chunk in one launch of the register-tile kernel (FP32),
+// The K sweeps of a chunk in one launch of the register-tile kernel, in
 // stages of m sweeps with a halo exchange between neighbouring tiles after
-// each.  pers_fits: the shape has a persistent form and all nt tiles'
-// blocks can be resident on the device at once (a host query; call it
-// outside stream capture).  flags: one stage counter per tile, zeroed
-// whenever the tile table changes.
-bool pers_fits(int waves, int ty, int m, int nt);
-void launch_ppe_sweep_pers(hipStream_t st, int waves, int ty, int m, const CaseDev *cs, const int2 *tiles,
-                           int nt, int K, GridBar *bar, unsigned *flags);
+// each (prec: the Poisson sweep precision).  pers_fits: the shape has a
+// persistent form in that precision and all nt tiles' blocks can be
+// resident on the device at once (a host query; call it outside stream
+// capture).  flags: one stage counter per tile, zeroed whenever the tile
+// table changes.
+bool pers_fits(int prec, int waves, int ty, int m, int nt);
+int pers_blocks(int prec, int waves, int ty, int m);  // 0: no persistent form
+void launch_ppe_sweep_pers(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
+                           const int2 *tiles, int nt, int K, GridBar *bar, unsigned *flags);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
                       int nt, int K);
 // zebra line SOR, one sweep (both colours) at chunk position s

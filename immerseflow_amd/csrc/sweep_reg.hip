@@ -190,12 +190,13 @@ __device__ __forceinline__ void wait_neighbours(const unsigned *flags, int nb, u
 
 }  // namespace
 
-// PERS (FP32 only): one launch runs the chunk's K sweeps as stages of M
+// PERS: one launch runs the chunk's K sweeps as stages of M
 // sweeps.  The tile stays in registers; after each stage a block writes the
 // border strips (depth 2M) of its stored region, waits for its neighbouring
 // tiles to have written theirs, and reloads its halo ring from them.  Stage
 // L leaves its output where launch L of the chunked form does
-// (e[(L + 1) & 1]), the last stage of a case the whole stored region, so
+// (FP32 e[(L + 1) & 1], FP64 p[(ppe_base + L + 1) & 1]), the last stage of
+// a case the whole stored region, so
 // the checks read the same buffer either way.  A block is at most one
 // stage ahead of its neighbours, so the two buffers never hold data a
 // neighbour still needs when they are overwritten.
@@ -204,7 +205,6 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                                                        int ntiles_pad, int s, int K, GridBar *bar,
                                                        unsigned *flags)
 {
-    static_assert(!PERS || sizeof(T) == 4, "persistent sweeps are FP32");
     constexpr int H = 2 * M;
     constexpr int TW = 128 - 2 * H;   // stored columns per tile
     constexpr int TY = NW * NRW - 2 * H;  // stored rows per tile
@@ -449,6 +449,33 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             }
         }
     };
+    // n half-sweeps (n even).  FP64: one loop body picks the colour order
+    // per half-sweep (two copies of the unrolled pair would exceed the
+    // register budget)
+    auto sweeps = [&](int n) {
+        if constexpr (sizeof(T) == 4) {
+            pairs(n);
+        } else {
+            for (int h = 0; h < n; h++) {
+                const int b = h & 1;
+                xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
+                xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+                __syncthreads();
+                V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
+                if (w > 0) sv = xch[b][w - 1][1][lane];
+                if (w < NW - 1) nv = xch[b][w + 1][0][lane];
+#define IFF_HALF(Q_, F_)                                                                              \
+    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
+                             cxa, cxpb, cxmb, cxb, om)
+                if (smask == 0) {
+                    if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+                } else {
+                    if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+                }
+#undef IFF_HALF
+            }
+        }
+    };
     // store the stored region of the tile (rows H .. H+TY-1, columns H ..
     // H+TW-1), or only its border strips of depth H (the halo of the
     // neighbouring tiles)
@@ -471,6 +498,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     };
     if constexpr (PERS) {
         using U2 = unsigned __attribute__((ext_vector_type(2)));
+        using U4 = unsigned __attribute__((ext_vector_type(4)));
         const int nexec = (kexec + M - 1) / M;  // stages with sweeps for this case
         const unsigned f0 = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // the neighbouring tiles of the case (lanes 0-7 of wave 0)
@@ -481,13 +509,22 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             const int qx = tx + d % 3 - 1, qy = ty + d / 3 - 1;
             if (qx >= 0 && qx < ntxs && qy >= 0 && qy < ntys) nb = t - tl.y + qy * ntxs + qx;
         }
-        const int nrec = P * ny * (int)sizeof(float);  // bytes of a field
-        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(c.e[0], 0, nrec, kRsrc3);
-        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(c.e[1], 0, nrec, kRsrc3);
+        const int nrec = P * ny * (int)sizeof(T);  // bytes of a field
+        // the chunk's two buffers: stage l writes buffer (l + 1) & 1
+        void *buf0, *buf1;
+        if constexpr (sizeof(T) == 8) {
+            buf0 = c.p[ct->ppe_base & 1];
+            buf1 = c.p[(ct->ppe_base + 1) & 1];
+        } else {
+            buf0 = c.e[0];
+            buf1 = c.e[1];
+        }
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(buf0, 0, nrec, kRsrc3);
+        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(buf1, 0, nrec, kRsrc3);
         for (int l = 0; l < nexec; l++) {
             int me = kexec - l * M;
             if (me > M) me = M;
-            pairs(2 * me);
+            sweeps(2 * me);
             const bool last = l + 1 == nexec;
             const __amdgpu_buffer_rsrc_t ro = ((l + 1) & 1) ? rs1 : rs0;
             // stage output at device scope: the border strips, or (last
@@ -500,12 +537,20 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                     const int gj = jt0 + k;
                     if (rr < H || rr >= H + TY || gj > ny - 2) continue;
                     if (!last && !lb && rr >= 2 * H && rr < TY) continue;
-                    const int off = (gj * P + ga) * (int)sizeof(float);
-                    if (ga + 1 <= nx - 2)
-                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro, off,
-                                                              0, kDev);
-                    else if (ga <= nx - 2)
-                        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, off, 0, kDev);
+                    const int off = (gj * P + ga) * (int)sizeof(T);
+                    if constexpr (sizeof(T) == 4) {
+                        if (ga + 1 <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro,
+                                                                  off, 0, kDev);
+                        else if (ga <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, off, 0, kDev);
+                    } else {
+                        if (ga + 1 <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, V2{R.pa[k], R.pb[k]}), ro,
+                                                                   off, 0, kDev);
+                        else if (ga <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, R.pa[k]), ro, off, 0, kDev);
+                    }
                 }
             }
             if (last) break;
@@ -531,14 +576,27 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 const int gj = jt0 + k;
                 if (gj < 1 || gj > ny - 2) continue;
                 if (!lh && rr >= H && rr < H + TY) continue;
-                const int off = (gj * P + ga) * (int)sizeof(float);
-                if (ina && inb) {
-                    const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
-                    R.pa[k] = v[0];
-                    R.pb[k] = v[1];
+                const int off = (gj * P + ga) * (int)sizeof(T);
+                if constexpr (sizeof(T) == 4) {
+                    if (ina && inb) {
+                        const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
+                        R.pa[k] = v[0];
+                        R.pb[k] = v[1];
+                    } else {
+                        if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off, 0, kDev));
+                        if (inb)
+                            R.pb[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off + 4, 0, kDev));
+                    }
                 } else {
-                    if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off, 0, kDev));
-                    if (inb) R.pb[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off + 4, 0, kDev));
+                    if (ina && inb) {
+                        const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(ro, off, 0, kDev));
+                        R.pa[k] = v[0];
+                        R.pb[k] = v[1];
+                    } else {
+                        if (ina) R.pa[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
+                        if (inb)
+                            R.pb[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, off + 8, 0, kDev));
+                    }
                 }
             }
         }
@@ -547,31 +605,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             __hip_atomic_store(&flags[t], f0 + (unsigned)nstage, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    if constexpr (sizeof(T) == 4) {
-        pairs(nst);
-    } else {
-        // FP64: one loop body picks the colour order per half-sweep (two
-        // copies of the unrolled pair would exceed the register budget)
-        for (int h = 0; h < nst; h++) {
-            const int b = h & 1;
-            xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
-            xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
-            __syncthreads();
-            V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
-            if (w > 0) sv = xch[b][w - 1][1][lane];
-            if (w < NW - 1) nv = xch[b][w + 1][0][lane];
-#define IFF_HALF(Q_, F_)                                                                              \
-    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
-                             cxa, cxpb, cxmb, cxb, om)
-            if (smask == 0) {
-                if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
-            } else {
-                if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
-            }
-#undef IFF_HALF
-        }
-    }
-
+    sweeps(nst);
     store(dst, false);
 }
 
@@ -585,13 +619,13 @@ static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt
 
 // Blocks of the persistent form that can be resident at once: resident
 // blocks per CU (occupancy calculator) times the CUs, per shape.
-template <int M, int NW, int NRW>
+template <typename T, int M, int NW, int NRW>
 static int pers_capacity()
 {
     static int capacity = -1;  // one device type per process
     if (capacity < 0) {
         int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbsor_reg<float, M, NW, NRW, true>, 64 * NW,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbsor_reg<T, M, NW, NRW, true>, 64 * NW,
                                                          0) != hipSuccess ||
             hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -601,37 +635,61 @@ static int pers_capacity()
     return capacity;
 }
 
-template <int M, int NW, int NRW>
+template <typename T, int M, int NW, int NRW>
 static void pers_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int K, GridBar *bar,
                      unsigned *flags)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_rbsor_reg<float, M, NW, NRW, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
                        pad, 0, K, bar, flags);
 }
 
-// the shapes with a persistent form
-#define IFF_PERS_SHAPES(X) X(16, 8, 96, 8) X(16, 8, 64, 6) X(16, 4, 48, 4) X(8, 8, 32, 8) X(8, 4, 32, 6)
+// the shapes with a persistent form: (waves, sweeps per stage, tile_y,
+// rows per wave), FP32 and FP64
+#define IFF_PERS_SHAPES32(X) \
+    X(float, 16, 8, 96, 8) X(float, 16, 8, 64, 6) X(float, 16, 4, 48, 4) X(float, 8, 8, 32, 8) X(float, 8, 4, 32, 6)
+#define IFF_PERS_SHAPES64(X) X(double, 16, 4, 80, 6) X(double, 16, 4, 48, 4)
 
-bool pers_fits(int waves, int ty, int m, int nt)
+int pers_blocks(int prec, int waves, int ty, int m)
+{
+#define IFF_PB(T_, W_, M_, TY_, NRW_) \
+    if (waves == W_ && m == M_ && ty == TY_) return pers_capacity<T_, M_, W_, NRW_>();
+    if (prec == IF_FP64) {
+        IFF_PERS_SHAPES64(IFF_PB)
+    } else {
+        IFF_PERS_SHAPES32(IFF_PB)
+    }
+#undef IFF_PB
+    return 0;
+}
+
+bool pers_fits(int prec, int waves, int ty, int m, int nt)
 {
     const int pad = ((nt + 7) / 8) * 8;
-#define IFF_PF(W_, M_, TY_, NRW_) \
-    if (waves == W_ && m == M_ && ty == TY_) return pad <= pers_capacity<M_, W_, NRW_>();
-    IFF_PERS_SHAPES(IFF_PF)
+#define IFF_PF(T_, W_, M_, TY_, NRW_) \
+    if (waves == W_ && m == M_ && ty == TY_) return pad <= pers_capacity<T_, M_, W_, NRW_>();
+    if (prec == IF_FP64) {
+        IFF_PERS_SHAPES64(IFF_PF)
+    } else {
+        IFF_PERS_SHAPES32(IFF_PF)
+    }
 #undef IFF_PF
     return false;
 }
 
-void launch_ppe_sweep_pers(hipStream_t st, int waves, int ty, int m, const CaseDev *cs, const int2 *tiles,
-                           int nt, int K, GridBar *bar, unsigned *flags)
+void launch_ppe_sweep_pers(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
+                           const int2 *tiles, int nt, int K, GridBar *bar, unsigned *flags)
 {
-#define IFF_PL(W_, M_, TY_, NRW_)                                  \
-    if (waves == W_ && m == M_ && ty == TY_) {                    \
-        pers_one<M_, W_, NRW_>(st, cs, tiles, nt, K, bar, flags); \
-        return;                                                   \
+#define IFF_PL(T_, W_, M_, TY_, NRW_)                                  \
+    if (waves == W_ && m == M_ && ty == TY_) {                        \
+        pers_one<T_, M_, W_, NRW_>(st, cs, tiles, nt, K, bar, flags); \
+        return;                                                       \
     }
-    IFF_PERS_SHAPES(IFF_PL)
+    if (prec == IF_FP64) {
+        IFF_PERS_SHAPES64(IFF_PL)
+    } else {
+        IFF_PERS_SHAPES32(IFF_PL)
+    }
 #undef IFF_PL
 }
 
@@ -646,7 +704,8 @@ bool reg_variant_ok(int prec, int waves, int ty, int m)
     if (waves == 8)
         return (m == 2 && (ty == 24 || ty == 40 || ty == 56)) || (m == 4 && (ty == 32 || ty == 48 || ty == 64)) ||
                (m == 8 && (ty == 32 || ty == 48 || ty == 64));
-    if (waves == 16) return (m == 4 && ty == 48) || (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64)));
+    if (waves == 16)
+        return (m == 4 && (ty == 48 || ty == 80)) || (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64)));
     return false;
 }
 
@@ -667,7 +726,7 @@ void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, co
     IFF_RG(T, 8, 2, 24) IFF_RG(T, 8, 2, 40) IFF_RG(T, 8, 2, 56)                           \
     IFF_RG(T, 8, 4, 32) IFF_RG(T, 8, 4, 48) IFF_RG(T, 8, 4, 64)                           \
     IFF_RG(T, 8, 8, 32) IFF_RG(T, 8, 8, 48) IFF_RG(T, 8, 8, 64)                           \
-    IFF_RG(T, 16, 4, 48) IFF_RG(T, 16, 8, 32) IFF_RG(T, 16, 8, 64)
+    IFF_RG(T, 16, 4, 48) IFF_RG(T, 16, 4, 80) IFF_RG(T, 16, 8, 32) IFF_RG(T, 16, 8, 64)
     if (prec == IF_FP64) {
         IFF_RG_ALL(double)
     } else {

@@ -227,12 +227,14 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * registers), tile height 16/32/48 rows, 1, 2 or 4 sweeps per launch;
  * kernel 3: the same with eight wave64s per tile, tile height 24/40/56
  * (2 sweeps per launch), 32/48/64 (4) or 32/48/64 (8); kernel 4: sixteen
- * wave64s per tile, tile height 48 (4 sweeps per launch) or 32/64/96 (8;
- * 96 in mixed precision only).  Several red-black sweeps per launch is
+ * wave64s per tile, tile height 48/80 (4 sweeps per launch) or 32/64/96
+ * (8; 96 in mixed precision only).  Several red-black sweeps per launch is
  * temporal blocking; check_every is the number of sweeps between
  * convergence checks.  Results do not depend on the kernel, the tile
  * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps
- * (mixed); kernel 3, 32 rows, 4 sweeps (FP64). */
+ * (mixed); FP64: kernel 4, 80 rows, 4 sweeps as persistent chunks when the
+ * running cases' tiles at that shape fill at least 3/4 of the device in
+ * one launch, else kernel 3, 32 rows, 4 sweeps (until tuned). */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Launch mode of the Poisson sweeps (no counterpart in the reference,
@@ -243,14 +245,15 @@ if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch
  * 0: the same kernels are launched one by one.  Results are identical. */
 if_status if_case_graphs(if_case *c, int enable);
 /* Persistent Poisson chunks (no counterpart in the reference).  enable = 1
- * (the default): in mixed precision with a register-tile kernel (2, 3 or
- * 4; shapes 16 waves x 96/64 rows x 8 sweeps, 16 x 48 x 4, 8 x 32 x 8 and
- * 8 x 32 x 4), when every block of the launch can be resident on the
- * device at once, each chunk of check_every sweeps runs as one launch
- * whose tiles stay in registers, in stages of sweeps_per_launch sweeps
- * separated by a grid-wide barrier; otherwise one launch per stage.
- * Results are identical.  A barrier wait that times out makes the step
- * return IF_ERR_HIP and turns the form off. */
+ * (the default): with a register-tile shape that has a persistent form
+ * (mixed: 16 waves x 96/64 rows x 8 sweeps, 16 x 48 x 4, 8 x 32 x 8 and
+ * 8 x 32 x 4; FP64: 16 x 80 x 4 and 16 x 48 x 4), when every block of the
+ * launch can be resident on the device at once, each chunk of check_every
+ * sweeps runs as one launch whose tiles stay in registers, in stages of
+ * sweeps_per_launch sweeps separated by a halo exchange with the
+ * neighbouring tiles; otherwise one launch per stage.  Results are
+ * identical.  A neighbour wait that times out makes the step return
+ * IF_ERR_HIP and turns the form off. */
 if_status if_case_persistent(if_case *c, int enable);
 /* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
  * body-force diagnostic with no counterpart there): after every step the

@@ -94,17 +94,19 @@ def test_c2_mixed_vs_fp64(require_gpu):
     _mixed_vs_fp64(cfg, _prm(cfg, 1.998, cfg["ppe_tol"], 160), 2, {iff.IF_MIXED: 1e-5, iff.IF_MIXED_V32: 1e-4})
 
 
-def test_c2_persistent_chunks_equal_launch_per_stage(require_gpu):
-    """The 2048x1024 case at the default shape (242 tiles, one per CU, halo
-    strips crossing every XCD boundary): persistent chunks give the same
-    bits and sweep counts as one launch per stage."""
+@pytest.mark.parametrize("prec,shape", [(iff.IF_MIXED, (4, 96, 8, 240)), (iff.IF_FP64, (4, 80, 4, 160))])
+def test_c2_persistent_chunks_equal_launch_per_stage(require_gpu, prec, shape):
+    """The 2048x1024 case at the mixed default shape (242 tiles) and at the
+    FP64 persistent shape (247 tiles), one block per CU, halo strips
+    crossing every XCD boundary: persistent chunks give the same bits and
+    sweep counts as one launch per stage."""
     cfg = configs.c2()
-    prm = _prm(cfg, 1.998, cfg["ppe_tol"], 240)
-    _say("C2 persistent vs per-stage, 2 steps")
+    prm = _prm(cfg, 1.998, cfg["ppe_tol"], shape[3])
+    _say("C2 persistent vs per-stage, 2 steps", prec, shape)
     out = []
     for pers in (True, False):
-        c = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_MIXED, bodies=cfg["bodies"], **prm)
-        c.tune(4, 96, 8, 240)
+        c = iff.Case(cfg["xf"], cfg["yf"], precision=prec, bodies=cfg["bodies"], **prm)
+        c.tune(*shape)
         c.persistent(pers)
         st = c.step(2)
         out.append((st["total_ppe_sweeps"], {f: c.get(f) for f in ("u", "v", "p", "uf", "vf")}))

@@ -153,7 +153,7 @@ def test_physics_fp64_matches_oracle(require_gpu, name):
                                          (1, 16, 4), (1, 32, 1),
                                          (2, 16, 1), (2, 32, 2), (2, 16, 2), (2, 48, 2), (2, 32, 4),
                                          (2, 48, 4), (3, 24, 2), (3, 48, 4), (3, 32, 8), (3, 48, 8),
-                                         (4, 48, 4), (4, 32, 8), (4, 64, 8), (4, 96, 8), (3, 64, 8)])
+                                         (4, 48, 4), (4, 80, 4), (4, 32, 8), (4, 64, 8), (4, 96, 8), (3, 64, 8)])
 def test_sweep_variants_are_identical(require_gpu, kind, ty, m):
     """Kernel, tile height and temporal blocking change the launch shape,
     never the result: equal to the oracle's red-black SOR sweep for sweep
@@ -317,23 +317,25 @@ def test_graph_replay_equals_direct_launches(require_gpu, prec):
             assert np.array_equal(runs[0][k], runs[1][k]), k
 
 
-@pytest.mark.parametrize("kind,ty,m,K,graphs", [(4, 96, 8, 40, True), (4, 64, 8, 13, True), (3, 32, 8, 40, False),
-                                                 (4, 48, 4, 12, True), (3, 32, 4, 10, True)])
-def test_persistent_chunks_equal_launch_per_stage(require_gpu, kind, ty, m, K, graphs):
+@pytest.mark.parametrize("prec,kind,ty,m,K,graphs", [
+    (iff.IF_MIXED, 4, 96, 8, 40, True), (iff.IF_MIXED, 4, 64, 8, 13, True), (iff.IF_MIXED, 3, 32, 8, 40, False),
+    (iff.IF_MIXED, 4, 48, 4, 12, True), (iff.IF_MIXED, 3, 32, 4, 10, True),
+    (iff.IF_FP64, 4, 80, 4, 12, True), (iff.IF_FP64, 4, 48, 4, 13, False)])
+def test_persistent_chunks_equal_launch_per_stage(require_gpu, prec, kind, ty, m, K, graphs):
     """A chunk run as one persistent launch (tiles kept in registers, halo
     strips exchanged between neighbouring tiles between stages) gives the
-    same bits as one launch per stage: in a batch with a point-Jacobi case
-    (its blocks only advance their stage counters), a case whose sweep
-    budget ends inside a chunk, a heaving body (flags, ghosts and weights
-    rebuilt every step), and chunks that are not a whole number of
-    stages."""
+    same bits as one launch per stage, in FP32 error sweeps and in FP64
+    sweeps: in a batch with a point-Jacobi case (its blocks only advance
+    their stage counters), a case whose sweep budget ends inside a chunk, a
+    heaving body (flags, ghosts and weights rebuilt every step), and chunks
+    that are not a whole number of stages."""
     xf, yf, bodies = CASES["circle-stretched"]
     xf2, yf2, bodies2 = CASES["ellipse"]
-    _say("persistent vs per-stage launches", kind, ty, m, K, graphs)
+    _say("persistent vs per-stage launches", prec, kind, ty, m, K, graphs)
     kw = dict(dt=0.01, re=100.0, ppe_omega=1.8, ppe_tol=1e-8, check_every=K)
     runs = []
     for pers in (True, False):
-        b = iff.Batch(iff.IF_MIXED)
+        b = iff.Batch(prec)
         b.tune(kind, ty, m, K)
         b.graphs(graphs)
         b.persistent(pers)
@@ -419,7 +421,7 @@ def test_autotune_leaves_state_and_matches_manual_tune(require_gpu, prec):
     kw = dict(precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8, ppe_itermax=200000,
               ppe_tol=1e-9, check_every=10)
     a, b = iff.Case(xf, yf, **kw), iff.Case(xf, yf, **kw)
-    ncand = 11 if prec == iff.IF_FP64 else 12   # the 16-wave x 8-row tile is FP32 only
+    ncand = 12 if prec == iff.IF_FP64 else 13   # the 16-wave x 8-row tile is FP32 only
     r0 = a.autotune()                       # before any step: interval unchanged
     assert r0["typical_sweeps"] == 0 and r0["check_every"] == 10 and r0["candidates"] == ncand
     a.step(2)
@@ -453,7 +455,7 @@ def test_batch_autotune(require_gpu):
         bt.step(1)
         bs.append((bt, sl))
     r = bs[0][0].autotune()
-    assert r["candidates"] == 12 and r["typical_sweeps"] > 0
+    assert r["candidates"] == 13 and r["typical_sweeps"] > 0
     bs[1][0].tune(r["kernel"], r["tile_y"], r["sweeps_per_launch"], r["check_every"])
     for bt, _ in bs:
         bt.step(2)
