@@ -388,7 +388,10 @@ def main():
         err_tight = fields_err(ct[0].get, ct[1].get)
         for x in ct:
             x.close()
+        pers64 = int(round(b64 / 25.0)) > a.fp64_sweeps_per_launch  # 25 B per cell-sweep (FP64)
         fp64 = dict(value=round(r64["mcs"], 3), ms_per_step=round(1e3 * r64["seconds"] / a.fp64_steps, 3),
+                    kernel=kernel_name(a.fp64_kernel, a.fp64_tile_y, a.fp64_sweeps_per_launch, "double", pers64),
+                    us_per_sweep=round(1e3 * avg64 / max(1, int(round(b64 / 25.0))), 3),
                     sweeps_last_step=r64["stats"]["ppe_sweeps"],
                     max_div=r64["stats"]["max_div"],
                     sweep_gbs=round(b64 * cells / (avg64 * 1e-3) / 1e9, 1) if avg64 > 0 else None,
