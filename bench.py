@@ -40,7 +40,7 @@ def parse():
                     help="0 LDS-tiled, 1 register-streaming, 2/3/4 register-tile (4/8/16 waves)")
     ap.add_argument("--tile-y", type=int, default=96)
     ap.add_argument("--sweeps-per-launch", type=int, default=8)
-    ap.add_argument("--check-every", type=int, default=240)
+    ap.add_argument("--check-every", type=int, default=480)
     ap.add_argument("--fp64-kernel", type=int, default=4)
     ap.add_argument("--fp64-tile-y", type=int, default=80)
     ap.add_argument("--fp64-sweeps-per-launch", type=int, default=4)
@@ -359,7 +359,8 @@ def main():
     if os.path.exists(tp):
         with open(tp) as f:
             tj = json.load(f)
-        if tj.get("kernel") == kname:  # measured for this launch shape only
+        # measured for this launch shape and sweeps per launch only
+        if tj.get("kernel") == kname and tj.get("sweeps_per_launch") == sweeps_per_launch:
             traffic = tj.get("mixed_bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
