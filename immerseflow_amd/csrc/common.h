@@ -151,6 +151,7 @@ struct CaseDev {
     double *uf[2], *vf[2];         // face velocities: [0] = U^n, [1] = U*
     double *p[2], *rhs;
     float *e[2], *r32;             // mixed Poisson error / residual
+    void *lm, *ly;                 // line solve: pivots m (Poisson precision) and, FP32, 1/m; line cases only
     uint8_t *flags;
     int8_t *bodyid;
     GhostDev *gh;

@@ -111,6 +111,7 @@ struct ArenaPlan {
     size_t off_u[2], off_v[2], off_sx, off_sy;
     size_t off_uf[2], off_vf[2], off_p[2], off_rhs;
     size_t off_e[2], off_r32;
+    size_t off_lm, off_ly;
     size_t off_flags, off_bodyid;
     size_t off_metd, off_xc, off_yc, off_metf;
     size_t off_bodies, off_poly, off_part, off_gcount;
@@ -118,7 +119,7 @@ struct ArenaPlan {
 };
 
 static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbody,
-                            size_t npoly, int gblocks)
+                            size_t npoly, int gblocks, bool line)
 {
     ArenaPlan a{};
     size_t o = 0;
@@ -141,6 +142,10 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
     if (prec != IF_FP64) {
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
+    }
+    if (line) {  // the line solve's pivots (and, for FP32 sweeps, their reciprocals)
+        a.off_lm = fieldT(prec == IF_FP64 ? 8 : 4);
+        if (prec != IF_FP64) a.off_ly = fieldT(4);
     }
     a.off_flags = fieldT(1);
     a.off_bodyid = fieldT(1);
@@ -181,7 +186,7 @@ size_t Engine::footprint(const if_case_desc &d, int prec)
     for (int b = 0; b < d.nbody; b++)
         if (d.bodies[b].kind == IF_BODY_POLYGON) npoly += 2 * (size_t)d.bodies[b].nvert;
     const ArenaPlan a = plan_arena(prec, nx, ny, P, ntx * nty, d.nbody, npoly,
-                                   (nx * ny + ghost_span() - 1) / ghost_span());
+                                   (nx * ny + ghost_span() - 1) / ghost_span(), d.ppe_solver == IF_PPE_LINE);
     // ghost records: a perimeter-sized estimate for the budget
     const size_t gest = (size_t)(8 * (nx + ny));
     return a.total + gest * sizeof(GhostDev);
@@ -310,7 +315,7 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     s.ghost_blocks = (s.nx * s.ny + ghost_span() - 1) / ghost_span();
 
     const ArenaPlan a = plan_arena(prec_, s.nx, s.ny, s.P, s.ntx * s.nty, d.nbody, npoly,
-                                   s.ghost_blocks);
+                                   s.ghost_blocks, d.ppe_solver == IF_PPE_LINE);
     if (budget_ && used_ + a.total > budget_)
         return fail(IF_ERR_BUDGET, "case footprint exceeds the remaining HBM budget");
     if (hipMalloc(&s.arena, a.total) != hipSuccess) {
@@ -362,6 +367,10 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.sy = A + a.off_sy;
     c.rhs = (double *)(A + a.off_rhs);
     if (pprec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
+    if (d.ppe_solver == IF_PPE_LINE) {
+        c.lm = A + a.off_lm;
+        if (pprec_ != IF_FP64) c.ly = A + a.off_ly;
+    }
     c.flags = (uint8_t *)(A + a.off_flags);
     c.bodyid = (int8_t *)(A + a.off_bodyid);
     c.part = (double *)(A + a.off_part);
@@ -847,6 +856,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
 
         // ---- P3: faces + RHS
         launch_faces(st_, vprec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
+        if (any_line) launch_line_factor(st_, pprec_, d_cases_, n, max_ring_);
 
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;

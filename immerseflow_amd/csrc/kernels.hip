@@ -627,36 +627,89 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 // Each row's tridiagonal system
 //     -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs
 // (fluid cells with sa != 0; identity rows elsewhere) is solved by the
-// Thomas algorithm and relaxed, p + om (x - p).  Reads the unknown from
-// buffer src (own row, and for colour 1 the rows above and below), writes
-// dst; colour 0 reads its neighbour rows from dst (just updated).  FP64 is
-// the oracle's expression and order (line_sor_sweep, oracle.c).
+// Thomas algorithm and relaxed, p + om (x - p).  FP64 is the oracle's
+// arithmetic (line_sor_sweep, oracle.c), to the bit.
 //
-// The elimination is a serial recurrence along a row, so a row costs the
-// latency of its dependence chain.  A block owns 64 rows of one colour and
-// walks them in chunks of kLineW columns staged in LDS, one lane per row.
-// Forward elimination is a pipeline over three LDS buffers with one
-// barrier per stage: the "mover" waves stage chunk t with coalesced row
-// loads (and write chunk t-3's results back), wave 0 runs the c' chain
-// (m = b - l c'[i-1], c' = u / m) on chunk t-1 and wave 1 the d' chain
-// (d' = (d - l d'[i-1]) / m) on chunk t-2; c' depends on the coefficients
-// only, so the two divisions per cell run on different SIMDs (FP64 divides
-// as the oracle does; the FP32 error sweeps form 1/m once with the hardware
-// reciprocal and both chains multiply by it).  c' and d'
-// go to the case's predictor source arrays (free during the Poisson
-// phase).  The back substitution stages c', d' right to left (two
-// buffers), wave 0 turns d' into x in place, and the movers relax and
-// store.
+// The elimination's pivots m[i] = b - l c'[i-1] and c'[i] = u / m[i] depend
+// on the cell flags and metric coefficients only: k_line_factor forms them
+// once per step (FP32 error sweeps: with y = 1/m from the hardware
+// reciprocal, c' = u y).  A sweep then runs only the d' recurrence,
+// d'[i] = (d - l d'[i-1]) / m[i] (FP32: times y), and the back
+// substitution.
+//
+// The recurrence is serial along a row, so a row costs the latency of its
+// dependence chain.  A block owns 64 rows of one colour and walks them in
+// chunks of kLineW columns staged in LDS, one lane per row: wave 0 runs the
+// recurrence on chunk t-1 while the "mover" waves stage chunk t with
+// coalesced row loads (issued a stage ahead, so they are in flight across
+// the barrier) and write chunk t-2's d' back; two LDS buffers, one barrier
+// per stage.  c' and d' live in the case's predictor source arrays (free
+// during the Poisson phase), m (and y) in the case's line-solve arrays.  The
+// back substitution stages c', d' right to left, wave 0 turns d' into x in
+// place, and the movers relax and store.
 constexpr int kLineW = 32;                          // columns per chunk (a multiple of 8)
-constexpr int kLineMovers = 4;                      // mover waves per block
-constexpr int kLineThreads = 64 * (2 + kLineMovers);
+constexpr int kLineMovers = 8;                      // mover waves per block
+constexpr int kLineThreads = 64 * (1 + kLineMovers);
 constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 fluid cell
 
+// The per-step factors of every interior row of every line-solve case: m,
+// c' (into the predictor source array sx) and, for FP32, y = 1/m.  One
+// lane per row; it runs once per step.
+template <typename T>
+__global__ __launch_bounds__(64) void k_line_factor(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    if (c.ppe_solver != IF_PPE_LINE || !c.ctrl->active) return;
+    const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j > c.ny - 2) return;
+    const T *cxp, *cxm, *cyp, *cym;
+    if constexpr (sizeof(T) == 8) {
+        cxp = c.x.cp; cxm = c.x.cm; cyp = c.y.cp; cym = c.y.cm;
+    } else {
+        cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
+    }
+    const size_t row = (size_t)j * c.P;
+    const auto fl = G(c.flags) + row;
+    const auto gxp = G(cxp), gxm = G(cxm);
+    const auto cp = GW(fld<T>(c.sx) + row), pm = GW(fld<T>(c.lm) + row);
+    const T yp = G(cyp)[j], ym = G(cym)[j];
+    const int n = c.nx - 2;
+    T cprev = (T)0;
+    for (int i = 1; i <= n; i++) {
+        const unsigned f = fl[i];
+        T l = (T)0, b = (T)1, u = (T)0;
+        if (f & F_FLUID) {
+            const T aE = (f & F_OPEN_E) ? gxp[i] : (T)0;
+            const T aW = (f & F_OPEN_W) ? gxm[i] : (T)0;
+            const T aN = (f & F_OPEN_N) ? yp : (T)0;
+            const T aS = (f & F_OPEN_S) ? ym : (T)0;
+            const T sa = (aE + aW) + (aN + aS);
+            if (sa != (T)0) {
+                l = -aW;
+                b = sa;
+                u = -aE;
+            }
+        }
+        const T m = (i == 1) ? b : b - l * cprev;
+        T cpi;
+        if constexpr (sizeof(T) == 8) {
+            cpi = u / m;
+        } else {
+            const T y = __builtin_amdgcn_rcpf(m);
+            cpi = u * y;
+            GW(fld<T>(c.ly) + row)[i] = y;
+        }
+        cp[i] = cpi;
+        pm[i] = m;
+        cprev = cpi;
+    }
+}
+
 template <typename T> struct LineStage {
-    T a[3][64][kLineW + 1];           // forward: b, then c'; backward: c'
-    T d[3][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
-    T m[3][64][kLineW + 1];           // forward: the pivots m (FP32: 1/m)
-    uint8_t f[3][64][kLineW + 4];     // forward: cell flags | kLineSolved
+    T a[2][64][kLineW + 1];           // forward: y = 1/m (FP32); backward: c'
+    T d[2][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
+    T m[2][64][kLineW + 1];           // forward: the pivots m
+    uint8_t f[2][64][kLineW + 4];     // forward: cell flags | kLineSolved
     T yp[64], ym[64];                 // the rows' y coefficients
 };
 
@@ -699,6 +752,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     const auto gcxp = G(cxp), gcxm = G(cxm), gcyp = G(cyp), gcym = G(cym);
     const auto gflags = G(c.flags);
     const auto gdst = GW(dst), gcpv = GW(fld<T>(c.sx)), gdpv = GW(fld<T>(c.sy));
+    const auto gm = G((const T *)c.lm), gy = G((const T *)c.ly);  // gy: FP32 only
 
     __shared__ LineStage<T> st;
     const int n = nx - 2;  // a row's cells are i = 1..n
@@ -709,7 +763,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     static_assert(kMv % kLineW == 0 && (64 * kLineW) % kMv == 0 && kLineW % 8 == 0, "line chunk shape");
     // mover thread mt owns column mt % kLineW of every chunk and local rows
     // mt / kLineW + q kMv / kLineW (q < kPer)
-    const int mt = (int)threadIdx.x - 128;
+    const int mt = (int)threadIdx.x - 64;
     const int mcol = mt % kLineW, mrow = mt / kLineW;
     if (threadIdx.x < 64) {
         const int j = min(j0 + 2 * (int)threadIdx.x, jlast);
@@ -718,12 +772,12 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     }
     __syncthreads();
 
-    // ---- forward elimination ------------------------------------------
+    // ---- forward elimination (the d' recurrence) ------------------------
     // The movers load a chunk's inputs one stage before they stage it, so
     // the loads are in flight across a barrier (indices clamped into the
     // row).
     unsigned fv[kPer];
-    T po[kPer], pn[kPer], ps[kPer], rh[kPer], xp = (T)0, xm = (T)0;
+    T po[kPer], pn[kPer], ps[kPer], rh[kPer], mv[kPer], yv[kPer], xp = (T)0, xm = (T)0;
     auto fwd_load = [&](int ch) {
         const int i = min(1 + ch * kLineW + mcol, n);
         xp = gcxp[i];
@@ -737,14 +791,16 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
             pn[q] = gnbr[id + P];
             ps[q] = gnbr[id - P];
             rh[q] = grhs[id];
+            mv[q] = gm[id];
+            if constexpr (sizeof(T) == 4) yv[q] = gy[id];
         }
     };
-    if (wave >= 2) fwd_load(0);
-    T cprev = (T)0, dprev = (T)0;
-    for (int t = 0; t < nch + 3; t++) {
-        if (wave == 0) {  // c' chain on chunk t-1
+    if (wave >= 1) fwd_load(0);
+    T dprev = (T)0;
+    for (int t = 0; t < nch + 2; t++) {
+        if (wave == 0) {  // the d' recurrence on chunk t-1
             if (t >= 1 && t <= nch) {
-                const int b = (t - 1) % 3;
+                const int b = (t - 1) & 1;
                 const int i0 = 1 + (t - 1) * kLineW;
                 const int kn = min(kLineW, n - i0 + 1);
                 // groups of 8 cells: the group's LDS and coefficient loads
@@ -752,83 +808,41 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                 // end (last chunk only) run on clamped inputs and are never
                 // stored.
                 for (int k8 = 0; k8 < kn; k8 += 8) {
-                    T bb[8], xp[8], xm[8];
+                    T dd[8], mm[8], yy[8], xw[8];
                     unsigned ff[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
-                        const int i = min(i0 + k8 + q, n);
-                        xp[q] = gcxp[i];
-                        xm[q] = gcxm[i];
-                        ff[q] = st.f[b][lane][k8 + q];
-                        bb[q] = st.a[b][lane][k8 + q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const unsigned f = ff[q];
-                        const bool solved = f & kLineSolved;
-                        const T aE = (f & F_OPEN_E) ? xp[q] : (T)0;
-                        const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
-                        const T l = solved ? -aW : (T)0, u = solved ? -aE : (T)0;
-                        const T m = (i0 + k8 + q == 1) ? bb[q] : bb[q] - l * cprev;
-                        T cpi, mq;
-                        if constexpr (sizeof(T) == 8) {  // IEEE division, as the oracle
-                            cpi = u / m;
-                            mq = m;
-                        } else {  // FP32 error sweeps: the hardware reciprocal
-                            mq = __builtin_amdgcn_rcpf(m);
-                            cpi = u * mq;
-                        }
-                        cprev = cpi;
-                        st.a[b][lane][k8 + q] = cpi;
-                        st.m[b][lane][k8 + q] = mq;
-                    }
-                }
-            }
-        } else if (wave == 1) {  // d' chain on chunk t-2
-            if (t >= 2 && t <= nch + 1) {
-                const int b = (t - 2) % 3;
-                const int i0 = 1 + (t - 2) * kLineW;
-                const int kn = min(kLineW, n - i0 + 1);
-                for (int k8 = 0; k8 < kn; k8 += 8) {
-                    T dd[8], mm[8], xm[8];
-                    unsigned ff[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        xm[q] = gcxm[min(i0 + k8 + q, n)];
+                        xw[q] = gcxm[min(i0 + k8 + q, n)];
                         ff[q] = st.f[b][lane][k8 + q];
                         dd[q] = st.d[b][lane][k8 + q];
                         mm[q] = st.m[b][lane][k8 + q];
+                        if constexpr (sizeof(T) == 4) yy[q] = st.a[b][lane][k8 + q];
                     }
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         const unsigned f = ff[q];
-                        const T aW = (f & F_OPEN_W) ? xm[q] : (T)0;
+                        const T aW = (f & F_OPEN_W) ? xw[q] : (T)0;
                         const T l = (f & kLineSolved) ? -aW : (T)0;
-                        // the first cell: m is b, and d' = d / b.  FP32
-                        // holds 1/m (the hardware reciprocal) in place of m.
+                        // the first cell: m is b, and d' = d / b
                         const T num = (i0 + k8 + q == 1) ? dd[q] : dd[q] - l * dprev;
                         T dpi;
                         if constexpr (sizeof(T) == 8)
-                            dpi = num / mm[q];
+                            dpi = num / mm[q];  // IEEE division, as the oracle
                         else
-                            dpi = num * mm[q];
+                            dpi = num * yy[q];
                         dprev = dpi;
                         st.d[b][lane][k8 + q] = dpi;
                     }
                 }
             }
         } else {
-            const int b = t % 3;
-            if (t >= 3) {  // results of chunk t-3
-                const int i = 1 + (t - 3) * kLineW + mcol;
+            const int b = t & 1;
+            if (t >= 2) {  // d' of chunk t-2
+                const int i = 1 + (t - 2) * kLineW + mcol;
 #pragma unroll
                 for (int q = 0; q < kPer; q++) {
                     const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
-                    if (i <= n && j <= jlast) {
-                        const size_t id = (size_t)j * P + i;
-                        gcpv[id] = st.a[b][rl][mcol];
-                        gdpv[id] = st.d[b][rl][mcol];
-                    }
+                    if (i <= n && j <= jlast) gdpv[(size_t)j * P + i] = st.d[b][rl][mcol];
                 }
             }
             if (t < nch) {  // stage chunk t from the registers loaded a stage ago
@@ -836,7 +850,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                 for (int q = 0; q < kPer; q++) {
                     const int rl = mrow + q * (kMv / kLineW);
                     const unsigned f = fv[q];
-                    T bb = (T)1, d = zero_src ? (T)0 : po[q];
+                    T d = zero_src ? (T)0 : po[q];
                     unsigned fs = f;
                     if (f & F_FLUID) {
                         const T aE = (f & F_OPEN_E) ? xp : (T)0;
@@ -846,14 +860,14 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                         const T sa = (aE + aW) + (aN + aS);
                         if (sa != (T)0) {
                             const T vn = zero_nbr ? (T)0 : pn[q], vs = zero_nbr ? (T)0 : ps[q];
-                            bb = sa;
                             d = (aN * vn + aS * vs) - rh[q];
                             fs |= kLineSolved;
                         }
                     }
                     st.f[b][rl][mcol] = (uint8_t)fs;
-                    st.a[b][rl][mcol] = bb;
                     st.d[b][rl][mcol] = d;
+                    st.m[b][rl][mcol] = mv[q];
+                    if constexpr (sizeof(T) == 4) st.a[b][rl][mcol] = yv[q];
                 }
             }
             if (t + 1 < nch) fwd_load(t + 1);
@@ -887,7 +901,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
             }
         }
     };
-    if (wave >= 2) bwd_load(0);
+    if (wave >= 1) bwd_load(0);
     T x = (T)0;
     for (int t = 0; t < nch + 2; t++) {
         if (wave == 0) {
@@ -912,7 +926,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
                     }
                 }
             }
-        } else if (wave >= 2) {
+        } else {
             const int b = t & 1;
             const int is = 1 + (nch + 1 - t) * kLineW + mcol;  // chunk staged at step t-2
 #pragma unroll
@@ -941,6 +955,16 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
         }
         __syncthreads();
     }
+}
+
+
+void launch_line_factor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny)
+{
+    const dim3 grid((unsigned)((max_ny - 2 + 63) / 64), (unsigned)ncase);
+    if (prec == IF_FP64)
+        hipLaunchKernelGGL(k_line_factor<double>, grid, dim3(64), 0, st, cs);
+    else
+        hipLaunchKernelGGL(k_line_factor<float>, grid, dim3(64), 0, st, cs);
 }
 
 void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny, int s, int K)
