@@ -647,7 +647,9 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 // during the Poisson phase), m (and y) in the case's line-solve arrays.  The
 // back substitution stages c', d' right to left, wave 0 turns d' into x in
 // place, and the movers relax and store.
-constexpr int kLineW = 32;                          // columns per chunk (a multiple of 8)
+// columns per chunk (a multiple of 8): 64 for FP32 cells, 32 for FP64
+// (three double-buffered chunk arrays must fit the LDS)
+template <typename T> constexpr int kLineW = sizeof(T) == 4 ? 64 : 32;
 constexpr int kLineMovers = 8;                      // mover waves per block
 constexpr int kLineThreads = 64 * (1 + kLineMovers);
 constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 fluid cell
@@ -706,10 +708,10 @@ __global__ __launch_bounds__(64) void k_line_factor(const CaseDev *cs)
 }
 
 template <typename T> struct LineStage {
-    T a[2][64][kLineW + 1];           // forward: y = 1/m (FP32); backward: c'
-    T d[2][64][kLineW + 1];           // forward: d, then d'; backward: d', then x
-    T m[2][64][kLineW + 1];           // forward: the pivots m
-    uint8_t f[2][64][kLineW + 4];     // forward: cell flags | kLineSolved
+    T a[2][64][kLineW<T> + 1];        // forward: y = 1/m (FP32); backward: c'
+    T d[2][64][kLineW<T> + 1];        // forward: d, then d'; backward: d', then x
+    T m[2][64][kLineW<T> + 1];        // forward: the pivots m
+    uint8_t f[2][64][kLineW<T> + 4];  // forward: cell flags | kLineSolved
     T yp[64], ym[64];                 // the rows' y coefficients
 };
 
@@ -755,16 +757,17 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     const auto gm = G((const T *)c.lm), gy = G((const T *)c.ly);  // gy: FP32 only
 
     __shared__ LineStage<T> st;
+    constexpr int kW = kLineW<T>;
     const int n = nx - 2;  // a row's cells are i = 1..n
-    const int nch = (n + kLineW - 1) / kLineW;
+    const int nch = (n + kW - 1) / kW;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int kMv = 64 * kLineMovers;
-    constexpr int kPer = 64 * kLineW / kMv;  // cells per mover thread and chunk
-    static_assert(kMv % kLineW == 0 && (64 * kLineW) % kMv == 0 && kLineW % 8 == 0, "line chunk shape");
-    // mover thread mt owns column mt % kLineW of every chunk and local rows
-    // mt / kLineW + q kMv / kLineW (q < kPer)
+    constexpr int kPer = 64 * kW / kMv;  // cells per mover thread and chunk
+    static_assert(kMv % kW == 0 && (64 * kW) % kMv == 0 && kW % 8 == 0, "line chunk shape");
+    // mover thread mt owns column mt % kW of every chunk and local rows
+    // mt / kW + q kMv / kW (q < kPer)
     const int mt = (int)threadIdx.x - 64;
-    const int mcol = mt % kLineW, mrow = mt / kLineW;
+    const int mcol = mt % kW, mrow = mt / kW;
     if (threadIdx.x < 64) {
         const int j = min(j0 + 2 * (int)threadIdx.x, jlast);
         st.yp[threadIdx.x] = gcyp[j];
@@ -779,12 +782,12 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     unsigned fv[kPer];
     T po[kPer], pn[kPer], ps[kPer], rh[kPer], mv[kPer], yv[kPer], xp = (T)0, xm = (T)0;
     auto fwd_load = [&](int ch) {
-        const int i = min(1 + ch * kLineW + mcol, n);
+        const int i = min(1 + ch * kW + mcol, n);
         xp = gcxp[i];
         xm = gcxm[i];
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
-            const int j = min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast);
+            const int j = min(j0 + 2 * (mrow + q * (kMv / kW)), jlast);
             const size_t id = (size_t)j * P + i;
             fv[q] = gflags[id];
             po[q] = gsrc[id];
@@ -801,8 +804,8 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
         if (wave == 0) {  // the d' recurrence on chunk t-1
             if (t >= 1 && t <= nch) {
                 const int b = (t - 1) & 1;
-                const int i0 = 1 + (t - 1) * kLineW;
-                const int kn = min(kLineW, n - i0 + 1);
+                const int i0 = 1 + (t - 1) * kW;
+                const int kn = min(kW, n - i0 + 1);
                 // groups of 8 cells: the group's LDS and coefficient loads
                 // are issued ahead of its recurrence.  Cells past the row's
                 // end (last chunk only) run on clamped inputs and are never
@@ -838,17 +841,17 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
         } else {
             const int b = t & 1;
             if (t >= 2) {  // d' of chunk t-2
-                const int i = 1 + (t - 2) * kLineW + mcol;
+                const int i = 1 + (t - 2) * kW + mcol;
 #pragma unroll
                 for (int q = 0; q < kPer; q++) {
-                    const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
+                    const int rl = mrow + q * (kMv / kW), j = j0 + 2 * rl;
                     if (i <= n && j <= jlast) gdpv[(size_t)j * P + i] = st.d[b][rl][mcol];
                 }
             }
             if (t < nch) {  // stage chunk t from the registers loaded a stage ago
 #pragma unroll
                 for (int q = 0; q < kPer; q++) {
-                    const int rl = mrow + q * (kMv / kLineW);
+                    const int rl = mrow + q * (kMv / kW);
                     const unsigned f = fv[q];
                     T d = zero_src ? (T)0 : po[q];
                     unsigned fs = f;
@@ -881,21 +884,21 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     // fv, po, xp, xm) and c', d' of the chunk staged at step t (pn, ps).
     auto bwd_load = [&](int t) {
         if (t >= 2 && t <= nch + 1) {
-            const int i = min(1 + (nch + 1 - t) * kLineW + mcol, n);
+            const int i = min(1 + (nch + 1 - t) * kW + mcol, n);
             xp = gcxp[i];
             xm = gcxm[i];
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
-                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + i;
+                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kW)), jlast) * P + i;
                 fv[q] = gflags[id];
                 po[q] = gsrc[id];
             }
         }
         if (t < nch) {
-            const int i = max(1, min(1 + (nch - 1 - t) * kLineW + mcol, n));
+            const int i = max(1, min(1 + (nch - 1 - t) * kW + mcol, n));
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
-                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kLineW)), jlast) * P + i;
+                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kW)), jlast) * P + i;
                 pn[q] = gcpv[id];
                 ps[q] = gdpv[id];
             }
@@ -907,8 +910,8 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
         if (wave == 0) {
             if (t >= 1 && t <= nch) {
                 const int b = (t - 1) & 1;
-                const int i0 = 1 + (nch - t) * kLineW;
-                const int kn = min(kLineW, n - i0 + 1);
+                const int i0 = 1 + (nch - t) * kW;
+                const int kn = min(kW, n - i0 + 1);
                 // groups of 8 cells from the right; in the row's last
                 // chunk the cells past its end come first and x restarts
                 // at i = n
@@ -928,10 +931,10 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
             }
         } else {
             const int b = t & 1;
-            const int is = 1 + (nch + 1 - t) * kLineW + mcol;  // chunk staged at step t-2
+            const int is = 1 + (nch + 1 - t) * kW + mcol;  // chunk staged at step t-2
 #pragma unroll
             for (int q = 0; q < kPer; q++) {
-                const int rl = mrow + q * (kMv / kLineW), j = j0 + 2 * rl;
+                const int rl = mrow + q * (kMv / kW), j = j0 + 2 * rl;
                 if (t >= 2) {
                     const unsigned f = fv[q];
                     const T pold = zero_src ? (T)0 : po[q];
