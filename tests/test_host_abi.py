@@ -212,3 +212,15 @@ def test_footprint_by_precision():
     assert 10 * cells <= fp[iff.IF_MIXED] - fp[iff.IF_FP64] <= 16 * cells
     with pytest.raises(iff.ImmerseFlowError):
         iff.Batch.footprint(xf, xf, precision=7, dt=0.01, re=100.0)
+
+
+def test_footprint_of_a_line_solve_case():
+    """A zebra line-SOR case holds its row pivots (FP64: m; FP32 error
+    sweeps: m and 1/m) for the whole step: one 8-byte or two 4-byte fields
+    more than the same case with red-black SOR."""
+    xf = np.linspace(0.0, 16.0, 257)
+    pitched = 320 * 258  # rows of 258 cells padded to a multiple of 64
+    for prec in (iff.IF_FP64, iff.IF_MIXED):
+        rb, line = (iff.Batch.footprint(xf, xf, precision=prec, dt=0.01, re=100.0, ppe_solver=sv)
+                    for sv in (iff.IF_PPE_RBSOR, iff.IF_PPE_LINE))
+        assert 8 * pitched <= line - rb <= 8 * pitched + 2 * 1024, (prec, line - rb)
