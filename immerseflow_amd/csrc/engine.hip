@@ -673,19 +673,27 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
-    // FP64 default shape: 16-wave tiles of 80 rows as persistent chunks
-    // when the running cases' tiles at that shape fill at least 3/4 of the
-    // device in one launch (the 2048 x 1024 case: 247 tiles, 1.42x faster
-    // per sweep); otherwise 8-wave tiles of 32 rows, which keep more
-    // blocks busy on smaller grids
+    // FP64 default shape, by measurement: 16-wave tiles of 80 rows as
+    // persistent chunks when the running cases' tiles at that shape fill at
+    // least 3/4 of the device in one launch (the 2048 x 1024 case: 247
+    // tiles, 1.42x faster per sweep); else 48-row persistent chunks when
+    // they fit (1.1-1.3x faster than per-stage launches on 128^2..1024^2);
+    // else per-stage launches of 8-wave tiles of 32 rows
     if (shape_auto && pprec_ == IF_FP64) {
-        int nt80 = 0;
-        for (int id : order_)
-            nt80 += ((slots_[id]->NX + reg_out_cols(4) - 1) / reg_out_cols(4)) * ((slots_[id]->NY + 79) / 80);
-        const int cap = pers_blocks(IF_FP64, 16, 80, 4);
-        const bool big = use_persistent && cap > 0 && pers_fits(IF_FP64, 16, 80, 4, nt80) && 4 * nt80 >= 3 * cap;
-        sweep_kind = big ? 4 : 3;
-        sweep_ty = big ? 80 : 32;
+        auto tiles_at = [&](int ty) {
+            int nt = 0;
+            for (int id : order_)
+                nt += ((slots_[id]->NX + reg_out_cols(4) - 1) / reg_out_cols(4)) * ((slots_[id]->NY + ty - 1) / ty);
+            return nt;
+        };
+        const int nt80 = tiles_at(80), cap80 = pers_blocks(IF_FP64, 16, 80, 4);
+        int ty = 32;
+        if (use_persistent && cap80 > 0 && pers_fits(IF_FP64, 16, 80, 4, nt80) && 4 * nt80 >= 3 * cap80)
+            ty = 80;
+        else if (use_persistent && pers_fits(IF_FP64, 16, 48, 4, tiles_at(48)))
+            ty = 48;
+        sweep_kind = ty == 32 ? 3 : 4;
+        sweep_ty = ty;
         sweep_m = 4;
     }
     if (n > cap_cases_) {

@@ -232,9 +232,10 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * temporal blocking; check_every is the number of sweeps between
  * convergence checks.  Results do not depend on the kernel, the tile
  * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps
- * (mixed); FP64: kernel 4, 80 rows, 4 sweeps as persistent chunks when the
- * running cases' tiles at that shape fill at least 3/4 of the device in
- * one launch, else kernel 3, 32 rows, 4 sweeps (until tuned). */
+ * (mixed); FP64 (until tuned): kernel 4, 80 rows, 4 sweeps as persistent
+ * chunks when the running cases' tiles at that shape fill at least 3/4 of
+ * the device in one launch, else kernel 4, 48 rows, 4 sweeps as persistent
+ * chunks when they fit, else kernel 3, 32 rows, 4 sweeps. */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Launch mode of the Poisson sweeps (no counterpart in the reference,
