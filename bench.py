@@ -1,24 +1,45 @@
 #!/usr/bin/env python3
 """Throughput of the fractional step on one MI355X.
 
-Workload: the single large case -- an ellipse (chord 1, thickness ratio 0.1)
-at 10 deg incidence, Re = 300, on a 2048 x 1024 grid (uniform h = 0.005 in a
-4 x 2 box around the body, stretched to a 30 x 20 domain), dt = 0.5 h.  A
-step is one full fractional step (predictor, divergence RHS, Poisson to
-max|div| <= 1e-6, projection, divergence check) on device-resident fields.
+Headline workload (the largest single-GPU configuration): eight seeded star
+bodies in a 20 x 20 box, 4096 x 4096 uniform grid, Re = 1000, dt = 0.5 h,
+Poisson stop max|div| <= 1e-7, mixed precision (FP32 Poisson error sweeps,
+FP64 residual and defect correction, FP64 velocity passes, faces and
+divergence).  A step is one full fractional step (predictor, divergence
+RHS, Poisson, projection, divergence check) on device-resident fields.
+Step 1 runs untimed and is followed by the autotuner; then --warmup - 1
+more untimed steps and --steps timed steps.
 
-value = interior cells x steps / wall seconds (Mcell-steps/s) of the
-mixed-precision path (FP32 Poisson error sweeps, FP64 residual and defect
-correction, FP64 velocity passes, faces and divergence); the FP64 path and
-the FP32-velocity variant (IF_MIXED_V32) are timed beside it, each with its
-error against the FP64 path.  roofline: the Poisson sweep kernel (the dominant kernel), algorithmic
-bytes per launch / mean launch time from HIP events recorded by the library
-on its stream during the timed steps.  cpu_baseline: the C oracle (the same
-per-cell routines, OpenMP) timed for one step from the GPU's state.
+value = interior cells x steps / wall seconds (Mcell-steps/s).
+
+roofline: the dominant kernel is the Poisson sweep.  Temporal blocking
+keeps several red-black sweeps per launch on chip, so the sweep is bound by
+vector-ALU issue, not by HBM: achieved = FP32 operations of the cell
+updates the launch executes (11 per update: one multiply and five fused
+multiply-adds; the temporal-blocking halo recompute included) / mean
+launch time from HIP events recorded by the library on its stream; peak =
+MI355X's published FP32 vector rate.  The measured HBM traffic of the same
+launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, profiles/) gives the HBM
+fraction beside it.
+
+cpu_baseline: the C oracle (the same per-cell routines in FP64, OpenMP) on
+the host's cores, on a bounded sample of the headline workload: one step
+from the GPU's state with the Poisson loop capped, extrapolated to the
+GPU's sweep count.  The 256^2 cylinder (the CPU-runnable workload) runs on
+the CPU in full (500 steps) beside it.
+
+Beside the headline: the 2048 x 1024 ellipse (all 2000 steps, mixed; FP64
+and the FP32-velocity variant with their errors), the 256^2 cylinder on the
+GPU (all 500 steps, FP64), the 256-case batch (all 200 steps, against the
+plain FP64 single-case path on all 256 cases over a window of steps), a
+prefix of the serving trace at 50 req/s and at a sub-saturating rate, and
+the line-SOR Poisson option.
 """
 import argparse
 import json
+import math
 import os
+import subprocess
 import sys
 import time
 
@@ -26,40 +47,45 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X published FP32 vector peak (TFLOP/s); it counts packed FP32 (two
+# lanes per operation).  Without packed instructions the issue limit is
+# half of it: 256 CUs x 64 lanes x 2 FLOP (FMA) x 2.4 GHz.
+FP32_PEAK_TFLOPS = 157.3
+FP32_NONPACKED_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12
+FLOP_PER_UPDATE = 11     # FP32 red-black update: 1 multiply + 5 FMA
+VALU_PER_UPDATE = 6      # vector instructions per update (lane shift folded in)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--fp64-steps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--kernel", type=int, default=4,
-                    help="0 LDS-tiled, 1 register-streaming, 2/3/4 register-tile (4/8/16 waves)")
-    ap.add_argument("--tile-y", type=int, default=96)
-    ap.add_argument("--sweeps-per-launch", type=int, default=8)
-    ap.add_argument("--check-every", type=int, default=480)
-    ap.add_argument("--fp64-kernel", type=int, default=4)
-    ap.add_argument("--fp64-tile-y", type=int, default=80)
-    ap.add_argument("--fp64-sweeps-per-launch", type=int, default=4)
-    ap.add_argument("--fp64-check-every", type=int, default=160)
-    ap.add_argument("--omega", type=float, default=1.998)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--full", action="store_true",
+                    help="also run the headline workload for all 1000 steps and the FP64 single-case path "
+                         "on all 256 batch cases for all 200 steps (long)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the Poisson sweep kernels one by one instead of replaying a hipGraph")
-    ap.add_argument("--no-persistent", action="store_true",
-                    help="one Poisson launch per sweeps-per-launch stage instead of one per check interval")
+    ap.add_argument("--no-persistent", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--no-v32", action="store_true")
     ap.add_argument("--no-line", action="store_true")
-    ap.add_argument("--line-cases", default="c1,c2", help="line-SOR workloads to time (c1, c2)")
-    ap.add_argument("--c3-steps", type=int, default=2)
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=2000)
+    ap.add_argument("--c2-fp64-steps", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-window", type=int, default=2)
+    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--cpu-sweeps", type=int, default=100,
+                    help="Poisson sweeps of each capped oracle step in the CPU sample")
     return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def kernel_name(kind, ty, m, t, pers=False):
@@ -72,8 +98,13 @@ def kernel_name(kind, ty, m, t, pers=False):
     return "k_rbsor_reg<%s, %d, %d, %d, %s>" % (t, m, nw, (ty + 4 * m) // nw, "true" if pers else "false")
 
 
-def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+def reg_updates_per_sweep(NX, NY, kind, ty, m):
+    """Cell updates one red-black sweep of the register-tile kernel executes
+    over an NX x NY interior: every tile computes its whole 128-column,
+    (ty + 4m)-row block (the stored region plus the 2m halo it recomputes)."""
+    tw = 128 - 4 * m
+    ntiles = -(-NX // tw) * -(-NY // ty)
+    return ntiles * 128 * (ty + 4 * m)
 
 
 def rel_err(a, b):
@@ -90,184 +121,433 @@ def fields_err(get_mixed, get_fp64):
     out = {f: rel_err(get_mixed(f), get_fp64(f)) for f in ("u", "v", "p")}
     pm, pf = get_mixed("p"), get_fp64("p")
     out["p_mod_const"] = rel_err(pm - pm.mean(), pf - pf.mean())
-    return out
+    return {f: float("%.3g" % v) for f, v in out.items()}
 
 
-def make_case(iff, cfg, precision, a):
-    if precision == iff.IF_FP64:
-        shape = (a.fp64_kernel, a.fp64_tile_y, a.fp64_sweeps_per_launch, a.fp64_check_every)
-    else:
-        shape = (a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
-    c = iff.Case(cfg["xf"], cfg["yf"], precision=precision, dt=cfg["dt"], re=cfg["re"],
-                 bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_RBSOR,
-                 ppe_omega=a.omega, ppe_itermax=200000, ppe_tol=cfg["ppe_tol"],
-                 check_every=shape[3])
-    c.tune(*shape)
+def case_kw(cfg, **over):
+    kw = dict(dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6,
+              ppe_itermax=400000, ppe_tol=cfg["ppe_tol"], check_every=160)
+    kw.update(over)
+    return kw
+
+
+def new_case(iff, cfg, prec, a, **over):
+    c = iff.Case(cfg["xf"], cfg["yf"], precision=prec, **case_kw(cfg, **over))
     c.graphs(not a.no_graphs)
     c.persistent(not a.no_persistent)
     return c
 
 
-def timed_run(case, warmup, steps, cells, label):
-    for k in range(warmup):
-        st = case.step(1)
-        log(label, "warmup", k, "sweeps", st["ppe_sweeps"], "div", "%.2e" % st["div_max"])
-    case.sweep_timing(reset=True)
-    t0 = time.perf_counter()
-    st = case.step(steps)
-    t1 = time.perf_counter()
-    ms, launches = case.sweep_timing(reset=True)
-    dt = t1 - t0
-    log(label, "timed", steps, "steps in %.3f s" % dt, "last sweeps", st["ppe_sweeps"],
-        "total sweeps", st["total_ppe_sweeps"], "max div %.2e" % st["max_div"])
-    return dict(seconds=dt, stats=st, sweep_ms=ms, sweep_launches=launches,
-                mcs=cells * steps / dt / 1e6)
+def cpu_threads():
+    """Threads for the oracle: OMP_NUM_THREADS when set (the GPU host sets
+    it to its per-GPU CPU share), else the cores this process may use."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if n <= 0:
+        n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return n
 
 
-def cpu_baseline(cfg, gpu_case, st, omega):
-    """One oracle step (FP64, OpenMP) from the GPU case's state."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+def host_cores():
+    """(physical cores, logical CPUs) the host reports (lscpu)."""
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+        cores = {tuple(ln.split(",")) for ln in out.splitlines() if ln and not ln.startswith("#")}
+        return len(cores), os.cpu_count()
+    except Exception:
+        return None, os.cpu_count()
+
+
+def oracle_case(cfg, omega, ppe_itermax, check_every):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    prm = dict(dt=cfg["dt"], re=cfg["re"], conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25,
-               ad_tol=1e-6, ppe_solver=1, ppe_omega=0.0, ppe_itermax=max(1, 2 * st["ppe_sweeps"]),
-               ppe_tol=cfg["ppe_tol"], check_every=20)
+    prm = dict(dt=cfg["dt"], re=cfg["re"], conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6,
+               ppe_solver=1, ppe_omega=omega, ppe_itermax=ppe_itermax, ppe_tol=cfg["ppe_tol"],
+               check_every=check_every, ad_omega=1.0)
     bodies = []
     for b in cfg["bodies"]:
         d = dict(b)
         d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
         bodies.append(d)
-    oc = O.PhysCase(cfg["xf"], cfg["yf"], bodies, prm)
-    oc.prm.ppe_omega = omega
-    for name in ("u", "v", "p", "uf", "vf"):
-        getattr(oc, name)[:] = gpu_case.get(name)
+    return O.PhysCase(cfg["xf"], cfg["yf"], bodies, prm)
+
+
+def gpu_omega(cfg):
+    """The library's default relaxation: 2/(1 + sin(pi/N)), N the larger cell count."""
+    n = max(len(cfg["xf"]) - 1, len(cfg["yf"]) - 1)
+    return 2.0 / (1.0 + math.sin(math.pi / n))
+
+
+# ---------------------------------------------------------------------------
+# headline: the 4096^2 eight-star workload
+# ---------------------------------------------------------------------------
+def headline_c3(iff, configs, a):
+    cfg = configs.c3()
+    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+    cells = NX * NY
+    c = new_case(iff, cfg, iff.IF_MIXED, a)
+    st1 = c.step(1)
+    log("c3 step 1: sweeps", st1["ppe_sweeps"], "div %.2e" % st1["div_max"])
     t0 = time.perf_counter()
-    out = oc.step(1)[-1]
-    t1 = time.perf_counter()
+    tr = c.autotune()
+    t_tune = time.perf_counter() - t0
+    log("c3 autotune", tr, "%.3f s" % t_tune)
+    for k in range(max(0, a.warmup - 1)):
+        st = c.step(1)
+        log("c3 warmup", k + 2, "sweeps", st["ppe_sweeps"], "div %.2e" % st["div_max"])
+    c.sweep_timing(reset=True)
+    t0 = time.perf_counter()
+    st = c.step(a.steps)
+    dt = time.perf_counter() - t0
+    ms, launches = c.sweep_timing(reset=True)
+    sweep_bpc, step_bpc = c.byte_model()
+    spl = int(round(sweep_bpc / 13.0))         # sweeps per launch (13 B/cell-sweep, mixed)
+    pers = spl > tr["sweeps_per_launch"]
+    log("c3 timed %d steps in %.3f s, last sweeps %d, total %d, max div %.2e"
+        % (a.steps, dt, st["ppe_sweeps"], st["total_ppe_sweeps"], st["max_div"]))
+    avg_s = ms * 1e-3 / max(1, launches)
+    upd = reg_updates_per_sweep(NX, NY, tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"]) * spl
+    useful = cells * spl
+    kname = kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float", pers)
+    achieved = upd * FLOP_PER_UPDATE / avg_s / 1e12 if avg_s > 0 else 0.0
+    roof = dict(bound="valu", achieved=round(achieved, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None, kernel=kname,
+                frac_of_nonpacked_issue=round(achieved / FP32_NONPACKED_TFLOPS, 4),
+                nonpacked_peak=round(FP32_NONPACKED_TFLOPS, 2),
+                updates_per_launch=upd, interior_updates_per_launch=useful,
+                recompute_factor=round(upd / useful, 3), sweeps_per_launch=spl,
+                flop_per_update=FLOP_PER_UPDATE, valu_per_update=VALU_PER_UPDATE,
+                mean_launch_us=round(avg_s * 1e6, 3), launches_timed=launches,
+                useful_tflops=round(useful * FLOP_PER_UPDATE / avg_s / 1e12, 2) if avg_s > 0 else None,
+                note="executed updates x 11 FLOP / HIP-event launch time; the published peak counts packed "
+                     "FP32, which this build does not emit (frac_of_nonpacked_issue: against 256 CUs x 64 "
+                     "lanes x 2 FLOP x 2.4 GHz)")
+    tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c3.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tj = json.load(f)
+        if tj.get("kernel") == kname and tj.get("sweeps_per_launch") == spl:
+            roof["traffic"] = tj.get("bytes_per_launch")
+            hb = tj["bytes_per_launch"] / avg_s / 1e9 if avg_s > 0 else None
+            roof["hbm"] = dict(achieved=round(hb, 1) if hb else None, peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=round(hb / HBM_PEAK_GBS, 4) if hb else None,
+                               source="profiles/traffic_rbsor_reg_c3.json (rocprofv3 FETCH_SIZE, WRITE_SIZE, "
+                                      "calibrated) / the HIP-event launch time")
+            if tj.get("valu_insts_per_launch"):
+                v = tj["valu_insts_per_launch"] * 64 / avg_s / 1e12
+                roof["valu_counter"] = dict(lane_ops_tops=round(v, 2),
+                                            frac_of_issue=round(v / (FP32_NONPACKED_TFLOPS / 2), 4),
+                                            source="SQ_INSTS_VALU x 64 lanes / launch time")
+    out = dict(case=c, cfg=cfg, tune=tr, cells=cells, NX=NX, NY=NY, seconds=dt, stats=st, roofline=roof,
+               step_bpc=step_bpc, mcs=cells * a.steps / dt / 1e6, tune_seconds=t_tune, pers=pers)
+    return out
+
+
+def c3_fp64(iff, cfg, mixed_case, steps, a):
+    """The FP64 path on the headline workload (library defaults) for the
+    same number of steps from the start: rate and the mixed path's error."""
+    c64 = new_case(iff, cfg, iff.IF_FP64, a)
+    t0 = time.perf_counter()
+    s64 = c64.step(steps)
+    dt = time.perf_counter() - t0
+    out = dict(mcell_steps_per_s=round((len(cfg["xf"]) - 1) * (len(cfg["yf"]) - 1) * steps / dt / 1e6, 3),
+               steps=steps, max_div=s64["max_div"])
+    return c64, out
+
+
+def c3_full(iff, configs, a):
+    """--full: all 1000 steps of the headline workload (mixed)."""
+    cfg = configs.c3()
+    c = new_case(iff, cfg, iff.IF_MIXED, a)
+    c.step(1)
+    c.autotune()
+    t0 = time.perf_counter()
+    done = 1
+    while done < cfg["steps"]:
+        n = min(100, cfg["steps"] - done)
+        st = c.step(n)
+        done += n
+        log("c3 full: %d steps, %.1f s, last sweeps %d, max div %.2e" % (done, time.perf_counter() - t0,
+                                                                        st["ppe_sweeps"], st["max_div"]))
+    dt = time.perf_counter() - t0
+    c.close()
+    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+    return dict(value=round(NX * NY * (cfg["steps"] - 1) / dt / 1e6, 3), unit="Mcell-steps/s",
+                steps=cfg["steps"] - 1, seconds=round(dt, 2),
+                sample="steps 2..1000 (step 1 untimed, then autotune)")
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines (the oracle)
+# ---------------------------------------------------------------------------
+def cpu_c3_sample(cfg, gpu_case, gpu_sweeps_per_step, nsw):
+    """Oracle steps (FP64, OpenMP) from the GPU's state with the Poisson
+    loop capped at nsw and 2*nsw sweeps: t(step) = t_nsw + (S - nsw) t_sweep,
+    S the GPU's mean sweeps per step, t_sweep from the difference."""
+    threads = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    oc = oracle_case(cfg, gpu_omega(cfg), nsw, nsw)
+    state = {name: gpu_case.get(name) for name in ("u", "v", "p", "uf", "vf")}
+    times = []
+    for cap in (nsw, 2 * nsw):
+        for name, arr in state.items():
+            getattr(oc, name)[:] = arr
+        oc.prm.ppe_itermax = cap
+        oc.prm.check_every = cap
+        t0 = time.perf_counter()
+        out = oc.step(1)[-1]
+        times.append(time.perf_counter() - t0)
+        assert out[2] == cap
+    t_sweep = (times[1] - times[0]) / nsw
+    t_step = times[0] + (gpu_sweeps_per_step - nsw) * t_sweep
     cells = (oc.nx - 2) * (oc.ny - 2)
-    return dict(value=round(cells / (t1 - t0) / 1e6, 4), unit="Mcell-steps/s", cores=threads,
-                kind="port",
-                sample="1 FP64 step of the same case from the GPU state (%d AD sweeps, %d "
-                       "Poisson sweeps, %.1f s)" % (out[0], out[2], t1 - t0))
+    phys, logical = host_cores()
+    return dict(value=round(cells / t_step / 1e6, 4), unit="Mcell-steps/s", cores=threads, kind="port",
+                host_physical_cores=phys, host_logical_cpus=logical,
+                sample="headline workload, one FP64 step from the GPU state timed twice with the Poisson loop "
+                       "capped at %d and %d sweeps (%.1f s, %.1f s); per-step time extrapolated to the GPU's "
+                       "%.0f sweeps per step: %.1f s (%.3f s per sweep)"
+                       % (nsw, 2 * nsw, times[0], times[1], gpu_sweeps_per_step, t_step, t_sweep))
+
+
+def cpu_c1_full(configs):
+    cfg = configs.c1()
+    threads = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    oc = oracle_case(cfg, gpu_omega(cfg), 200000, 10)
+    t0 = time.perf_counter()
+    out = oc.step(cfg["steps"])
+    dt = time.perf_counter() - t0
+    cells = (oc.nx - 2) * (oc.ny - 2)
+    return dict(value=round(cells * cfg["steps"] / dt / 1e6, 4), unit="Mcell-steps/s", cores=threads,
+                kind="port", seconds=round(dt, 2), steps=cfg["steps"], max_div=max(o[4] for o in out),
+                sweeps_last_step=out[-1][2],
+                sample="256^2 cylinder, all 500 FP64 steps from the initial state, stop max|div| <= 1e-8")
+
+
+# ---------------------------------------------------------------------------
+# other workloads
+# ---------------------------------------------------------------------------
+def gpu_c1(iff, configs, a):
+    cfg = configs.c1()
+    c = new_case(iff, cfg, iff.IF_FP64, a, check_every=10)
+    t0 = time.perf_counter()
+    st = c.step(cfg["steps"])
+    dt = time.perf_counter() - t0
+    c.close()
+    return dict(value=round(256 * 256 * cfg["steps"] / dt / 1e6, 3), unit="Mcell-steps/s", steps=cfg["steps"],
+                seconds=round(dt, 3), max_div=st["max_div"], sweeps_last_step=st["ppe_sweeps"],
+                sample="all 500 steps, FP64, library defaults, stop max|div| <= 1e-8")
+
+
+def c2_workload(iff, configs, a):
+    """The 2048 x 1024 ellipse: all --c2-steps steps mixed (check every 480),
+    the FP64 path and the FP32-velocity variant over a window, errors."""
+    cfg = configs.c2()
+    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
+    cells = NX * NY
+    kw = dict(ppe_omega=1.998, ppe_itermax=200000, check_every=480)
+    c = new_case(iff, cfg, iff.IF_MIXED, a, **kw)
+    c.tune(4, 96, 8, 480)
+    t0 = time.perf_counter()
+    done, tot = 0, 0
+    while done < a.c2_steps:
+        n = min(500, a.c2_steps - done)
+        st = c.step(n)
+        done += n
+        tot += st["total_ppe_sweeps"]
+        log("c2 %d steps %.2f s, last sweeps %d, max div %.2e" % (done, time.perf_counter() - t0, st["ppe_sweeps"],
+                                                                 st["max_div"]))
+    dt = time.perf_counter() - t0
+    c.close()
+    out = dict(value=round(cells * a.c2_steps / dt / 1e6, 3), unit="Mcell-steps/s", steps=a.c2_steps,
+               seconds=round(dt, 2), mean_sweeps_per_step=round(tot / a.c2_steps, 1),
+               sample="all %d steps from the initial state, mixed, 16 waves x 96 rows x 8 sweeps, persistent, "
+                      "check every 480" % a.c2_steps)
+    if a.no_fp64:
+        return out
+    n = a.c2_fp64_steps
+    c64 = new_case(iff, cfg, iff.IF_FP64, a, **dict(kw, check_every=160))
+    c64.tune(4, 80, 4, 160)
+    c64.step(1)
+    t0 = time.perf_counter()
+    s64 = c64.step(n)
+    dt64 = time.perf_counter() - t0
+    cm = new_case(iff, cfg, iff.IF_MIXED, a, **kw)
+    cm.tune(4, 96, 8, 480)
+    t0 = time.perf_counter()
+    cm.step(1)
+    sm = cm.step(n)
+    dtm = time.perf_counter() - t0
+    out["fp64"] = dict(value=round(cells * n / dt64 / 1e6, 3), steps="2..%d" % (1 + n), max_div=s64["max_div"],
+                       mixed_same_window=round(cells * (1 + n) / dtm / 1e6, 3))
+    out["mixed_error_vs_fp64"] = fields_err(cm.get, c64.get)
+    cm.close()
+    cv = new_case(iff, cfg, iff.IF_MIXED_V32, a, **kw)
+    cv.tune(4, 96, 8, 480)
+    cv.step(1 + n)
+    out["mixed_fp32_velocity_error_vs_fp64"] = fields_err(cv.get, c64.get)
+    cv.close()
+    c64.close()
+    out["error_note"] = ("max|x - FP64| / max(1, max|FP64|) after %d steps from the same start; both solves "
+                         "stop at max|div| <= 1e-6" % (1 + n))
+    return out
 
 
 def batched_c4(iff, configs, a):
-    """The batched workload (256 seeded uneven cases, 128^2..1024^2) in one
-    resident batch, mixed precision, against the plain FP64 single-case path
-    (library defaults, one case at a time) on a sample of the same cases.
-    Step 1 runs untimed on both; the batch is then autotuned (launch shape
-    and check interval, timed separately) and steps 2 .. 1+--batch-steps are
-    timed on both, expressed per 200-step case."""
+    """The 256 seeded uneven cases (128^2..1024^2) resident in one batch,
+    mixed precision, for all --batch-steps steps: step 1 untimed, autotune,
+    steps 2..200 timed.  Beside it, over the window of steps 2..1+W: the same
+    batch and the plain FP64 single-case path (library defaults, one case at
+    a time) on every one of the 256 cases."""
     cases = configs.c4_cases(256)
     b = iff.Batch(iff.IF_MIXED, hbm_budget=0)
-    b.tune(a.kernel, a.tile_y, a.sweeps_per_launch, a.check_every)
     b.graphs(not a.no_graphs)
     b.persistent(not a.no_persistent)
-    t0 = time.perf_counter()
     slots = [b.add(c["xf"], c["yf"], c["steps"], dt=c["dt"], re=c["re"], bodies=c["bodies"],
-                   ppe_tol=1e-6, check_every=a.check_every) for c in cases]
-    t_setup = time.perf_counter() - t0
+                   ppe_tol=1e-7, check_every=160) for c in cases]
     b.step(1)
     t0 = time.perf_counter()
     tr = b.autotune()
     t_tune = time.perf_counter() - t0
     log("batch autotune:", tr, "%.3f s" % t_tune)
+    W = a.batch_window
     i0 = b.info()
     t0 = time.perf_counter()
-    b.step(a.batch_steps)
-    t_b = time.perf_counter() - t0
-    info = b.info()
-    divs = [b.state(s)[2]["max_div"] for s in slots]
+    b.step(W)
+    t_win = time.perf_counter() - t0
+    i1 = b.info()
+    rest = a.batch_steps - 1 - W
+    t1 = time.perf_counter()
+    done = 0
+    while done < rest:
+        n = min(50, rest - done)
+        b.step(n)
+        done += n
+        log("batch: %d of %d steps, %.1f s" % (1 + W + done, a.batch_steps, time.perf_counter() - t0))
+    t_all = time.perf_counter() - t0
+    states = [b.state(s) for s in slots]
+    finished = sum(1 for s in states if s[0] == 1)
+    divs = [s[2]["max_div"] for s in states]
+    timed_steps = a.batch_steps - 1
+    cps = len(cases) / (t_all * a.batch_steps / timed_steps)
+    ms = i1["sweep_ms"] - i0["sweep_ms"]
+    ups = (i1["cell_sweeps"] - i0["cell_sweeps"]) / (ms * 1e-3) if ms > 0 else None
+    # the FP64 single-case path on every case, the same window of steps;
+    # every 16th case's mixed result is compared with it field by field
+    keep = {k: {f: b.get(slots[k], f) for f in ("u", "v", "p")} for k in range(0, 256, 16)}
+    b.close()
     cells = sum((len(c["xf"]) - 1) * (len(c["yf"]) - 1) for c in cases)
-    per_step = t_b / a.batch_steps
-    cps = len(cases) / (200.0 * per_step)
-    ms = info["sweep_ms"] - i0["sweep_ms"]
-    gbs = (info["cell_sweeps"] - i0["cell_sweeps"]) * 13.0 / (ms * 1e-3) / 1e9 if ms > 0 else None
-    # plain FP64 single-case path on every 32nd case, run one after another
-    # (same steps as the batch: the sampled cases' mixed results are compared
-    # with these FP64 results field by field)
     tot = 0.0
-    sample = list(range(0, len(cases), 32))
     err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
-    for k in sample:
-        c = cases[k]
+    for k, c in enumerate(cases):
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
-                     ppe_tol=1e-6, check_every=a.fp64_check_every)
+                     ppe_tol=1e-7, check_every=160)
         s.graphs(not a.no_graphs)
         s.persistent(not a.no_persistent)
         s.step(1)
         t0 = time.perf_counter()
-        s.step(a.batch_steps)
+        s.step(W)
         tot += time.perf_counter() - t0
-        e = fields_err(lambda f: b.get(slots[k], f), s.get)
-        err = {f: max(err[f], e[f]) for f in err}
+        if k in keep and W + 1 == a.batch_steps:
+            e = fields_err(lambda f: keep[k][f], s.get)
+            err = {f: max(err[f], e[f]) for f in err}
         s.close()
-    b.close()
-    single_cps = len(sample) / (200.0 * tot / a.batch_steps)
-    log("batched: %.3f s/step for 256 cases, %.2f cases/s (FP64 single-case %.3f cases/s)"
-        % (per_step, cps, single_cps))
-    return dict(value=round(cps, 3), unit="cases/s", cases=len(cases), steps_timed=a.batch_steps,
-                ms_per_batched_step=round(1e3 * per_step, 2), mcell_steps_per_s=round(cells / per_step / 1e6, 2),
-                sweep_gbs=round(gbs, 1) if gbs else None,
-                sweep_roofline_frac=round(gbs / HBM_PEAK_GBS, 3) if gbs else None, max_div=max(divs),
-                fp64_single_case_cases_per_s=round(single_cps, 4),
-                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
-                speedup_vs_fp64_single=round(cps / single_cps, 2), setup_s=round(t_setup, 2),
-                autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
-                              check_every=tr["check_every"], sweep_ns_per_cell=round(tr["sweep_ns"], 4),
-                              check_in_sweeps=round(tr["check_sweeps"], 2), typical_sweeps=tr["typical_sweeps"],
-                              candidates=tr["candidates"], seconds=round(t_tune, 3)),
-                sample="all 256 cases resident; step 1 untimed, then autotune, then steps 2..%d of 200 timed; "
-                       "FP64 single-case (library defaults) on 8 of them (every 32nd), same steps; rates per "
-                       "200-step case; error_vs_fp64: max over those 8 of max|mixed - FP64| / max(1, max|FP64|) "
-                       "after step %d" % (1 + a.batch_steps, 1 + a.batch_steps))
+    single_cps_win = len(cases) / (tot * a.batch_steps / W)
+    batch_cps_win = len(cases) / (t_win * a.batch_steps / W)
+    log("batched: %.2f cases/s over steps 2..%d; window %.2f vs FP64 single %.3f cases/s"
+        % (cps, a.batch_steps, batch_cps_win, single_cps_win))
+    out = dict(value=round(cps, 3), unit="cases/s", cases=len(cases), steps=a.batch_steps,
+               finished=finished, seconds_steps_2_to_end=round(t_all, 2),
+               mcell_steps_per_s=round(cells * timed_steps / t_all / 1e6, 2), max_div=max(divs),
+               window=dict(steps="2..%d" % (1 + W), batch_cases_per_s=round(batch_cps_win, 3),
+                           fp64_single_case_cases_per_s=round(single_cps_win, 4),
+                           speedup_vs_fp64_single=round(batch_cps_win / single_cps_win, 2),
+                           fp64_cases=len(cases)),
+               sweep_gcell_sweeps_per_s=round(ups / 1e9, 1) if ups else None,
+               autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
+                             check_every=tr["check_every"], candidates=tr["candidates"], seconds=round(t_tune, 3)),
+               sample="all 256 cases resident; step 1 untimed, autotune, steps 2..%d timed (value, per case of "
+                      "%d steps); window: steps 2..%d of the batch against the FP64 single-case path on all 256 "
+                      "cases one after another, same steps, rates per %d-step case"
+                      % (a.batch_steps, a.batch_steps, 1 + W, a.batch_steps))
+    if W + 1 != a.batch_steps:
+        out["error_note"] = "fields compared only when the window is the whole run (--batch-window 199)"
+    else:
+        out["error_vs_fp64"] = err
+    return out
 
 
-def complex_c3(iff, configs, a):
-    """The complex-geometry workload (4096^2, eight star bodies, Re 1000),
-    mixed precision: step 1 untimed, autotune, then --c3-steps timed; the
-    FP64 path (library defaults) runs the same steps for the error."""
-    cfg = configs.c3()
-    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
-    kw = dict(dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"], ad_itermax=25, ad_tol=1e-6,
-              ppe_itermax=400000, ppe_tol=cfg["ppe_tol"], check_every=a.check_every)
-    c = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_MIXED, **kw)
-    c.graphs(not a.no_graphs)
-    c.persistent(not a.no_persistent)
-    c.step(1)
-    tr = c.autotune()
+def fp64_single_full(iff, configs, a):
+    """--full: the FP64 single-case path on all 256 batch cases for all 200
+    steps (one case at a time)."""
+    cases = configs.c4_cases(256)
     t0 = time.perf_counter()
-    st = c.step(a.c3_steps)
+    for k, c in enumerate(cases):
+        s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
+                     ppe_tol=1e-7, check_every=160)
+        s.step(1)
+        if k == 0:
+            t0 = time.perf_counter()
+        s.step(c["steps"] - 1)
+        s.close()
+        if k % 16 == 15:
+            log("fp64 single full: %d cases, %.1f s" % (k + 1, time.perf_counter() - t0))
     dt = time.perf_counter() - t0
-    c64 = iff.Case(cfg["xf"], cfg["yf"], precision=iff.IF_FP64, **dict(kw, check_every=a.fp64_check_every))
-    t1 = time.perf_counter()
-    s64 = c64.step(1 + a.c3_steps)
-    dt64 = time.perf_counter() - t1
-    err = fields_err(c.get, c64.get)
-    c.close()
-    c64.close()
-    log("c3: %.3f s/step mixed, %.3f s/step FP64" % (dt / a.c3_steps, dt64 / (1 + a.c3_steps)))
-    return dict(value=round(NX * NY * a.c3_steps / dt / 1e6, 3), unit="Mcell-steps/s", NX=NX, NY=NY,
-                ms_per_step=round(1e3 * dt / a.c3_steps, 2), sweeps_last_step=st["ppe_sweeps"],
-                max_div=st["max_div"], ppe_tol=cfg["ppe_tol"],
-                fp64_mcell_steps_per_s=round(NX * NY * (1 + a.c3_steps) / dt64 / 1e6, 3),
-                fp64_max_div=s64["max_div"],
-                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
-                autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
-                              check_every=tr["check_every"]),
-                sample="step 1 untimed, autotune, steps 2..%d timed; FP64 from the same start, %d steps "
-                       "(the FP64 rate includes its first step)" % (1 + a.c3_steps, 1 + a.c3_steps))
+    return dict(value=round(len(cases) / (dt * 200 / 199), 4), unit="cases/s", seconds=round(dt, 1),
+                sample="all 256 cases, steps 2..200 each (step 1 untimed), one case at a time")
+
+
+def serving_c5(iff, configs, a):
+    """The serving trace's first --serve-requests requests, unclipped (grids
+    64^2..2048^2, 100..1000 steps, 20 % heaving cylinders), served at the
+    trace's 50 req/s and again with the arrival times stretched so that the
+    offered load is half the measured service rate."""
+    from immerseflow_amd import serve
+    trace = [(t, dict(c, check_every=160)) for t, c in configs.c5_trace(a.serve_requests)]
+    budget = 64 << 30
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto")
+    log("serving (50 req/s):", summ)
+    # sub-saturating: mean inter-arrival = 2x the mean service time per request
+    svc = summ["wall_s"] / max(1, summ["completed"])
+    span = trace[-1][0] if trace[-1][0] > 0 else 1.0
+    scale = 2.0 * svc * len(trace) / span
+    recs2, summ2 = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto", time_scale=scale)
+    log("serving (sub-saturating):", summ2)
+    # each result against the same request run alone (mixed) -- bit for bit
+    small = sorted(range(len(trace)), key=lambda k: (len(trace[k][1]["xf"]) ** 2) * trace[k][1]["steps"])[:2]
+    recs3, _ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto",
+                           keep_fields=set(small))
+    err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
+    for r in recs3:
+        if r["id"] in small and r["state"] == 1:
+            e = serve.error_vs_fp64(r, trace[r["id"]][1])
+            err = {f: max(err[f], e[f]) for f in err}
+
+    def pick(s):
+        return dict(cases_per_s=round(s["cases_per_s"], 4), p50_s=round(s["p50_s"], 3), p99_s=round(s["p99_s"], 3),
+                    mcell_steps_per_s=round(s["mcell_steps_per_s"], 2), max_div=s["max_div"],
+                    max_resident=s["max_resident"], wall_s=round(s["wall_s"], 2), completed=s["completed"])
+    sizes = [len(c["xf"]) - 1 for _, c in trace]
+    return dict(value=round(summ["cases_per_s"], 4), unit="cases/s", requests=len(trace),
+                at_50_req_per_s=pick(summ), sub_saturating=dict(pick(summ2), arrival_scale=round(scale, 2),
+                                                                 offered_req_per_s=round(len(trace) / (span * scale), 4)),
+                budget_bytes=budget, grids=sizes, steps=[c["steps"] for _, c in trace],
+                heaving=sum(1 for _, c in trace if c["bodies"][0].get("motion", 0)),
+                error_vs_fp64=dict({f: float("%.3g" % v) for f, v in err.items()}, requests=small),
+                sample="first %d requests of the seeded trace, unclipped (n %d..%d, %d..%d steps), served at 50 "
+                       "req/s and with arrivals stretched to half the measured service rate; error_vs_fp64: the "
+                       "two smallest requests' served fields against the FP64 path alone"
+                       % (len(trace), min(sizes), max(sizes), min(c["steps"] for _, c in trace),
+                          max(c["steps"] for _, c in trace)))
 
 
 def line_sor_f2(iff, configs, a):
-    """The deck's PPE_Solver 2 (zebra line SOR, one Thomas solve per row):
-    whole steps of the 256^2 cylinder to its 1e-8 stop, FP64 and mixed; and
-    the time per sweep on the 2048x1024 workload, from the sweep events of a
-    solve capped at 400 sweeps (a full solve there takes ~10^4 sweeps)."""
+    """The deck's PPE_Solver 2 (zebra line SOR): whole steps of the 256^2
+    cylinder to its 1e-8 stop, FP64 and mixed; and the time per sweep on the
+    2048x1024 workload, from the sweep events of a solve capped at 400
+    sweeps."""
     out = {}
-    runs = [("c1", configs.c1, 2, 200000), ("c2", configs.c2, 1, 400)]
-    for cname, mk, steps, cap in (r for r in runs if r[0] in a.line_cases.split(",")):
+    for cname, mk, steps, cap in (("c1", configs.c1, 2, 200000), ("c2", configs.c2, 1, 400)):
         cfg = mk()
         NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
         for pname, prec in (("fp64", iff.IF_FP64), ("mixed", iff.IF_MIXED)):
@@ -290,45 +570,8 @@ def line_sor_f2(iff, configs, a):
             log("line sor", cname, pname, r)
     out["sample"] = ("c1: 256^2 cylinder, step 1 untimed, steps 2..3 timed to max|div| <= 1e-8; c2: 2048x1024, "
                      "one step with the solve capped at 400 sweeps; us_per_sweep from HIP events around the "
-                     "sweep chunks (both colours, forward elimination and back substitution)")
+                     "sweep chunks (both colours)")
     return out
-
-
-def serving_c5(iff, configs, a):
-    """A bounded sample of the serving workload: the seeded Poisson-arrival
-    trace (50 req/s) with grids clipped to <= 512^2 and step counts / 20."""
-    from immerseflow_amd import serve
-    trace = []
-    for t, c in configs.c5_trace(a.serve_requests):
-        n = min(len(c["xf"]) - 1, 512)
-        xf = configs.uniform_faces(n, 20.0)
-        c = dict(c, xf=xf, yf=xf.copy(), dt=0.5 * 20.0 / n, steps=max(5, c["steps"] // 20), ppe_tol=1e-6,
-                 check_every=a.check_every)
-        trace.append((t, c))
-    sample = set(range(0, len(trace), 8))
-    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=16 << 30, tune="auto", keep_fields=sample)
-    log("serving:", summ)
-    # the error of sampled results against the FP64 path (after the loop)
-    err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
-    for r in recs:
-        if r["id"] in sample and r["state"] == 1:
-            e = serve.error_vs_fp64(r, trace[r["id"]][1])
-            err = {f: max(err[f], e[f]) for f in err}
-    return dict(value=round(summ["cases_per_s"], 3), unit="cases/s", requests=summ["requests"],
-                completed=summ["completed"], p50_s=round(summ["p50_s"], 4), p99_s=round(summ["p99_s"], 4),
-                mcell_steps_per_s=round(summ["mcell_steps_per_s"], 2), max_div=summ["max_div"],
-                max_resident=summ["max_resident"], budget_bytes=16 << 30,
-                error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
-                sample="first %d requests of the seeded trace (mean 50 req/s), grids clipped to <= 512^2, "
-                       "steps/20 (5..50), 20%% heaving cylinders; error_vs_fp64: max over every 8th request of "
-                       "max|served - FP64 alone| / max(1, max|FP64|)" % a.serve_requests)
-
-
-def gpu_omega(cfg):
-    """The library's default relaxation: 2/(1 + sin(pi/N)), N the larger cell count."""
-    import math
-    n = max(len(cfg["xf"]) - 1, len(cfg["yf"]) - 1)
-    return 2.0 / (1.0 + math.sin(math.pi / n))
 
 
 def main():
@@ -341,130 +584,91 @@ def main():
     from immerseflow_amd import configs
     n, arch, cus, mem = iff.device_query()
     log("device", arch, cus, "CUs")
-    cfg = configs.c2()
-    NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
-    cells = NX * NY
-    log("workload", cfg["name"], "ratios", ["%.6f" % r for r in cfg["ratios"]])
 
-    case = make_case(iff, cfg, iff.IF_MIXED, a)
-    mixed = timed_run(case, a.warmup, a.steps, cells, "mixed")
-    sweep_bpc, step_bpc = case.byte_model()
-    sweeps_per_launch = int(round(sweep_bpc / 13.0))  # 13 B per cell-sweep (mixed)
-    pers = sweeps_per_launch > a.sweeps_per_launch    # a whole check interval per launch
-    avg_ms = mixed["sweep_ms"] / max(1, mixed["sweep_launches"])
-    achieved = sweep_bpc * cells / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c2.json")
-    kname = kernel_name(a.kernel, a.tile_y, a.sweeps_per_launch, "float", pers)
-    if os.path.exists(tp):
-        with open(tp) as f:
-            tj = json.load(f)
-        # measured for this launch shape and sweeps per launch only
-        if tj.get("kernel") == kname and tj.get("sweeps_per_launch") == sweeps_per_launch:
-            traffic = tj.get("mixed_bytes_per_launch")
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel=kname,
-                bytes_per_launch=sweep_bpc * cells, sweeps_per_launch=sweeps_per_launch,
-                mean_launch_us=round(avg_ms * 1e3, 3), launches_timed=mixed["sweep_launches"])
-
-    fp64 = None
-    if not a.no_fp64 and a.fp64_steps > 0:
-        c64 = make_case(iff, cfg, iff.IF_FP64, a)
-        r64 = timed_run(c64, max(1, a.warmup), a.fp64_steps, cells, "fp64")
-        b64, _ = c64.byte_model()
-        avg64 = r64["sweep_ms"] / max(1, r64["sweep_launches"])
-        # the mixed path's error: a fresh mixed case run the same steps
-        nst = max(1, a.warmup) + a.fp64_steps
-        cm = make_case(iff, cfg, iff.IF_MIXED, a)
-        cm.step(nst)
-        err = fields_err(cm.get, c64.get)
-        cm.close()
-        # and at a matched tight Poisson stop (1e-9), where the stopping
-        # error of either solve no longer shows: the precision effect alone
-        tight = dict(cfg, ppe_tol=1e-9)
-        ct = [make_case(iff, tight, pr, a) for pr in (iff.IF_MIXED, iff.IF_FP64)]
-        for x in ct:
-            x.step(3)
-        err_tight = fields_err(ct[0].get, ct[1].get)
-        for x in ct:
-            x.close()
-        pers64 = int(round(b64 / 25.0)) > a.fp64_sweeps_per_launch  # 25 B per cell-sweep (FP64)
-        fp64 = dict(value=round(r64["mcs"], 3), ms_per_step=round(1e3 * r64["seconds"] / a.fp64_steps, 3),
-                    kernel=kernel_name(a.fp64_kernel, a.fp64_tile_y, a.fp64_sweeps_per_launch, "double", pers64),
-                    us_per_sweep=round(1e3 * avg64 / max(1, int(round(b64 / 25.0))), 3),
-                    sweeps_last_step=r64["stats"]["ppe_sweeps"],
-                    max_div=r64["stats"]["max_div"],
-                    sweep_gbs=round(b64 * cells / (avg64 * 1e-3) / 1e9, 1) if avg64 > 0 else None,
-                    mixed_error_vs_fp64={f: float("%.3g" % v) for f, v in err.items()},
-                    error_after_steps=nst,
-                    mixed_error_vs_fp64_at_1e9={f: float("%.3g" % v) for f, v in err_tight.items()},
-                    error_note="max|mixed - FP64| / max(1, max|FP64|), same case and steps; at the production "
-                               "stop (max|div| <= 1e-6) the two solves stop at different iterates, so p differs "
-                               "by the stopping error; at a 1e-9 stop (3 steps) only precision differs")
-        c64.close()
-
-    # the FP32-velocity variant of the mixed path (IF_MIXED_V32), same case
-    v32 = None
-    if not a.no_v32:
-        cv = make_case(iff, cfg, iff.IF_MIXED_V32, a)
-        rv = timed_run(cv, a.warmup, a.steps, cells, "mixed_v32")
-        v32 = dict(value=round(rv["mcs"], 3), ms_per_step=round(1e3 * rv["seconds"] / a.steps, 3),
-                   max_div=rv["stats"]["max_div"])
-        cv.close()
-        if fp64 is not None:  # error after the FP64 run's step count, as for the mixed path
-            cv = make_case(iff, cfg, iff.IF_MIXED_V32, a)
-            cv.step(fp64["error_after_steps"])
-            c64 = make_case(iff, cfg, iff.IF_FP64, a)
-            c64.step(fp64["error_after_steps"])
-            v32["error_vs_fp64"] = {f: float("%.3g" % v) for f, v in fields_err(cv.get, c64.get).items()}
-            cv.close()
-            c64.close()
-
+    h = headline_c3(iff, configs, a)
+    case = h["case"]
     cpu = None
+    cpu_extra = {}
     if not a.no_cpu:
-        cpu = cpu_baseline(cfg, case, mixed["stats"], a.omega if a.omega > 0 else gpu_omega(cfg))
-        log("cpu baseline", cpu)
+        S = h["stats"]["total_ppe_sweeps"] / a.steps
+        cpu = cpu_c3_sample(h["cfg"], case, S, a.cpu_sweeps)
+        log("cpu baseline (headline sample)", cpu)
+    fp64 = None
+    if not a.no_fp64:
+        nst = 1 + max(0, a.warmup - 1) + a.steps
+        n64 = 3
+        c64, fp64 = c3_fp64(iff, h["cfg"], case, n64, a)
+        cm = new_case(iff, h["cfg"], iff.IF_MIXED, a)
+        cm.step(1)
+        cm.autotune()
+        cm.step(n64 - 1)
+        fp64["mixed_error_vs_fp64"] = fields_err(cm.get, c64.get)
+        fp64["error_note"] = ("max|mixed - FP64| / max(1, max|FP64|) after %d steps from the same start, both "
+                              "stopping at max|div| <= 1e-7; FP64 with library defaults" % n64)
+        cm.close()
+        c64.close()
+        del nst
+    case.close()
+    full = c3_full(iff, configs, a) if a.full else None
 
+    c2 = None if a.no_c2 else c2_workload(iff, configs, a)
+    c1 = None
+    if not a.no_c1:
+        c1 = dict(gpu_fp64=gpu_c1(iff, configs, a))
+        if not a.no_cpu:
+            c1["cpu_oracle"] = cpu_c1_full(configs)
+            c1["gpu_vs_cpu"] = round(c1["gpu_fp64"]["value"] / c1["cpu_oracle"]["value"], 2)
+            cpu_extra["c1_full"] = c1["cpu_oracle"]
+        log("c1", c1)
     batched = None if a.no_batch else batched_c4(iff, configs, a)
-    c3 = None if a.no_c3 else complex_c3(iff, configs, a)
+    if batched is not None and a.full:
+        batched["fp64_single_case_full"] = fp64_single_full(iff, configs, a)
+        batched["speedup_vs_fp64_single_full"] = round(batched["value"] /
+                                                       batched["fp64_single_case_full"]["value"], 2)
     serving = None if a.no_serve else serving_c5(iff, configs, a)
     line = None if a.no_line else line_sor_f2(iff, configs, a)
 
-    st = mixed["stats"]
+    st = h["stats"]
+    tr = h["tune"]
     out = {
         "metric": "Mcell-steps/s + % HBM roofline; batched cases/s and p50/p99 latency at div<=1e-6",
-        "value": round(mixed["mcs"], 3),
+        "value": round(h["mcs"], 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(1e3 * mixed["seconds"] / a.steps, 3),
+        "ms_per_step": round(1e3 * h["seconds"] / a.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": cfg["name"], "NX": NX, "NY": NY, "dt": cfg["dt"], "Re": cfg["re"],
+        "config": {"workload": "C3 eight seeded star bodies, 4096x4096 uniform, 20x20 box",
+                   "NX": h["NX"], "NY": h["NY"], "dt": h["cfg"]["dt"], "Re": h["cfg"]["re"],
+                   "bodies": len(h["cfg"]["bodies"]),
                    "precision": "mixed (FP32 Poisson error sweeps in registers, FP64 residual/defect "
-                                 "correction, FP64 velocity passes)",
-                   "poisson": "red-black SOR, stop max|div|<=%g" % cfg["ppe_tol"],
-                   "sweeps_last_step": st["ppe_sweeps"], "ad_iters_last_step": st["ad_iters"],
-                   "max_div": st["max_div"], "parallelism": "single GPU",
-                   "poisson_launch": ("persistent launch per check interval" if pers else
-                                      "launch per %d sweeps" % a.sweeps_per_launch) +
+                                "correction, FP64 velocity passes)",
+                   "poisson": "red-black SOR, stop max|div|<=%g" % h["cfg"]["ppe_tol"],
+                   "timed_steps": "%d..%d of %d" % (a.warmup + 1, a.warmup + a.steps, h["cfg"]["steps"]),
+                   "sweeps_last_step": st["ppe_sweeps"], "mean_sweeps_per_step": st["total_ppe_sweeps"] / a.steps,
+                   "ad_iters_last_step": st["ad_iters"], "max_div": st["max_div"], "parallelism": "single GPU",
+                   "autotune": dict(kernel=h["roofline"]["kernel"], check_every=tr["check_every"],
+                                    seconds=round(h["tune_seconds"], 3)),
+                   "poisson_launch": ("persistent launch per check interval" if h["pers"] else
+                                      "launch per %d sweeps" % tr["sweeps_per_launch"]) +
                                      ("" if a.no_graphs else ", hipGraph replay")},
-        "roofline": roof,
+        "roofline": h["roofline"],
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_extra or None,
         "fp64": fp64,
-        "mixed_fp32_velocity": v32,
+        "full_length": full,
+        "ellipse_2048x1024": c2,
+        "cylinder_256": c1,
         "batched": batched,
-        "complex": c3,
         "serving": serving,
         "line_sor": line,
-        "step_bytes_per_cell": round(step_bpc, 1),
+        "step_bytes_per_cell": round(h["step_bpc"], 1),
     }
-    case.close()
     print(json.dumps(out), flush=True)
 
 

@@ -170,7 +170,7 @@ class _BodyArray:
 def make_desc(xf, yf, *, mode=IF_MODE_PHYSICS, precision=IF_FP64, dt, re, conv_factor=0.5,
               ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6, ppe_solver=IF_PPE_RBSOR,
               ppe_omega=0.0, ppe_itermax=100000, ppe_tol=1e-8, check_every=10, t0=0.0,
-              bodies=()):
+              bodies=(), ad_omega=1.0):
     """A case descriptor plus the arrays it points into (keep both alive)."""
     xf = np.ascontiguousarray(xf, dtype=np.float64)
     yf = np.ascontiguousarray(yf, dtype=np.float64)
@@ -183,6 +183,7 @@ def make_desc(xf, yf, *, mode=IF_MODE_PHYSICS, precision=IF_FP64, dt, re, conv_f
     d.ppe_solver, d.ppe_omega, d.ppe_itermax = ppe_solver, ppe_omega, ppe_itermax
     d.ppe_tol, d.check_every, d.t0 = ppe_tol, check_every, t0
     d.nbody, d.bodies = len(bodies), ba.arr
+    d.ad_omega = ad_omega
     return d, (xf, yf, ba)
 
 
@@ -329,6 +330,11 @@ class Case:
             _check(_lib().if_case_monitor_read(self._h, _dptr(out), n.value, C.byref(n), C.byref(dropped)),
                    "monitor")
         return out[:n.value]
+
+    def monitor_sample(self):
+        """Append one monitor record of the current fields at the current
+        time, without a step (if_case_monitor_sample)."""
+        _check(_lib().if_case_monitor_sample(self._h), "monitor")
 
     def autotune(self):
         """Time every Poisson sweep launch shape on the current fields, keep
@@ -477,7 +483,8 @@ class ImmerseFlow:
                   ppe_itermax=I["ppe_itermax"], ppe_omega=I["ppe_omega"],
                   ppe_tol=I["ppe_tol"], check_every=I["check_every"],
                   ppe_solver={0: IF_PPE_JACOBI, 2: IF_PPE_LINE}.get(I["ppe_solver"], IF_PPE_RBSOR),
-                  ad_tol=1e-6, bodies=I["bodies"])
+                  ad_tol=1e-6, bodies=I["bodies"],
+                  ad_omega=I["ad_omega"] if I["ad_omega"] > 0 else float(I["w_ad"] if I["w_ad"] > 0 else 1))
         self.case = Case(self.xf, self.yf, **kw)
         return self.case
 

@@ -37,6 +37,7 @@ class IfInputs(C.Structure):
                 ("re", C.c_double), ("mu", C.c_double), ("write_interval", C.c_int),
                 ("mode", C.c_int), ("precision", C.c_int), ("conv_factor", C.c_double),
                 ("ppe_omega", C.c_double), ("ppe_tol", C.c_double), ("check_every", C.c_int),
+                ("ad_omega", C.c_double),
                 ("nbody", C.c_int), ("bodies", IfBody * IF_MAX_BODIES),
                 ("poly_store", ((C.c_double * 512) * 2) * IF_MAX_BODIES)]
 
@@ -48,7 +49,7 @@ class IfCaseDesc(C.Structure):
                 ("ad_itermax", C.c_int), ("ad_tol", C.c_double), ("ppe_solver", C.c_int),
                 ("ppe_omega", C.c_double), ("ppe_itermax", C.c_int), ("ppe_tol", C.c_double),
                 ("check_every", C.c_int), ("t0", C.c_double), ("nbody", C.c_int),
-                ("bodies", C.POINTER(IfBody))]
+                ("bodies", C.POINTER(IfBody)), ("ad_omega", C.c_double)]
 
 
 class IfStepStats(C.Structure):
@@ -104,6 +105,7 @@ SIGNATURES = [
     ("if_case_monitor", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("if_case_monitor_read", C.c_int, [C.c_void_p, _dp, C.c_size_t, C.POINTER(C.c_size_t),
                                        C.POINTER(C.c_size_t)]),
+    ("if_case_monitor_sample", C.c_int, [C.c_void_p]),
     ("if_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
     ("if_case_sweep_timing", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     ("if_case_byte_model", C.c_int, [C.c_void_p, _dp, _dp]),

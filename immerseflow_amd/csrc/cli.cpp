@@ -92,6 +92,9 @@ int main(int argc, char **argv)
     d.check_every = in->check_every;
     d.nbody = in->nbody;
     d.bodies = in->bodies;
+    // w_AD: the deck's integer (CFDInput::w_AD, globalVariables.cuh:27) or
+    // the real value of the "Mode" line
+    d.ad_omega = in->ad_omega > 0 ? in->ad_omega : in->w_ad > 0 ? (double)in->w_ad : 1.0;
     const bool restart = in->mode == IF_MODE_PHYSICS && (in->restart == 1 || !restart_dir.empty());
     if (restart) {
         if (restart_dir.empty()) restart_dir = res;

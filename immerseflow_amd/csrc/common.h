@@ -143,6 +143,7 @@ struct CaseDev {
     int ad_itermax, ppe_itermax, check_every;
     double dt, rdt, omega, ppe_tol, ad_tol, ubc, vbc;
     double mu;             // 1/Re
+    double ad_omega;       // w_AD relaxation of the viscous sweep (1: none)
     float omega_f;
     AxisD x, y;
     AxisF xf, yf;

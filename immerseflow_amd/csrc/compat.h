@@ -10,6 +10,7 @@ namespace iff {
 struct CompatCase {
     int nx = 0, ny = 0, nblk = 0;
     double dt = 0, re = 0;
+    double w_ad = 1.0;  // w_AD relaxation (1: none, as the reference runs)
     int ad_itermax = 0, ppe_itermax = 0;
     int cur = 0, pcur = 0;
     long long steps = 0;

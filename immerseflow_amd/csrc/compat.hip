@@ -121,11 +121,12 @@ __global__ void kc_source(int nx, int ny, const double *dx, const double *dy, do
 }
 
 // ADusolver_kernel + ADvsolver_kernel, ADSolver.cu:81-120: interior only;
-// coefficients per axis (calculateADCoefficients, :34-39).
+// coefficients per axis (calculateADCoefficients, :34-39).  w != 1: the
+// w_AD relaxation written and commented out at :95,:116.
 __global__ void kc_sweep(int nx, int ny, const double *xp, const double *xm, const double *px,
                          const double *yp, const double *ym, const double *py,
                          const double *iblank, const double *sx, const double *sy,
-                         const double *u, const double *v, double *un, double *vn)
+                         const double *u, const double *v, double *un, double *vn, double w)
 {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nx * ny) return;
@@ -133,45 +134,74 @@ __global__ void kc_sweep(int nx, int ny, const double *xp, const double *xm, con
     if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
         const double ap = px[i] + py[j];
         double t = sx[id] + xp[i] * u[id + 1] + xm[i] * u[id - 1] + yp[j] * u[id + nx] + ym[j] * u[id - nx];
-        un[id] = iblank[id] * t / ap;
+        double qu = iblank[id] * t / ap;
         t = sy[id] + xp[i] * v[id + 1] + xm[i] * v[id - 1] + yp[j] * v[id + nx] + ym[j] * v[id - nx];
-        vn[id] = iblank[id] * t / ap;
+        double qv = iblank[id] * t / ap;
+        if (w != 1.0) {
+            qu = (1.0 - w) * u[id] + w * qu;
+            qv = (1.0 - w) * v[id] + w * qv;
+        }
+        un[id] = qu;
+        vn[id] = qv;
     }
 }
 
-// Compute_{u,v}Residual_AD (ADSolver.cu:122-160): per-block partial sums of
-// |new - old| on interior cells with iBlank == 1, u and v separately.
+// Reduction, preSim.cu:376-445, in the reference's summation order: pass 1
+// is reduce6<256> (:22-50) over n values with CUDAQuery's grid of
+// ceil(n/256) blocks (:190-193): thread tid of block b starts at
+// i = b*512 + tid and adds val(i) + val(i+256) (val(i) alone when i+256 >= n)
+// per grid-stride step of grid_size = 512 * blocks; the block then folds its
+// 256 sums pairwise, s[t] += s[t+h] for h = 128 .. 1 (warpReduce's lockstep
+// reads leave the same s[0]).  Pass 2 is the same over the block sums with
+// one block (grid_size 512).
+template <typename F>
+__device__ __forceinline__ double reduce6_block(F val, unsigned n, unsigned grid_size)
+{
+    __shared__ double sd[256];
+    const unsigned tid = threadIdx.x;
+    double acc = 0;
+    for (unsigned i = blockIdx.x * 512 + tid; i < n; i += grid_size) {
+        if (i + 256 < n) acc += val(i) + val(i + 256);
+        else acc += val(i);
+    }
+    sd[tid] = acc;
+    __syncthreads();
+    for (unsigned h = 128; h >= 1; h >>= 1) {
+        if (tid < h) sd[tid] += sd[tid + h];
+        __syncthreads();
+    }
+    const double r = sd[0];
+    __syncthreads();
+    return r;
+}
+
+// Compute_{u,v}Residual_AD (ADSolver.cu:122-160): |new - old| on interior
+// cells with iBlank == 1, 0 elsewhere; pass 1 of Reduction for u and v.
 __global__ __launch_bounds__(256) void kc_l1(int nx, int ny, const double *iblank,
                                              const double *a, const double *b, const double *c,
                                              const double *d, double *part)
 {
-    const int id = blockIdx.x * 256 + threadIdx.x;
-    double su = 0.0, sv = 0.0;
-    if (id < nx * ny) {
-        const int i = id % nx, j = id / nx;
-        if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && iblank[id] == 1) {
-            su = fabs(a[id] - b[id]);
-            sv = fabs(c[id] - d[id]);
-        }
-    }
-    su = blk_sum(su);
-    sv = blk_sum(sv);
+    const unsigned n = (unsigned)(nx * ny), gs = 512u * gridDim.x;
+    auto val = [&](const double *x, const double *y) {
+        return [=](unsigned id) {
+            const int i = (int)id % nx, j = (int)id / nx;
+            return (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && iblank[id] == 1) ? fabs(x[id] - y[id]) : 0.0;
+        };
+    };
+    const double su = reduce6_block(val(a, b), n, gs);
+    const double sv = reduce6_block(val(c, d), n, gs);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = su;
         part[2 * blockIdx.x + 1] = sv;
     }
 }
 
-// Sum of the block partials (the second pass of Reduction, preSim.cu:411-438).
+// Pass 2 of Reduction over nb block sums (stride 2: u and v interleaved).
 __global__ __launch_bounds__(256) void kc_finish(const double *part, int nb, int stride, double *out)
 {
-    double s0 = 0.0, s1 = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 256) {
-        s0 += part[stride * k];
-        if (stride == 2) s1 += part[2 * k + 1];
-    }
-    s0 = blk_sum(s0);
-    s1 = blk_sum(s1);
+    const double s0 = reduce6_block([=](unsigned k) { return part[stride * k]; }, (unsigned)nb, 512u);
+    double s1 = 0.0;
+    if (stride == 2) s1 = reduce6_block([=](unsigned k) { return part[2 * k + 1]; }, (unsigned)nb, 512u);
     if (threadIdx.x == 0) {
         out[0] = s0;
         out[1] = s1;
@@ -192,23 +222,22 @@ __global__ void kc_jacobi(int nx, int ny, const double *cxp, const double *cxm, 
     }
 }
 
-// Compute_Residual, PPESolver.cu:33-50: signed A.p per interior cell,
-// reduced per block.
+// Compute_Residual, PPESolver.cu:33-50: signed A.p per interior cell (0 on
+// the ring), pass 1 of Reduction.
 __global__ __launch_bounds__(256) void kc_presid(int nx, int ny, const double *cxp,
                                                  const double *cxm, const double *cyp,
                                                  const double *cym, const double *p, double *part)
 {
-    const int id = blockIdx.x * 256 + threadIdx.x;
-    double r = 0.0;
-    if (id < nx * ny) {
-        const int i = id % nx, j = id / nx;
-        if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+    const unsigned n = (unsigned)(nx * ny), gs = 512u * gridDim.x;
+    const double r = reduce6_block(
+        [=](unsigned id) {
+            const int i = (int)id % nx, j = (int)id / nx;
+            if (!(i > 0 && i < nx - 1 && j > 0 && j < ny - 1)) return 0.0;
             const double ap = -1 * ((cxp[i] + cxm[i]) + (cyp[j] + cym[j]));
-            r = p[id] * ap + p[id + 1] * cxp[i] + p[id - 1] * cxm[i] + p[id + nx] * cyp[j]
-                + p[id - nx] * cym[j];
-        }
-    }
-    r = blk_sum(r);
+            return p[id] * ap + p[id + 1] * cxp[i] + p[id - 1] * cxm[i] + p[id + nx] * cyp[j] +
+                   p[id - nx] * cym[j];
+        },
+        n, gs);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
@@ -242,6 +271,7 @@ if_status CompatCase::create(const if_case_desc &d)
     re = d.re;
     ad_itermax = d.ad_itermax;
     ppe_itermax = d.ppe_itermax;
+    w_ad = d.ad_omega > 0 ? d.ad_omega : 1.0;
     xc.resize(nx);
     dx.resize(nx);
     yc.resize(ny);
@@ -329,7 +359,7 @@ if_status CompatCase::ad_step(if_step_stats *stt)
         bc(u[cur], v[cur]);                                             // :317
         velf(u[cur], v[cur]);                                           // :321
         hipLaunchKernelGGL(kc_sweep, dim3(nblk), dim3(256), 0, st, nx, ny, xp, xm, px, yp, ym, py,
-                           iblank, sx, sy, u[cur], v[cur], u[1 - cur], v[1 - cur]);  // :326,:331
+                           iblank, sx, sy, u[cur], v[cur], u[1 - cur], v[1 - cur], w_ad);  // :326,:331
         bc(u[cur], v[cur]);                                             // :337 (old buffer)
         cur = 1 - cur;                                                  // :342-348
         hipLaunchKernelGGL(kc_l1, dim3(nblk), dim3(256), 0, st, nx, ny, iblank, u[cur], u[1 - cur],

@@ -354,6 +354,7 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.ubc = d.ubc;
     c.vbc = d.vbc;
     c.mu = 1.0 / d.re;
+    c.ad_omega = d.ad_omega > 0 ? d.ad_omega : 1.0;
     c.mon = nullptr;
     for (int b = 0; b < 2; b++) {
         c.u[b] = A + a.off_u[b];
@@ -1041,6 +1042,10 @@ if_status Engine::autotune(if_tune_result *out)
         return hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(ms, e0, e1) == hipSuccess;
     };
     const Shape keep{sweep_kind, sweep_ty, sweep_m};
+    const bool keep_auto = shape_auto;
+    // each candidate is timed over its own tile table: refresh() must not
+    // replace the candidate's shape by the FP64 default while tuning
+    shape_auto = false;
     Shape best = keep;
     double best_ns = 1e300;
     int ncand = 0;
@@ -1051,6 +1056,19 @@ if_status Engine::autotune(if_tune_result *out)
         sweep_m = c.m;
         rc = refresh();
         if (rc != IF_OK) break;
+        // the launch below covers exactly the candidate's tiles
+        {
+            const int wo = c.kind == 1 ? stream_out_cols(c.m) : c.kind >= 2 ? reg_out_cols(c.m) : 0;
+            int want = 0;
+            for (int id : order_) {
+                const Slot &sl = *slots_[id];
+                want += (wo > 0 ? (sl.NX + wo - 1) / wo : sl.ntx) * ((sl.NY + c.ty - 1) / c.ty);
+            }
+            if (want != nt_sweep_ || tiles_sweep_ty_ != c.ty) {
+                rc = fail(IF_ERR_STATE, "autotune: tile table does not match the candidate shape");
+                break;
+            }
+        }
         // a shape that will run as persistent chunks is timed that way, over
         // 32 stages (a typical check interval); it writes only the scratch
         // error buffers and advances the stage counters of every tile alike.
@@ -1115,6 +1133,7 @@ if_status Engine::autotune(if_tune_result *out)
         sweep_kind = keep.kind;
         sweep_ty = keep.ty;
         sweep_m = keep.m;
+        shape_auto = keep_auto;
         mark_dirty();
         return rc;
     }
@@ -1369,6 +1388,31 @@ if_status Engine::monitor_read(int k, double *out, size_t cap, size_t *nrec, siz
     }
     const unsigned long long zero = 0;
     IFF_HIP(hipMemcpy(m.count, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return IF_OK;
+}
+
+// One monitor record of a case's current fields, at its current time,
+// without a step (the initial state, a restart, or fields set by the
+// caller); the case's own kernels run on a one-case table.
+if_status Engine::monitor_sample(int k)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "monitor_sample: empty slot");
+    if (!s->mon_mem) return fail(IF_ERR_STATE, "monitor_sample: no monitors on this case");
+    CaseDev *tmp = nullptr;
+    int2 *dtl = nullptr;
+    std::vector<int2> tl(s->ntx * s->nty);
+    for (int t = 0; t < (int)tl.size(); t++) tl[t] = make_int2(0, t);
+    bool ok = hipMalloc(&tmp, sizeof(CaseDev)) == hipSuccess && hipMalloc(&dtl, sizeof(int2) * tl.size()) == hipSuccess;
+    if (ok) {
+        ok = hipMemcpyAsync(tmp, &s->dev, sizeof(CaseDev), hipMemcpyHostToDevice, st_) == hipSuccess &&
+             hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * tl.size(), hipMemcpyHostToDevice, st_) == hipSuccess;
+        if (ok) launch_monitor(st_, vprec_, tmp, 1, dtl, (int)tl.size(), 1);
+        ok = ok && hipStreamSynchronize(st_) == hipSuccess;  // tl and the scratch live until here
+    }
+    if (tmp) (void)hipFree(tmp);
+    if (dtl) (void)hipFree(dtl);
+    if (!ok) return fail(IF_ERR_HIP, "monitor_sample: device update failed");
     return IF_OK;
 }
 

@@ -135,6 +135,8 @@ if_status parse_inputs(const char *path, if_inputs *out)
             std::string mode, prec;
             iss >> mode >> prec >> out->conv_factor >> out->ppe_omega >> out->ppe_tol
                 >> out->check_every;
+            double w = 0.0;  // optional real-valued w_AD
+            if (iss >> w) out->ad_omega = w;
             if (mode == "physics") out->mode = IF_MODE_PHYSICS;
             else if (mode == "compat") out->mode = IF_MODE_COMPAT;
             else return fail(IF_ERR_PARSE, "unknown mode '" + mode + "'");

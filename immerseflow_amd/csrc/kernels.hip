@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void k_ad_sweep(const CaseDev *cs, const int2 
     const TS *sx = fld<TS>(c.sx), *sy = fld<TS>(c.sy);
     const int P = c.P, nx = c.nx, ny = c.ny;
     const TS ub2 = (TS)c.ubc * (TS)2.0, vb2 = (TS)c.vbc * (TS)2.0;
+    const double w = c.ad_omega;
     const int i = 1 + tc.tx * kTileX + (threadIdx.x & 63);
     double acc = 0.0;
     if (i <= nx - 2) {
@@ -209,6 +210,10 @@ __global__ __launch_bounds__(256) void k_ad_sweep(const CaseDev *cs, const int2 
             } else {
                 qu = jacobi_cell<TS>(u, sx, id, P, c.xf, c.yf, i, j, blank);
                 qv = jacobi_cell<TS>(v, sy, id, P, c.xf, c.yf, i, j, blank);
+            }
+            if (w != 1.0) {  // w_AD relaxation (ADSolver.cu:95,116)
+                qu = (TS)(1.0 - w) * u[id] + (TS)w * qu;
+                qv = (TS)(1.0 - w) * v[id] + (TS)w * qv;
             }
             un[id] = qu;
             vn[id] = qv;
@@ -1353,15 +1358,17 @@ void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *til
 // body velocity that of the neighbouring body.  Per-tile sums, then a
 // per-case sum in tile order (deterministic).
 // ---------------------------------------------------------------------------
+// sample != 0: record the current fields of every listed case, running or
+// not, at the current time (if_case_monitor_sample).
 template <typename TS>
 __global__ __launch_bounds__(256) void k_monitor_part(const CaseDev *cs, const int2 *tiles, int ntiles,
-                                                      int ntiles_pad)
+                                                      int ntiles_pad, int sample)
 {
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
     const CaseDev &c = cs[tc.cid];
     const Ctrl *ct = c.ctrl;
-    if (!c.mon || !ct->active) return;
+    if (!c.mon || !(ct->active || sample)) return;
     const TS *u = fld<TS>(c.u[ct->ad_base]), *v = fld<TS>(c.v[ct->ad_base]);
     const double *p = c.p[ct->ppe_base];
     const int nx = c.nx, ny = c.ny, P = c.P;
@@ -1404,11 +1411,11 @@ __global__ __launch_bounds__(256) void k_monitor_part(const CaseDev *cs, const i
 }
 
 template <typename TS>
-__global__ void k_monitor_rec(const CaseDev *cs, int ncase)
+__global__ void k_monitor_rec(const CaseDev *cs, int ncase, int sample)
 {
     const CaseDev &c = cs[blockIdx.x];
     const Ctrl *ct = c.ctrl;
-    if (!c.mon || !ct->active || threadIdx.x != 0) return;
+    if (!c.mon || !(ct->active || sample) || threadIdx.x != 0) return;
     Monitor &m = *c.mon;
     const int nt = c.ntx * c.nty;
     double fx = 0.0, fy = 0.0;
@@ -1419,7 +1426,7 @@ __global__ void k_monitor_rec(const CaseDev *cs, int ncase)
     const int stride = 3 + 3 * m.nprobe;
     const unsigned long long n = *m.count;
     double *rec = m.ring + (size_t)(n % (unsigned long long)m.cap) * stride;
-    rec[0] = ct->t + c.dt;  // the time this step ends at
+    rec[0] = sample ? ct->t : ct->t + c.dt;  // the time this step ends at (sample: now)
     rec[1] = fx;
     rec[2] = fy;
     const TS *u = fld<TS>(c.u[ct->ad_base]), *v = fld<TS>(c.v[ct->ad_base]);
@@ -1433,14 +1440,17 @@ __global__ void k_monitor_rec(const CaseDev *cs, int ncase)
     *m.count = n + 1;
 }
 
-void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt)
+void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt,
+                    int sample)
 {
     if (prec == IF_FP64) {
-        hipLaunchKernelGGL(k_monitor_part<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
-        hipLaunchKernelGGL(k_monitor_rec<double>, dim3(ncase), dim3(64), 0, st, cs, ncase);
+        hipLaunchKernelGGL(k_monitor_part<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt),
+                           sample);
+        hipLaunchKernelGGL(k_monitor_rec<double>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample);
     } else {
-        hipLaunchKernelGGL(k_monitor_part<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt));
-        hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase);
+        hipLaunchKernelGGL(k_monitor_part<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt),
+                           sample);
+        hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample);
     }
 }
 

@@ -96,6 +96,9 @@ typedef struct {
     double ppe_omega;       /* SOR relaxation; <= 0: from the grid */
     double ppe_tol;         /* dt*||b - A p||_inf stop level */
     int check_every;        /* sweeps per Poisson check */
+    double ad_omega;        /* real-valued w_AD (optional 7th value of the
+                               "Mode" line); 0 when absent: the CLI then
+                               uses the deck's integer w_ad */
     int nbody;
     if_body bodies[IF_MAX_BODIES];
     double poly_store[IF_MAX_BODIES][2][512]; /* storage for parsed polygons */
@@ -121,6 +124,10 @@ typedef struct {
     double t0;              /* start time, for moving bodies */
     int nbody;
     const if_body *bodies;
+    double ad_omega;        /* w_AD: relaxation of the viscous Jacobi sweep,
+                               q' = (1 - w) q + w q_J, as the reference writes
+                               it and comments it out (ADSolver.cu:95,116);
+                               0 means 1 (no relaxation) */
 } if_case_desc;
 
 typedef struct {
@@ -273,6 +280,11 @@ if_status if_case_monitor(if_case *c, int nprobe, const double *xy, int capacity
  * out may be NULL to query *nrec only (the ring is then kept). */
 if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size_t *nrec,
                                size_t *dropped);
+/* One monitor record of the current fields at the case's current time,
+ * without a step: the initial state, the state after a restart, or fields
+ * loaded with if_case_set_field (the predecessor writes its probes from
+ * the current state too, test/UTIL_PRE_SIM.f90:172-202). */
+if_status if_case_monitor_sample(if_case *c);
 /* Autotuner (no counterpart in the reference, whose loop is fixed):
  * times every Poisson sweep launch shape of if_case_tune on the case's
  * current fields (a few launches each, on the solver's scratch buffers;

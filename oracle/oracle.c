@@ -164,9 +164,11 @@ static void source_cells(int nx, int ny, const double *dx, const double *dy,
 }
 
 /* ADusolver_kernel / ADvsolver_kernel, ADSolver.cu:81-120: one Jacobi sweep
- * on interior cells, masked by iBlank; ghost cells of qn are not written. */
+ * on interior cells, masked by iBlank; ghost cells of qn are not written.
+ * w != 1: the relaxation the reference comments out at :95,:116,
+ * qn = (1.0 - w) * q + w * qn, applied after the masked update. */
 static void ad_sweep(int nx, int ny, const ad_coef *c, const double *iblank,
-                     const double *s, const double *q, double *qn)
+                     const double *s, const double *q, double *qn, double w)
 {
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < ny - 1; j++)
@@ -174,7 +176,9 @@ static void ad_sweep(int nx, int ny, const ad_coef *c, const double *iblank,
             const int id = IDX(i, j, nx);
             double t = s[id] + c->xp[i] * q[id + 1] + c->xm[i] * q[id - 1]
                              + c->yp[j] * q[id + nx] + c->ym[j] * q[id - nx];
-            qn[id] = iblank[id] * t / (c->px[i] + c->py[j]);
+            double r = iblank[id] * t / (c->px[i] + c->py[j]);
+            if (w != 1.0) r = (1.0 - w) * q[id] + w * r;
+            qn[id] = r;
         }
 }
 
@@ -200,11 +204,68 @@ static double l1_change(int nx, int ny, const double *iblank, const double *a,
     return s;
 }
 
+/* ImmerseFlow::Reduction, preSim.cu:376-445: two launches of reduce6<256>
+ * (:22-50), the first with CUDAQuery's blocksPerGrid = ceil(n/256) blocks
+ * (:190-193) over the n values, the second with one block over the block
+ * sums.  The summation order is restated exactly: thread tid of block b
+ * starts at i = b*512 + tid and adds g[i] + g[i+256] (or g[i] alone when
+ * i+256 >= n) per grid-stride step; the block then folds its 256 sums
+ * pairwise, s[t] += s[t+h] for h = 128, 64, ..., 1 (warpReduce's lockstep
+ * reads give the same s[0]). */
+static double reduce6_block(const double *g, unsigned n, unsigned b, unsigned grid_size)
+{
+    double sd[256];
+    for (unsigned tid = 0; tid < 256; tid++) {
+        double acc = 0;
+        for (unsigned i = b * 512 + tid; i < n; i += grid_size) {
+            if (i + 256 < n) acc += g[i] + g[i + 256];
+            else acc += g[i];
+        }
+        sd[tid] = acc;
+    }
+    for (unsigned h = 128; h >= 1; h >>= 1)
+        for (unsigned tid = 0; tid < h; tid++) sd[tid] += sd[tid + h];
+    return sd[0];
+}
+
+static double ref_reduction(const double *g, unsigned n)
+{
+    const unsigned nb = (n + 255) / 256;
+    double *part = malloc(sizeof(double) * nb);
+    for (unsigned b = 0; b < nb; b++) part[b] = reduce6_block(g, n, b, 512 * nb);
+    const double s = reduce6_block(part, nb, 0, 512);
+    free(part);
+    return s;
+}
+
+/* Compute_{u,v}Residual_AD (ADSolver.cu:122-160): |a - b| on interior cells
+ * with iBlank == 1, 0 elsewhere, summed by Reduction. */
+static double compat_l1(int nx, int ny, const double *iblank, const double *a,
+                        const double *b, double *tmp)
+{
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            const int id = IDX(i, j, nx);
+            tmp[id] = (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && iblank[id] == 1)
+                          ? fabs(a[id] - b[id]) : 0.0;
+        }
+    return ref_reduction(tmp, (unsigned)(nx * ny));
+}
+
 /* ImmerseFlow::ADsolver, ADSolver.cu:268-395, as coded. */
 int of_compat_ad_step(int nx, int ny, const double *dx, const double *dy,
                       double dt, double re, int ad_itermax,
                       const double *iblank, double *u, double *v,
                       double *uf, double *vf, double *res_out)
+{
+    return of_compat_ad_step_w(nx, ny, dx, dy, dt, re, ad_itermax, 1.0, iblank, u, v, uf, vf, res_out);
+}
+
+/* The same with the w_AD relaxation of ADSolver.cu:95,116. */
+int of_compat_ad_step_w(int nx, int ny, const double *dx, const double *dy,
+                        double dt, double re, int ad_itermax, double w_ad,
+                        const double *iblank, double *u, double *v,
+                        double *uf, double *vf, double *res_out)
 {
     const size_t n = (size_t)nx * ny;
     ad_coef c;
@@ -215,6 +276,7 @@ int of_compat_ad_step(int nx, int ny, const double *dx, const double *dy,
     double *bu[2] = {malloc(n * sizeof(double)), calloc(n, sizeof(double))};
     double *bv[2] = {malloc(n * sizeof(double)), calloc(n, sizeof(double))};
     double *sx = calloc(n, sizeof(double)), *sy = calloc(n, sizeof(double));
+    double *rtmp = malloc(n * sizeof(double));
     memcpy(bu[0], u, n * sizeof(double));
     memcpy(bv[0], v, n * sizeof(double));
     int cur = 0;
@@ -231,20 +293,20 @@ int of_compat_ad_step(int nx, int ny, const double *dx, const double *dy,
         compat_bc(nx, ny, bu[cur], 1.0);                          /* :317 */
         compat_bc(nx, ny, bv[cur], 0.0);
         compat_velf(nx, ny, dx, dy, bu[cur], bv[cur], uf, vf);   /* :321 */
-        ad_sweep(nx, ny, &c, iblank, sx, bu[cur], bu[1 - cur]);   /* :326 */
-        ad_sweep(nx, ny, &c, iblank, sy, bv[cur], bv[1 - cur]);   /* :331 */
+        ad_sweep(nx, ny, &c, iblank, sx, bu[cur], bu[1 - cur], w_ad); /* :326 */
+        ad_sweep(nx, ny, &c, iblank, sy, bv[cur], bv[1 - cur], w_ad); /* :331 */
         compat_bc(nx, ny, bu[cur], 1.0);                          /* :337 old buffer */
         compat_bc(nx, ny, bv[cur], 0.0);
         cur = 1 - cur;                                            /* :342-348 swap */
-        ures = l1_change(nx, ny, iblank, bu[cur], bu[1 - cur]);   /* :350-366 */
-        vres = l1_change(nx, ny, iblank, bv[cur], bv[1 - cur]);
+        ures = compat_l1(nx, ny, iblank, bu[cur], bu[1 - cur], rtmp); /* :350-366 */
+        vres = compat_l1(nx, ny, iblank, bv[cur], bv[1 - cur], rtmp);
         iter += 1;
     }
     memcpy(u, bu[cur], n * sizeof(double));
     memcpy(v, bv[cur], n * sizeof(double));
     if (res_out) *res_out = ures + vres;
     free(bu[0]); free(bu[1]); free(bv[0]); free(bv[1]);
-    free(sx); free(sy);
+    free(sx); free(sy); free(rtmp);
     ad_coef_free(&c);
     return iter;
 }
@@ -281,7 +343,7 @@ int of_compat_ppe(int nx, int ny, const double *dx, const double *dy,
     ppe_axis(nx, dx, cxp, cxm);
     ppe_axis(ny, dy, cyp, cym);
     double *pt = calloc(n, sizeof(double)); /* managed memory reads as zero */
-    double *row = calloc(ny, sizeof(double));
+    double *res_cell = calloc(n, sizeof(double));
     double *pc = p;
     compat_pbc(nx, ny, pc);
     double residual = 1.0;
@@ -300,27 +362,28 @@ int of_compat_ppe(int nx, int ny, const double *dx, const double *dy,
                 }
             }
         double *t = pc; pc = pt; pt = t; /* :178-180 */
-        /* Compute_Residual (signed A.p, no RHS) + Reduction, :33-50,:182 */
+        /* Compute_Residual (signed A.p, no RHS; the boundary entries of the
+         * managed array stay zero), :33-50, summed by Reduction, :182 */
 #pragma omp parallel for schedule(static)
-        for (int j = 1; j < ny - 1; j++) {
-            double acc = 0.0;
-            for (int i = 1; i < nx - 1; i++) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
                 const int id = IDX(i, j, nx);
-                const double ap = -1 * ((cxp[i] + cxm[i]) + (cyp[j] + cym[j]));
-                acc += pc[id] * ap + pc[id + 1] * cxp[i] + pc[id - 1] * cxm[i]
-                       + pc[id + nx] * cyp[j] + pc[id - nx] * cym[j];
+                double r = 0.0;
+                if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+                    const double ap = -1 * ((cxp[i] + cxm[i]) + (cyp[j] + cym[j]));
+                    r = pc[id] * ap + pc[id + 1] * cxp[i] + pc[id - 1] * cxm[i]
+                        + pc[id + nx] * cyp[j] + pc[id - nx] * cym[j];
+                }
+                res_cell[id] = r;
             }
-            row[j] = acc;
-        }
-        residual = 0.0;
-        for (int j = 1; j < ny - 1; j++) residual += row[j];
+        residual = ref_reduction(res_cell, (unsigned)n);
         iter += 1;
     }
     compat_pbc(nx, ny, pc); /* :195 */
     if (pc != p) memcpy(p, pc, n * sizeof(double));
     if (res_out) *res_out = residual;
     free(pc == p ? pt : pc);
-    free(cxp); free(cxm); free(cyp); free(cym); free(row);
+    free(cxp); free(cxm); free(cyp); free(cym); free(res_cell);
     return iter;
 }
 
@@ -983,6 +1046,17 @@ void of_phys_init(int nx, int ny, const double *dx, const double *dy,
     faces_from_cells(nx, ny, dx, dy, flags, bid, bodies, prm, u, v, uf, vf);
 }
 
+void of_phys_restart(int nx, int ny, const double *dx, const double *dy,
+                     const uint8_t *flags, const int8_t *bid, int nbody,
+                     const of_body *bodies, int ng, const of_ghost *gh,
+                     const of_params *prm, double *u, double *v, double *uf,
+                     double *vf, double *p)
+{
+    (void)p;
+    of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, u, v);
+    faces_from_cells(nx, ny, dx, dy, flags, bid, bodies, prm, u, v, uf, vf);
+}
+
 void of_phys_step(int nx, int ny, const double *dx, const double *dy,
                   const uint8_t *flags, const int8_t *bid, int nbody,
                   const of_body *bodies, int ng, const of_ghost *gh,
@@ -1009,9 +1083,10 @@ void of_phys_step(int nx, int ny, const double *dx, const double *dy,
     double res = 1.0 + 1.0;
     int iter = 0;
     double *uc = u, *vc = v, *uo = un, *vo = vn;
+    const double w_ad = prm->ad_omega > 0.0 ? prm->ad_omega : 1.0;
     while (res > prm->ad_tol && iter < prm->ad_itermax) {
-        ad_sweep(nx, ny, &ac, iblank, sx, uc, uo);
-        ad_sweep(nx, ny, &ac, iblank, sy, vc, vo);
+        ad_sweep(nx, ny, &ac, iblank, sx, uc, uo, w_ad);
+        ad_sweep(nx, ny, &ac, iblank, sy, vc, vo, w_ad);
         domain_bc_vel(nx, ny, uo, prm->ubc);
         domain_bc_vel(nx, ny, vo, prm->vbc);
         of_phys_ghost_velocity(nx, ny, ng, gh, nbody, bodies, uo, vo);

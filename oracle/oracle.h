@@ -57,6 +57,14 @@ int of_compat_ad_step(int nx, int ny, const double *dx, const double *dy,
                       const double *iblank, double *u, double *v,
                       double *uf, double *vf, double *res_out);
 
+/* The same with the w_AD relaxation the reference writes and comments out
+ * (ADSolver.cu:95,116): unew = (1.0 - w_AD) * u + w_AD * unew after each
+ * Jacobi update.  w_ad = 1 is of_compat_ad_step. */
+int of_compat_ad_step_w(int nx, int ny, const double *dx, const double *dy,
+                        double dt, double re, int ad_itermax, double w_ad,
+                        const double *iblank, double *u, double *v,
+                        double *uf, double *vf, double *res_out);
+
 /* ImmerseFlow::PPESolver as intended (PPESolver.cu:13-71,137-205):
  * set_pressure_BC, Jacobi sweeps until the signed residual sum <= 1e-6 or
  * itermax, final set_pressure_BC.  Returns sweeps executed. */
@@ -137,6 +145,9 @@ typedef struct {
     int ppe_itermax;
     double ppe_tol;       /* stop when dt*||b - A p||_inf <= ppe_tol */
     int check_every;      /* sweeps between Poisson residual checks */
+    double ad_omega;      /* w_AD relaxation of the viscous sweep,
+                             q' = (1 - w) q + w q_Jacobi (ADSolver.cu:95,116);
+                             0 (unset) means w = 1, no relaxation */
 } of_params;
 
 typedef struct {
@@ -156,6 +167,17 @@ void of_phys_init(int nx, int ny, const double *dx, const double *dy,
                   const of_body *bodies, int ng, const of_ghost *gh,
                   const of_params *prm, double *u, double *v, double *uf,
                   double *vf, double *p);
+
+/* Restart from u, v, p (ghost ring included, e.g. read from the writer's
+ * files): ghost-cell velocities refreshed from the image points, then the
+ * faces interpolated from the cells (blanked faces at the body velocity,
+ * domain faces at the BC), as at initialisation; the predecessor's restart
+ * rebuilds its faces from u, v in the same way (test/UTIL_PRE_SIM.f90:130-164). */
+void of_phys_restart(int nx, int ny, const double *dx, const double *dy,
+                     const uint8_t *flags, const int8_t *body_id, int nbody,
+                     const of_body *bodies, int ng, const of_ghost *gh,
+                     const of_params *prm, double *u, double *v, double *uf,
+                     double *vf, double *p);
 
 /* One fractional step. */
 void of_phys_step(int nx, int ny, const double *dx, const double *dy,

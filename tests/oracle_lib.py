@@ -48,7 +48,8 @@ class Params(C.Structure):
                 ("vbc", C.c_double), ("ad_itermax", C.c_int),
                 ("ad_tol", C.c_double), ("ppe_solver", C.c_int),
                 ("ppe_omega", C.c_double), ("ppe_itermax", C.c_int),
-                ("ppe_tol", C.c_double), ("check_every", C.c_int)]
+                ("ppe_tol", C.c_double), ("check_every", C.c_int),
+                ("ad_omega", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -75,6 +76,10 @@ def lib():
                                     C.c_double, C.c_int, _dp, _dp, _dp, _dp,
                                     _dp, _dp]
     L.of_compat_ad_step.restype = C.c_int
+    L.of_compat_ad_step_w.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, _dp, _dp, _dp,
+                                      _dp, _dp, _dp]
+    L.of_compat_ad_step_w.restype = C.c_int
     L.of_compat_ppe.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
     L.of_compat_ppe.restype = C.c_int
     L.of_classify.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int,
@@ -93,6 +98,7 @@ def lib():
                                       _u8p, _i8p, _dp, _dp]
     L.of_phys_move_bodies.restype = C.c_int
     L.of_phys_init.argtypes = phys_sig
+    L.of_phys_restart.argtypes = phys_sig
     L.of_phys_step.argtypes = phys_sig + [C.POINTER(Stats)]
     L.of_phys_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _u8p, C.c_double,
                               _dp, _dp, _dp]
@@ -243,6 +249,16 @@ class PhysCase:
         self.glist, self.gh, self.ng = ghost_records(self.xc, self.yc, self.flags, self.bid,
                                                      self.bodies)
         return fresh
+
+    def restart(self, u, v, p):
+        """Load u, v, p and rebuild ghost velocities and faces (of_phys_restart)."""
+        self.u[:] = u
+        self.v[:] = v
+        self.p[:] = p
+        lib().of_phys_restart(self.nx, self.ny, dptr(self.dx), dptr(self.dy), u8ptr(self.flags),
+                              i8ptr(self.bid), self.bodies.n, self.bodies.arr, self.ng, self.gh,
+                              C.byref(self.prm), dptr(self.u), dptr(self.v), dptr(self.uf),
+                              dptr(self.vf), dptr(self.p))
 
     def step(self, n=1):
         st = Stats()

@@ -65,6 +65,52 @@ def test_restart_from_reference_outputs(require_gpu, tmp_path):
         x.close()
 
 
+def _read_f(path, n):
+    """Values of a file in the writer's format, parsed here independently of
+    the library's reader (header of 3 lines, then 'x,y,value' rows)."""
+    v = np.loadtxt(path, skiprows=3, delimiter=",", usecols=2)
+    assert v.size == n
+    return np.ascontiguousarray(v)
+
+
+@pytest.mark.parametrize("name", ["ellipse", "circle-stretched"])
+def test_restart_from_files_matches_oracle(require_gpu, name, tmp_path):
+    """The GPU and the oracle restart from the same %f files (u, v, p; faces
+    interpolated) and then agree step for step: sweep counts identical,
+    fields to 1e-10 (the restart path of the deck's Restart flag,
+    src/main.cu:27-30; the predecessor's, test/UTIL_PRE_SIM.f90:130-164)."""
+    import oracle_lib as O
+    print("[gpu-test] restart from %f files vs oracle", name, flush=True)
+    xf, yf, bodies = CASES[name]
+    prm = dict(dt=0.01, re=100.0, conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6, ppe_solver=1,
+               ppe_omega=1.8, ppe_itermax=200000, ppe_tol=1e-9, check_every=10)
+    a = iff.Case(xf, yf, bodies=bodies, **prm)
+    a.step(3)
+    for fname, field in (("uc.dat", "u"), ("vc.dat", "v"), ("p.dat", "p")):
+        iff.write_tecplot(str(tmp_path / fname), a.xc, a.yc, a.get(field))
+    a.close()
+    n = len(a.xc) * len(a.yc)
+    u, v, p = (_read_f(str(tmp_path / f), n) for f in ("uc.dat", "vc.dat", "p.dat"))
+    g = iff.Case(xf, yf, bodies=bodies, **prm)
+    g.restart_from(str(tmp_path))
+    ob = []
+    for b in bodies:
+        d = dict(b)
+        d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
+        ob.append(d)
+    o = O.PhysCase(xf, yf, ob, prm)
+    o.restart(u, v, p)
+    for f, ref in (("u", o.u), ("v", o.v), ("p", o.p), ("uf", o.uf), ("vf", o.vf)):
+        assert rel(g.get(f), ref) <= 1e-13, f
+    for k in range(3):
+        sg = g.step(1)
+        so = o.step(1)[-1]
+        assert (sg["ad_iters"], sg["ppe_sweeps"]) == (so[0], so[2]), k
+    for f, ref in (("u", o.u), ("v", o.v), ("p", o.p), ("uf", o.uf), ("vf", o.vf)):
+        assert rel(g.get(f), ref) <= 1e-10, f
+    g.close()
+
+
 # ---------------------------------------------------------------------------
 # monitors: probes and body force
 # ---------------------------------------------------------------------------

@@ -1,0 +1,125 @@
+"""GPU parity of the reference's remaining code paths: the compat mode's
+vf quirk on a grid with nx > ny, the compat Poisson sweep count, and the
+w_AD relaxation of the viscous sweep in both modes.
+
+Tolerances as in test_gpu_parity: counts identical; FP64 fields to 1e-13
+(compat: the initial condition uses the device's exp/pow) or 1e-10
+(physics)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from conftest import GOLDEN
+
+import immerseflow_amd as iff
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def rel(a, b):
+    s = max(1.0, float(np.max(np.abs(b))))
+    return float(np.max(np.abs(a - b))) / s
+
+
+def _compat_oracle(xf, yf, steps, w=1.0, dt=0.001, re=150.0, itermax=25):
+    xc, dx = O.metric(xf)
+    yc, dy = O.metric(yf)
+    nx, ny = len(xc), len(yc)
+    u, v, p = np.zeros(nx * ny), np.zeros(nx * ny), np.zeros(nx * ny)
+    uf, vf = np.zeros((nx - 1) * (ny - 2)), np.zeros((nx - 2) * (ny - 1))
+    L = O.lib()
+    L.of_compat_init(nx, ny, O.dptr(xc), O.dptr(yc), O.dptr(u), O.dptr(v), O.dptr(uf), O.dptr(vf), O.dptr(p))
+    ib = np.ones(nx * ny)
+    iters = []
+    for _ in range(steps):
+        r = C.c_double()
+        iters.append(L.of_compat_ad_step_w(nx, ny, O.dptr(dx), O.dptr(dy), dt, re, itermax, w, O.dptr(ib),
+                                           O.dptr(u), O.dptr(v), O.dptr(uf), O.dptr(vf), C.byref(r)))
+    return dict(u=u, v=v, uf=uf, vf=vf, iters=iters, nx=nx, ny=ny)
+
+
+def test_compat_vf_quirk_on_wide_grid(require_gpu):
+    """Compute_velf's vf loop (ADSolver.cu:181-188) runs only for ids in
+    [(nx-1)(ny-2), (nx-2)(ny-1)), which is non-empty when nx > ny: a 60 x 40
+    grid (nx = 62, ny = 42) sets exactly those 20 north faces."""
+    _say("compat ADsolver 60x40 (nx > ny) x 8 steps")
+    xf = np.linspace(0.0, 1.2, 61)
+    yf = np.linspace(0.0, 0.8, 41)
+    c = iff.Case(xf, yf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ad_itermax=25)
+    iters = [c.ad_step()["ad_iters"] for _ in range(8)]
+    o = _compat_oracle(xf, yf, 8)
+    assert iters == o["iters"]
+    for name in ("u", "v", "uf", "vf"):
+        assert rel(c.get(name), o[name]) < 1e-13, name
+    nx, ny = o["nx"], o["ny"]
+    lo, hi = (nx - 1) * (ny - 2), (nx - 2) * (ny - 1)
+    assert hi - lo == 20
+    vf = c.get("vf")
+    assert np.count_nonzero(vf[lo:hi]) == hi - lo
+    assert np.count_nonzero(vf[:lo]) == 0 and np.count_nonzero(vf[hi:]) == 0
+    c.close()
+
+
+def test_compat_ppe_sweep_count_equals_oracle(require_gpu):
+    """PPESolver on the fixture grid: the device sums the signed residual in
+    the order of the reference's Reduction, so its stop decision -- and the
+    sweep count -- equals the oracle's exactly; the pressure too."""
+    _say("compat PPESolver 52x52, sweep count")
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat2"))
+    c = iff.Case(xf, xf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ppe_itermax=100000)
+    st = c.ppe_solve()
+    xc, dx = O.metric(xf)
+    n = len(xc)
+    p = np.zeros(n * n)
+    r = C.c_double()
+    sweeps = O.lib().of_compat_ppe(n, n, O.dptr(dx), O.dptr(dx), 100000, O.dptr(p), C.byref(r))
+    assert st["ppe_sweeps"] == sweeps == 14807
+    assert st["ppe_res"] == r.value
+    assert rel(c.get("p"), p) < 1e-13
+    c.close()
+
+
+@pytest.mark.parametrize("w", [0.7, 1.25])
+def test_compat_w_ad_relaxation(require_gpu, w):
+    """The relaxation the reference writes and comments out
+    (ADSolver.cu:95,116), applied in compat mode: GPU = oracle, and w != 1
+    changes the result."""
+    _say("compat ADsolver 52x52 w_AD", w)
+    xf = O.read_grid_file(os.path.join(GOLDEN, "xgrid.dat2"))
+    c = iff.Case(xf, xf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ad_itermax=25, ad_omega=w)
+    iters = [c.ad_step()["ad_iters"] for _ in range(6)]
+    o = _compat_oracle(xf, xf, 6, w=w)
+    o1 = _compat_oracle(xf, xf, 6, w=1.0)
+    assert iters == o["iters"]
+    for name in ("u", "v", "uf"):
+        assert rel(c.get(name), o[name]) < 1e-13, name
+    assert rel(o["u"], o1["u"]) > 1e-8
+    c.close()
+
+
+@pytest.mark.parametrize("w", [0.8, 1.2])
+def test_physics_w_ad_relaxation_matches_oracle(require_gpu, w):
+    """Physics mode, FP64: the relaxed viscous sweep q' = (1 - w) q + w q_J
+    equals the oracle's sweep for sweep (AD iteration counts identical)."""
+    _say("physics FP64 w_AD", w)
+    xf = np.linspace(0.0, 4.0, 81)
+    yf = np.linspace(0.0, 3.0, 61)
+    body = iff.circle(1.5, 1.4, 0.4)
+    prm = dict(dt=0.01, re=100.0, conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6,
+               ppe_solver=1, ppe_omega=1.8, ppe_itermax=200000, ppe_tol=1e-9, check_every=10)
+    oc = O.PhysCase(xf, yf, [dict(body)], dict(prm, ad_omega=w))
+    gc = iff.Case(xf, yf, bodies=[body], ad_omega=w, **prm)
+    for k in range(3):
+        g = gc.step(1)
+        o = oc.step(1)[-1]
+        assert g["ad_iters"] == o[0] and g["ppe_sweeps"] == o[2], k
+    for name, ref in (("u", oc.u), ("v", oc.v), ("p", oc.p), ("uf", oc.uf), ("vf", oc.vf)):
+        assert rel(gc.get(name), ref) <= 1e-10, name
+    gc.close()

@@ -82,7 +82,10 @@ def test_compat_ppe_reproduces_p():
     p = np.zeros(nx * ny)
     r = C.c_double()
     sweeps = L.of_compat_ppe(nx, ny, O.dptr(dx), O.dptr(dy), 100000, O.dptr(p), C.byref(r))
-    assert 14000 < sweeps < 15500 and r.value <= 1e-6
+    # the signed residual is summed in the order of the reference's Reduction
+    # (reduce6<256>, preSim.cu:22-50,376-445); the fixture itself is matched
+    # from sweep 12,606 on, so it does not fix the count
+    assert sweeps == 14807 and r.value <= 1e-6
     assert O.format_tecplot(nx, ny, xc, yc, p) == _read("p.dat")
 
 
