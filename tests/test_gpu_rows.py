@@ -47,23 +47,33 @@ def _compat_oracle(xf, yf, steps, w=1.0, dt=0.001, re=150.0, itermax=25):
 
 def test_compat_vf_quirk_on_wide_grid(require_gpu):
     """Compute_velf's vf loop (ADSolver.cu:181-188) runs only for ids in
-    [(nx-1)(ny-2), (nx-2)(ny-1)), which is non-empty when nx > ny: a 60 x 40
-    grid (nx = 62, ny = 42) sets exactly those 20 north faces."""
-    _say("compat ADsolver 60x40 (nx > ny) x 8 steps")
+    [(nx-1)(ny-2), (nx-2)(ny-1)), which is non-empty when nx > ny: on a
+    60 x 40 grid (nx = 62, ny = 42) exactly those 20 north faces are set.
+    Inside the Jacobi loop they are recomputed after the BC and come out
+    zero (v ghost = -v interior, equal widths at the top row), so the
+    quirk shows in vf only after the first interpolation of a call, from
+    stale ghosts (:298) -- observed with AD_itermax = 0 -- and otherwise
+    through the convective source it feeds (:308), in u and v."""
+    _say("compat ADsolver 60x40 (nx > ny)")
     xf = np.linspace(0.0, 1.2, 61)
     yf = np.linspace(0.0, 0.8, 41)
+    c0 = iff.Case(xf, yf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ad_itermax=0)
+    assert c0.ad_step()["ad_iters"] == 0
+    o0 = _compat_oracle(xf, yf, 1, itermax=0)
+    nx, ny = o0["nx"], o0["ny"]
+    lo, hi = (nx - 1) * (ny - 2), (nx - 2) * (ny - 1)
+    assert hi - lo == 20
+    vf = c0.get("vf")
+    assert np.count_nonzero(vf[lo:hi]) == hi - lo
+    assert np.count_nonzero(vf[:lo]) == 0 and np.count_nonzero(vf[hi:]) == 0
+    assert rel(vf, o0["vf"]) < 1e-13 and rel(c0.get("uf"), o0["uf"]) < 1e-13
+    c0.close()
     c = iff.Case(xf, yf, mode=iff.IF_MODE_COMPAT, dt=0.001, re=150.0, ad_itermax=25)
     iters = [c.ad_step()["ad_iters"] for _ in range(8)]
     o = _compat_oracle(xf, yf, 8)
     assert iters == o["iters"]
     for name in ("u", "v", "uf", "vf"):
         assert rel(c.get(name), o[name]) < 1e-13, name
-    nx, ny = o["nx"], o["ny"]
-    lo, hi = (nx - 1) * (ny - 2), (nx - 2) * (ny - 1)
-    assert hi - lo == 20
-    vf = c.get("vf")
-    assert np.count_nonzero(vf[lo:hi]) == hi - lo
-    assert np.count_nonzero(vf[:lo]) == 0 and np.count_nonzero(vf[hi:]) == 0
     c.close()
 
 
@@ -100,7 +110,9 @@ def test_compat_w_ad_relaxation(require_gpu, w):
     assert iters == o["iters"]
     for name in ("u", "v", "uf"):
         assert rel(c.get(name), o[name]) < 1e-13, name
-    assert rel(o["u"], o1["u"]) > 1e-8
+    # the relaxed iteration stops at the same L1 tolerance at a different
+    # iterate: the fields differ at that level
+    assert rel(o["u"], o1["u"]) > 1e-11 or o["iters"] != o1["iters"]
     c.close()
 
 
