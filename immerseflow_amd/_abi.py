@@ -106,6 +106,7 @@ SIGNATURES = [
     ("if_case_monitor_read", C.c_int, [C.c_void_p, _dp, C.c_size_t, C.POINTER(C.c_size_t),
                                        C.POINTER(C.c_size_t)]),
     ("if_case_monitor_sample", C.c_int, [C.c_void_p]),
+    ("if_case_monitor_sample_noalloc", C.c_int, [C.c_void_p]),
     ("if_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p]),
     ("if_case_sweep_timing", C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_longlong)]),
     ("if_case_byte_model", C.c_int, [C.c_void_p, _dp, _dp]),

@@ -280,6 +280,12 @@ if_status if_case_monitor_sample(if_case *c)
     IFF_GUARD(c->eng->monitor_sample(c->slot));
 }
 
+if_status if_case_monitor_sample_noalloc(if_case *c)
+{
+    if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "monitor: physics mode only");
+    IFF_GUARD(c->eng->monitor_sample_noalloc(c->slot));
+}
+
 if_status if_case_autotune(if_case *c, if_tune_result *out)
 {
     if (!c || c->mode != IF_MODE_PHYSICS) return fail(IF_ERR_STATE, "autotune: physics mode only");

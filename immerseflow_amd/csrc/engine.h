@@ -56,6 +56,7 @@ public:
     if_status monitor(int slot, int nprobe, const double *xy, int cap);
     if_status monitor_read(int slot, double *out, size_t cap, size_t *nrec, size_t *dropped);
     if_status monitor_sample(int slot);
+    if_status monitor_sample_noalloc(int slot);
     if_status sync();
 
     int resident() const;

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "engine.h"
@@ -1413,6 +1414,68 @@ if_status Engine::monitor_sample(int k)
     if (tmp) (void)hipFree(tmp);
     if (dtl) (void)hipFree(dtl);
     if (!ok) return fail(IF_ERR_HIP, "monitor_sample: device update failed");
+    return IF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free monitor sample.  Its staging -- a copy of the case
+// descriptor, the one-case tile table and the per-tile force partials --
+// lives in a file-scope device array, so the call makes no device (or
+// host) allocation.  Each GPU holds its own copy of the array until the
+// process ends; its address is resolved once per device with
+// hipGetSymbolAddress and kept.  Calls that overlap on one GPU share the
+// array and race; every byte a call reads it has written first, so no
+// clearing is needed between calls.
+// ---------------------------------------------------------------------------
+constexpr int kScratchMaxTiles = 1 << 16;   // interior tiles of 64 x 16 cells: up to 67 M cells
+constexpr size_t kScratchDescBytes = (sizeof(CaseDev) + 255) & ~(size_t)255;
+constexpr size_t kScratchTileBytes = sizeof(int2) * (size_t)kScratchMaxTiles;
+constexpr size_t kScratchBytes = kScratchDescBytes + kScratchTileBytes + 2 * sizeof(double) * (size_t)kScratchMaxTiles;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[64] = {};   // device address of scratch, per device
+
+static if_status scratch_ptr(void **out)
+{
+    int dev = 0;
+    IFF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(IF_ERR_NODEV, "noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(IF_ERR_HIP, std::string("noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                                        hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return IF_OK;
+}
+
+if_status Engine::monitor_sample_noalloc(int k)
+{
+    Slot *s = slot(k);
+    if (!s || s->state == -2) return fail(IF_ERR_ARG, "monitor_sample_noalloc: empty slot");
+    if (!s->mon_mem) return fail(IF_ERR_STATE, "monitor_sample_noalloc: no monitors on this case");
+    const int nt = s->ntx * s->nty;
+    if (nt > kScratchMaxTiles)
+        return fail(IF_ERR_ARG, "monitor_sample_noalloc: the case has " + std::to_string(nt) +
+                                    " tiles of 64 x 16 cells; the scratch array holds " +
+                                    std::to_string(kScratchMaxTiles));
+    void *base = nullptr;
+    if_status rc = scratch_ptr(&base);
+    if (rc != IF_OK) return rc;
+    char *b = (char *)base;
+    CaseDev *desc = (CaseDev *)b;
+    int2 *tiles = (int2 *)(b + kScratchDescBytes);
+    double *part = (double *)(b + kScratchDescBytes + kScratchTileBytes);
+    IFF_HIP(hipMemcpyAsync(desc, &s->dev, sizeof(CaseDev), hipMemcpyHostToDevice, st_));
+    launch_fill_tiles(st_, tiles, nt);
+    launch_monitor(st_, vprec_, desc, 1, tiles, nt, 1, part);
+    IFF_HIP(hipGetLastError());
+    IFF_HIP(hipStreamSynchronize(st_));
     return IF_OK;
 }
 

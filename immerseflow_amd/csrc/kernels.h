@@ -43,7 +43,8 @@ void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);
 void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt,
-                    int sample = 0);
+                    int sample = 0, double *part = nullptr);
+void launch_fill_tiles(hipStream_t st, int2 *tiles, int nt);
 void launch_tune_arm(hipStream_t st, const CaseDev *cs, int ncase, const Ctrl *ctrl0, const Ctrl *saved);
 
 // immersed-boundary setup (geometry.hip); fresh != 0: only moving cases,

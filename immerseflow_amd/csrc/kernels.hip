@@ -1359,10 +1359,11 @@ void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *til
 // per-case sum in tile order (deterministic).
 // ---------------------------------------------------------------------------
 // sample != 0: record the current fields of every listed case, running or
-// not, at the current time (if_case_monitor_sample).
+// not, at the current time (if_case_monitor_sample).  part: the per-tile
+// partials (Fx, Fy per tile of the listed case); null: the case's own.
 template <typename TS>
 __global__ __launch_bounds__(256) void k_monitor_part(const CaseDev *cs, const int2 *tiles, int ntiles,
-                                                      int ntiles_pad, int sample)
+                                                      int ntiles_pad, int sample, double *part)
 {
     TileCtx tc;
     if (!tile_setup(tiles, ntiles, ntiles_pad, cs, tc)) return;
@@ -1405,13 +1406,14 @@ __global__ __launch_bounds__(256) void k_monitor_part(const CaseDev *cs, const i
     fx = block_sum(fx);
     fy = block_sum(fy);
     if (threadIdx.x == 0) {
-        c.mon->part[2 * tc.t] = fx;
-        c.mon->part[2 * tc.t + 1] = fy;
+        double *pp = part ? part : c.mon->part;
+        pp[2 * tc.t] = fx;
+        pp[2 * tc.t + 1] = fy;
     }
 }
 
 template <typename TS>
-__global__ void k_monitor_rec(const CaseDev *cs, int ncase, int sample)
+__global__ void k_monitor_rec(const CaseDev *cs, int ncase, int sample, const double *part)
 {
     const CaseDev &c = cs[blockIdx.x];
     const Ctrl *ct = c.ctrl;
@@ -1420,8 +1422,8 @@ __global__ void k_monitor_rec(const CaseDev *cs, int ncase, int sample)
     const int nt = c.ntx * c.nty;
     double fx = 0.0, fy = 0.0;
     for (int k = 0; k < nt; k++) {
-        fx += m.part[2 * k];
-        fy += m.part[2 * k + 1];
+        fx += (part ? part : m.part)[2 * k];
+        fy += (part ? part : m.part)[2 * k + 1];
     }
     const int stride = 3 + 3 * m.nprobe;
     const unsigned long long n = *m.count;
@@ -1441,17 +1443,31 @@ __global__ void k_monitor_rec(const CaseDev *cs, int ncase, int sample)
 }
 
 void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt,
-                    int sample)
+                    int sample, double *part)
 {
     if (prec == IF_FP64) {
         hipLaunchKernelGGL(k_monitor_part<double>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt),
-                           sample);
-        hipLaunchKernelGGL(k_monitor_rec<double>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample);
+                           sample, part);
+        hipLaunchKernelGGL(k_monitor_rec<double>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample,
+                           (const double *)part);
     } else {
         hipLaunchKernelGGL(k_monitor_part<float>, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt),
-                           sample);
-        hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample);
+                           sample, part);
+        hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample,
+                           (const double *)part);
     }
+}
+
+// The tile table of one case, tiles (0, t) for t < nt, written on the device.
+__global__ void k_fill_tiles(int2 *tiles, int nt)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) tiles[t] = make_int2(0, t);
+}
+
+void launch_fill_tiles(hipStream_t st, int2 *tiles, int nt)
+{
+    if (nt > 0) hipLaunchKernelGGL(k_fill_tiles, dim3((nt + 255) / 256), dim3(256), 0, st, tiles, nt);
 }
 
 // Launch-shape timing: reset every running case's control block to the

@@ -285,6 +285,16 @@ if_status if_case_monitor_read(if_case *c, double *out, size_t cap_records, size
  * loaded with if_case_set_field (the predecessor writes its probes from
  * the current state too, test/UTIL_PRE_SIM.f90:172-202). */
 if_status if_case_monitor_sample(if_case *c);
+/* if_case_monitor_sample without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and the same record, with the descriptor copy, the tile table and
+ * the per-tile force partials staged in a static device array of the
+ * library (about 1.5 MiB per GPU, held until the process ends) instead of
+ * memory allocated per call.  Cases of more than 65,536 interior tiles of
+ * 64 x 16 cells (67 M cells) are refused with IF_ERR_ARG.  Calls that
+ * overlap on one GPU share that array and race: serialise them. */
+if_status if_case_monitor_sample_noalloc(if_case *c);
 /* Autotuner (no counterpart in the reference, whose loop is fixed):
  * times every Poisson sweep launch shape of if_case_tune on the case's
  * current fields (a few launches each, on the solver's scratch buffers;
