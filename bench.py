@@ -17,10 +17,12 @@ keeps several red-black sweeps per launch on chip, so the sweep is bound by
 vector-ALU issue, not by HBM: achieved = FP32 operations of the cell
 updates the launch executes (11 per update: one multiply and five fused
 multiply-adds; the temporal-blocking halo recompute included) / mean
-launch time from HIP events recorded by the library on its stream; peak =
-MI355X's published FP32 vector rate.  The measured HBM traffic of the same
-launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, profiles/) gives the HBM
-fraction beside it.
+launch time, from device time stamps the library records on its stream
+around each chunk of sweeps; peak = the FP32 rate without packed
+instructions (the published rate counts packed FP32).  The measured HBM
+traffic and VALU instruction count of the same launch (rocprofv3
+FETCH_SIZE / WRITE_SIZE / SQ_INSTS_VALU, profiles/) give the HBM fraction
+and the measured issue fraction beside it.
 
 cpu_baseline: the C oracle (the same per-cell routines in FP64, OpenMP) on
 the host's cores, on a bounded sample of the headline workload: one step
@@ -78,7 +80,7 @@ def parse():
     ap.add_argument("--c2-fp64-steps", type=int, default=3)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-window", type=int, default=2)
-    ap.add_argument("--serve-requests", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--cpu-sweeps", type=int, default=100,
                     help="Poisson sweeps of each capped oracle step in the CPU sample")
     return ap.parse_args()
@@ -209,18 +211,20 @@ def headline_c3(iff, configs, a):
     useful = cells * spl
     kname = kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float", pers)
     achieved = upd * FLOP_PER_UPDATE / avg_s / 1e12 if avg_s > 0 else 0.0
-    roof = dict(bound="valu", achieved=round(achieved, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP32_PEAK_TFLOPS, 4), traffic=None, kernel=kname,
-                frac_of_nonpacked_issue=round(achieved / FP32_NONPACKED_TFLOPS, 4),
-                nonpacked_peak=round(FP32_NONPACKED_TFLOPS, 2),
+    roof = dict(bound="valu", achieved=round(achieved, 2), peak=round(FP32_NONPACKED_TFLOPS, 2), unit="TFLOP/s",
+                frac=round(achieved / FP32_NONPACKED_TFLOPS, 4), traffic=None, kernel=kname,
+                published_fp32_peak=FP32_PEAK_TFLOPS,
+                frac_of_published=round(achieved / FP32_PEAK_TFLOPS, 4),
                 updates_per_launch=upd, interior_updates_per_launch=useful,
                 recompute_factor=round(upd / useful, 3), sweeps_per_launch=spl,
                 flop_per_update=FLOP_PER_UPDATE, valu_per_update=VALU_PER_UPDATE,
                 mean_launch_us=round(avg_s * 1e6, 3), launches_timed=launches,
                 useful_tflops=round(useful * FLOP_PER_UPDATE / avg_s / 1e12, 2) if avg_s > 0 else None,
-                note="executed updates x 11 FLOP / HIP-event launch time; the published peak counts packed "
-                     "FP32, which this build does not emit (frac_of_nonpacked_issue: against 256 CUs x 64 "
-                     "lanes x 2 FLOP x 2.4 GHz)")
+                note="achieved = cell updates the launch executes (the temporal-blocking halo recompute "
+                     "included) x 11 FP32 FLOP (1 multiply + 5 FMA: 6 VALU instructions) / mean launch time "
+                     "from device time stamps around each chunk; peak = 256 CUs x 64 lanes x 2 FLOP x 2.4 GHz, "
+                     "the FP32 rate without packed instructions, which this build does not emit (the "
+                     "published 157.3 TFLOP/s counts packed FP32)")
     tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c3.json")
     if os.path.exists(tp):
         with open(tp) as f:
@@ -234,9 +238,11 @@ def headline_c3(iff, configs, a):
                                       "calibrated) / the HIP-event launch time")
             if tj.get("valu_insts_per_launch"):
                 v = tj["valu_insts_per_launch"] * 64 / avg_s / 1e12
-                roof["valu_counter"] = dict(lane_ops_tops=round(v, 2),
+                roof["valu_counter"] = dict(lane_ops_per_s_T=round(v, 2),
                                             frac_of_issue=round(v / (FP32_NONPACKED_TFLOPS / 2), 4),
-                                            source="SQ_INSTS_VALU x 64 lanes / launch time")
+                                            valu_per_update=round(tj["valu_insts_per_launch"] * 64 / upd, 2),
+                                            source="SQ_INSTS_VALU (profiles/traffic_rbsor_reg_c3.json) x 64 "
+                                                   "lanes / launch time, against 256 CUs x 64 lanes x 2.4 GHz")
     out = dict(case=c, cfg=cfg, tune=tr, cells=cells, NX=NX, NY=NY, seconds=dt, stats=st, roofline=roof,
                step_bpc=step_bpc, mcs=cells * a.steps / dt / 1e6, tune_seconds=t_tune, pers=pers)
     return out
@@ -505,7 +511,8 @@ def serving_c5(iff, configs, a):
     from immerseflow_amd import serve
     trace = [(t, dict(c, check_every=160)) for t, c in configs.c5_trace(a.serve_requests)]
     budget = 64 << 30
-    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto")
+    small = sorted(range(len(trace)), key=lambda k: (len(trace[k][1]["xf"]) ** 2) * trace[k][1]["steps"])[:2]
+    recs, summ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto", keep_fields=set(small))
     log("serving (50 req/s):", summ)
     # sub-saturating: mean inter-arrival = 2x the mean service time per request
     svc = summ["wall_s"] / max(1, summ["completed"])
@@ -513,12 +520,8 @@ def serving_c5(iff, configs, a):
     scale = 2.0 * svc * len(trace) / span
     recs2, summ2 = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto", time_scale=scale)
     log("serving (sub-saturating):", summ2)
-    # each result against the same request run alone (mixed) -- bit for bit
-    small = sorted(range(len(trace)), key=lambda k: (len(trace[k][1]["xf"]) ** 2) * trace[k][1]["steps"])[:2]
-    recs3, _ = serve.serve(trace, precision=iff.IF_MIXED, hbm_budget=budget, tune="auto",
-                           keep_fields=set(small))
     err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
-    for r in recs3:
+    for r in recs:
         if r["id"] in small and r["state"] == 1:
             e = serve.error_vs_fp64(r, trace[r["id"]][1])
             err = {f: max(err[f], e[f]) for f in err}

@@ -9,15 +9,29 @@ C2  ellipse (chord 1, thickness ratio 0.1) at 10 deg incidence, Re=300,
     stretching outward to a 30x20 domain (per-side ratio solved so the
     cells fill the domain; <= 1.02), dt = 0.5 h (CFL 0.5 at U=1).
 C3  eight seeded star bodies in a 20x20 box, 4096^2 uniform, Re=1000.
-C4  256 seeded cases, n log-uniform in [128, 1024], circle or ellipse.
+C4  256 seeded cases, n log-uniform in [128, 1024], circle or ellipse,
+    dt = min(0.5 h, 1/Re) (stable_dt).
 C5  Poisson-arrival request trace: star cases (64^2..2048^2) plus 20 %
-    heaving cylinders, 100..1000 steps.
+    heaving cylinders, 100..1000 steps, dt = min(0.5 h, 1/Re).
 """
 import math
 
 import numpy as np
 
 from . import circle, ellipse, heaving_circle, star
+
+
+def stable_dt(h, re, cfl=0.5):
+    """Time step of the uniform-grid workloads: CFL 0.5 at U = 1, capped at
+    1/Re.  The predictor advances the convective term explicitly (forward
+    Euler, central differences, the reference's factor f = 0.5,
+    ADSolver.cu:66-74) with the viscous term implicit; its von Neumann
+    amplification factor (1 - i c sin t) / (1 + 4 d sin^2(t/2)), c = f|u|dt/h,
+    d = dt/(Re h^2), stays within the unit circle only for
+    dt <= 2 / (Re f^2 |u|^2).  The flow around the bodies reaches |u| ~ 2.4,
+    so the limit is ~1.4/Re; coarse grids at high Re (CFL 0.5 gives dt up to
+    0.16 on 64^2 in 20 x 20) diverge in a few hundred steps."""
+    return min(cfl * h, 1.0 / re)
 
 
 def write_grid_file(path, faces):
@@ -134,7 +148,7 @@ def c4_cases(count=256, seed=4):
             ang = rng.uniform(0.0, math.pi)
             body = ellipse(5.0, 8.0, 0.5, 0.5 / asp, ang)
         re = rng.uniform(50.0, 500.0)
-        cases.append(dict(xf=xf, yf=xf.copy(), bodies=[body], dt=0.5 * h, re=re, steps=200,
+        cases.append(dict(xf=xf, yf=xf.copy(), bodies=[body], dt=stable_dt(h, re), re=re, steps=200,
                           ppe_tol=1e-7))
     return cases
 
@@ -154,6 +168,7 @@ def c5_trace(count=500, seed=5, rate=50.0):
             bodies = [heaving_circle(6.0, 10.0, 0.5, 0.5, 0.2)]
         else:
             bodies = star_set(rng, 1 + int(rng.uniform() * 3), 2.0, 18.0)
-        out.append((t, dict(xf=xf, yf=xf.copy(), bodies=bodies, dt=0.5 * h,
-                            re=rng.uniform(100.0, 1000.0), steps=steps, ppe_tol=1e-7)))
+        re = rng.uniform(100.0, 1000.0)
+        out.append((t, dict(xf=xf, yf=xf.copy(), bodies=bodies, dt=stable_dt(h, re),
+                            re=re, steps=steps, ppe_tol=1e-7)))
     return out

@@ -94,8 +94,8 @@ private:
     if_status read_ctrl_sync(std::vector<Ctrl> &out);
     if_status update_moving();
     // the K sweeps of one Poisson chunk, as kernel launches on st_
-    void enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers);
-    if_status capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers);
+    void enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers, bool stamp);
+    if_status capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers, bool stamp);
     if_status check_barrier();
     void drop_graph();
 
@@ -121,11 +121,22 @@ private:
     static constexpr int kRing = 8;
     Ctrl *h_snap_ = nullptr;           // pinned, kRing * cap_ctrl_
     hipEvent_t ev_[kRing] = {};
-    hipEvent_t tev_[2 * 64] = {};      // timing events
+    // device time stamps of the timed Poisson chunks: [0] count, then a
+    // start and an end stamp per chunk (at most kStamps)
+    static constexpr int kStamps = 2 * 64;
+    unsigned long long *d_stamps_ = nullptr, *h_stamps_ = nullptr;  // 1 + kStamps each (host: pinned)
+    double stamp_hz_ = 100.0e6;        // rate of the real-time counter
     // captured sweep chunk, valid for the resident set of the last
     // refresh() and the launch shape in g_key_
     hipGraphExec_t g_sweeps_ = nullptr;
-    int g_key_[8] = {};
+    int g_key_[9] = {};
+    bool g_valid_ = false;     // g_sweeps_ matches the current tables
+
+public:
+    long long graph_instantiations = 0, graph_updates = 0;
+    long long persistent_timeouts = 0;  // persistent chunks re-solved per stage
+
+private:
     GridBar *d_bar_ = nullptr;     // error flag of the persistent sweep
     unsigned *d_flags_ = nullptr;  // its per-tile stage counters
     int cap_flags_ = 0;

@@ -69,8 +69,8 @@ Engine::~Engine()
     if (h_snap_) hipHostFree(h_snap_);
     for (auto &e : ev_)
         if (e) hipEventDestroy(e);
-    for (auto &e : tev_)
-        if (e) hipEventDestroy(e);
+    if (d_stamps_) hipFree(d_stamps_);
+    if (h_stamps_) hipHostFree(h_stamps_);
     if (st_) hipStreamDestroy(st_);
 }
 
@@ -92,7 +92,15 @@ if_status Engine::init()
     IFF_HIP(hipMemcpyAsync(d_bar_, &b, sizeof(GridBar), hipMemcpyHostToDevice, st_));
     IFF_HIP(hipHostMalloc((void **)&h_snap_, sizeof(Ctrl) * cap_ctrl_ * kRing, hipHostMallocDefault));
     for (auto &e : ev_) IFF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &e : tev_) IFF_HIP(hipEventCreate(&e));
+    IFF_HIP(hipMalloc(&d_stamps_, sizeof(unsigned long long) * (1 + kStamps)));
+    IFF_HIP(hipMemsetAsync(d_stamps_, 0, sizeof(unsigned long long) * (1 + kStamps), st_));
+    IFF_HIP(hipHostMalloc((void **)&h_stamps_, sizeof(unsigned long long) * (1 + kStamps), hipHostMallocDefault));
+    {
+        int dev = 0, khz = 0;
+        IFF_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            stamp_hz_ = 1e3 * (double)khz;
+    }
     IFF_HIP(hipStreamSynchronize(st_));
     return IF_OK;
 }
@@ -604,9 +612,10 @@ void Engine::drop_graph()
 // given resident set and shape: the kernels read the loop state (buffer
 // parity, sweep counts, done flags) from the control blocks on the device,
 // so a captured chunk replays unchanged.
-void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers)
+void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers, bool stamp)
 {
     const int n = (int)order_.size();
+    if (stamp) launch_stamp(st_, d_stamps_, kStamps);
     if (any_rb && pers)  // the whole chunk in one launch
         launch_ppe_sweep_pers(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
                               K, d_bar_, d_flags_);
@@ -625,28 +634,51 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
             launch_ppe_sweep(st_, pprec_, IF_PPE_JACOBI, kTileY, 1, d_cases_, d_tiles_base_, nt_base_, s, K);
         if (any_line) launch_line_sor(st_, pprec_, d_cases_, n, max_ring_, s, K);
     }
+    if (stamp) launch_stamp(st_, d_stamps_, kStamps);
 }
 
 // Capture the chunk into a graph (kernel nodes only) unless the cached one
-// matches the shape.  refresh() drops the graph, since it re-uploads the
-// case list and tile tables the launches point at.
-if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers)
+// matches the shape and the resident set.  refresh() invalidates the key,
+// since it re-uploads the case list and tile tables the launches point at
+// (and may reallocate them).  A new capture with the cached graph's
+// topology (same kernels, same count) updates the instantiated graph in
+// place (hipGraphExecUpdate: new kernel arguments and launch sizes); only
+// a different topology destroys it and instantiates a new one.  The stream
+// is idle whenever this runs (every step ends with a synchronisation).
+if_status Engine::capture_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_line, bool pers, bool stamp)
 {
-    const int key[8] = {K, M, sweep_kind, tiles_sweep_ty_, (int)any_rb, (int)any_jac, (int)any_line, (int)pers};
-    if (g_sweeps_ && std::equal(key, key + 8, g_key_)) return IF_OK;
-    drop_graph();
+    const int key[9] = {K, M, sweep_kind, tiles_sweep_ty_, (int)any_rb, (int)any_jac, (int)any_line, (int)pers,
+                        (int)stamp};
+    if (g_sweeps_ && g_valid_ && std::equal(key, key + 9, g_key_)) return IF_OK;
     hipGraph_t g = nullptr;
     IFF_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
-    enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers);
+    enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers, stamp);
     hipError_t e = hipStreamEndCapture(st_, &g);
     if (e != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph capture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&g_sweeps_, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        g_sweeps_ = nullptr;
-        return fail(IF_ERR_HIP, std::string("sweep graph instantiate: ") + hipGetErrorString(e));
+    bool updated = false;
+    if (g_sweeps_) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(g_sweeps_, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();  // a refused update leaves no error behind
+            drop_graph();
+        } else {
+            graph_updates++;
+        }
     }
-    std::copy(key, key + 8, g_key_);
+    if (!updated) {
+        e = hipGraphInstantiate(&g_sweeps_, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            g_sweeps_ = nullptr;
+            return fail(IF_ERR_HIP, std::string("sweep graph instantiate: ") + hipGetErrorString(e));
+        }
+        graph_instantiations++;
+    }
+    (void)hipGraphDestroy(g);
+    std::copy(key, key + 9, g_key_);
+    g_valid_ = true;
     return IF_OK;
 }
 
@@ -670,7 +702,7 @@ if_status Engine::check_barrier()
 
 if_status Engine::refresh()
 {
-    drop_graph();
+    g_valid_ = false;  // the captured launches point at the old tables
     order_.clear();
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
@@ -870,39 +902,66 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
 
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;
-        int timed = 0;
         // persistent chunk: a register-tile shape with a persistent form in
         // this precision, and the grid fits on the device
-        const bool pers = any_rb && use_persistent && sweep_kind >= 2 &&
-                          pers_fits(pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
+        bool pers = any_rb && use_persistent && sweep_kind >= 2 &&
+                    pers_fits(pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, nt_sweep_);
         ran_pers = ran_pers || pers;
         if (any_rb) launch_sweeps = pers ? K : M;
         if (use_graphs) {
-            rc = capture_sweeps(K, M, any_rb, any_jac, any_line, pers);
+            rc = capture_sweeps(K, M, any_rb, any_jac, any_line, pers, timing);
             if (rc != IF_OK) return rc;
         }
+        if (timing) launch_stamp_reset(st_, d_stamps_);
         hipError_t glaunch = hipSuccess;
-        rc = spec(nunits, 1, [&](int u) {
+        bool reopened = false;  // the solve after a persistent time-out (no entry check)
+        auto ppe_unit = [&](int u) {
             if (u == 0) {
-                launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
+                if (!reopened) launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, 0);
                 return;
             }
-            const bool tm = timing && (u - 1) < 64;
-            if (tm) hipEventRecord(tev_[2 * (u - 1)], st_);
             if (use_graphs) {
                 const hipError_t e = hipGraphLaunch(g_sweeps_, st_);
                 if (e != hipSuccess && glaunch == hipSuccess) glaunch = e;
             } else {
-                enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers);
+                enqueue_sweeps(K, M, any_rb, any_jac, any_line, pers, timing);
             }
-            if (tm) {
-                hipEventRecord(tev_[2 * (u - 1) + 1], st_);
-                timed = u;
-            }
-            launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, K);
-        });
+            launch_ppe_check(st_, pprec_, d_cases_, n, d_tiles_base_, nt_base_, K, pers ? d_bar_ : nullptr);
+        };
+        rc = spec(nunits, 1, ppe_unit);
         if (rc != IF_OK) return rc;
         if (glaunch != hipSuccess) return fail(IF_ERR_HIP, std::string("sweep graph launch: ") + hipGetErrorString(glaunch));
+        if (pers) {
+            // A neighbour wait of the persistent sweep that timed out (the
+            // blocks were not all resident) ended the Poisson loop early,
+            // with a finite but unconverged pressure.  Turn the persistent
+            // form off, clear the flag and the stage counters, and solve
+            // again from that pressure with one launch per stage; the step
+            // then completes as usual.
+            GridBar b{};
+            IFF_HIP(hipMemcpyAsync(&b, d_bar_, sizeof(GridBar), hipMemcpyDeviceToHost, st_));
+            IFF_HIP(hipStreamSynchronize(st_));
+            if (b.err) {
+                use_persistent = false;
+                pers = false;
+                if (agg) agg->persistent_timeouts++;
+                launch_sweeps = M;
+                persistent_timeouts++;
+                IFF_HIP(hipMemsetAsync(&d_bar_->err, 0, sizeof(unsigned), st_));
+                if (cap_flags_ > 0) IFF_HIP(hipMemsetAsync(d_flags_, 0, sizeof(unsigned) * cap_flags_, st_));
+                launch_ppe_rearm(st_, d_cases_, n);
+                if (use_graphs) {
+                    rc = capture_sweeps(K, M, any_rb, any_jac, any_line, false, timing);
+                    if (rc != IF_OK) return rc;
+                }
+                if (timing) launch_stamp_reset(st_, d_stamps_);
+                reopened = true;
+                rc = spec(nunits, 1, ppe_unit);
+                if (rc != IF_OK) return rc;
+                if (glaunch != hipSuccess)
+                    return fail(IF_ERR_HIP, std::string("sweep graph launch: ") + hipGetErrorString(glaunch));
+            }
+        }
 
         // ---- pressure BC, projection, divergence, ghost cells
         launch_pressure_bc(st_, d_cases_, n, max_ring_, max_ng_);
@@ -912,6 +971,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         launch_step_end(st_, d_cases_, n);
         for (int id : order_) slots_[id]->steps_host++;
 
+        if (timing)  // completes with the synchronisation below
+            IFF_HIP(hipMemcpyAsync(h_stamps_, d_stamps_, sizeof(unsigned long long) * (1 + kStamps),
+                                   hipMemcpyDeviceToHost, st_));
         std::vector<Ctrl> all;
         rc = read_ctrl_sync(all);
         if (rc != IF_OK) return rc;
@@ -919,16 +981,14 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             rc = check_barrier();
             if (rc != IF_OK) return rc;
         }
-        if (timing && timed > 0) {
+        const int timed = timing ? (int)std::min<unsigned long long>(h_stamps_[0], kStamps) / 2 : 0;
+        if (timed > 0) {
             int maxsw = 0;
             for (int id : order_) maxsw = std::max(maxsw, all[id].ppe_sweeps);
             const int nfull = std::min(timed, maxsw / K);
             for (int u = 0; u < nfull; u++) {
-                float ms = 0.f;
-                if (hipEventElapsedTime(&ms, tev_[2 * u], tev_[2 * u + 1]) == hipSuccess) {
-                    sweep_ms += ms;
-                    sweep_launches += ran_pers ? 1 : (K + m_eff - 1) / m_eff;
-                }
+                sweep_ms += 1e3 * (double)(h_stamps_[2 + 2 * u] - h_stamps_[1 + 2 * u]) / stamp_hz_;
+                sweep_launches += pers ? 1 : (K + m_eff - 1) / m_eff;
             }
             for (int id : order_)
                 sweep_cells += (double)std::min(all[id].ppe_sweeps, nfull * K) *

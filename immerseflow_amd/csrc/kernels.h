@@ -35,7 +35,8 @@ int pers_blocks(int prec, int waves, int ty, int m);  // 0: no persistent form
 void launch_ppe_sweep_pers(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                            const int2 *tiles, int nt, int K, GridBar *bar, unsigned *flags);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
-                      int nt, int K);
+                      int nt, int K, const GridBar *bar = nullptr);
+void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase);
 // zebra line SOR: the per-step factors (pivots, c', 1/m) of every line
 // case, then one sweep (both colours) at chunk position s
 void launch_line_factor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny);
@@ -45,6 +46,9 @@ void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);
 void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles, int nt,
                     int sample = 0, double *part = nullptr);
 void launch_fill_tiles(hipStream_t st, int2 *tiles, int nt);
+// device time stamps of the Poisson chunks (buf[0]: count, then the stamps)
+void launch_stamp(hipStream_t st, unsigned long long *buf, int cap);
+void launch_stamp_reset(hipStream_t st, unsigned long long *buf);
 void launch_tune_arm(hipStream_t st, const CaseDev *cs, int ncase, const Ctrl *ctrl0, const Ctrl *saved);
 
 // immersed-boundary setup (geometry.hip); fresh != 0: only moving cases,

@@ -1088,7 +1088,11 @@ __global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, con
 // One block per case: the Poisson stopping rule on the device.
 // nl_rule: 0 = FP64 ping-pong (base advances by the launch count),
 //          1 = mixed (base flips once when an update happened).
-__global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int ncase, int K, int mixed)
+// bar (persistent chunks only): once a neighbour wait of the persistent
+// sweep has timed out, the loop ends here too (ppe_done, not failed); the
+// host then solves again with one launch per stage.
+__global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int ncase, int K, int mixed,
+                                                      const GridBar *bar)
 {
     const CaseDev &c = cs[blockIdx.x];
     Ctrl *ct = c.ctrl;
@@ -1112,7 +1116,26 @@ __global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int nca
         ct->ppe_res = res;
         if (!(res <= 1.0e300)) ct->failed = 1;
         if (!(res > c.ppe_tol) || ct->ppe_sweeps >= c.ppe_itermax || ct->failed) ct->ppe_done = 1;
+        if (bar && __hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ct->ppe_done = 1;
     }
+}
+
+// Re-open the Poisson loop of every running case (after a persistent chunk
+// timed out): the solve starts again from the pressure the stopped chunk
+// left, which is finite, and the iteration converges from any start.
+__global__ void k_ppe_rearm(const CaseDev *cs, int ncase)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncase) return;
+    Ctrl *c = cs[k].ctrl;
+    if (!c->active || c->failed) return;
+    c->ppe_done = 0;
+    c->ppe_sweeps = 0;
+}
+
+void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase)
+{
+    hipLaunchKernelGGL(k_ppe_rearm, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase);
 }
 
 // ---------------------------------------------------------------------------
@@ -1330,14 +1353,14 @@ void launch_ppe_sweep(hipStream_t st, int prec, int solver, int ty, int m, const
 }
 
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
-                      int nt, int K)
+                      int nt, int K, const GridBar *bar)
 {
     if (prec == IF_FP64)
         hipLaunchKernelGGL(k_ppe_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt, tile_pad(nt), K);
     else
         hipLaunchKernelGGL(k_ppe_update_resid, tile_grid(nt), dim3(256), 0, st, cs, tiles, nt,
                            tile_pad(nt), K);
-    hipLaunchKernelGGL(k_ppe_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase, K, prec == IF_FP64 ? 0 : 1);
+    hipLaunchKernelGGL(k_ppe_finalize, dim3(ncase), dim3(256), 0, st, cs, ncase, K, prec == IF_FP64 ? 0 : 1, bar);
 }
 
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt)
@@ -1456,6 +1479,33 @@ void launch_monitor(hipStream_t st, int prec, const CaseDev *cs, int ncase, cons
         hipLaunchKernelGGL(k_monitor_rec<float>, dim3(ncase), dim3(64), 0, st, cs, ncase, sample,
                            (const double *)part);
     }
+}
+
+// Chunk timing on the device: thread 0 appends the real-time counter
+// (s_memrealtime, constant rate) to buf[1 ..], buf[0] counting the stamps
+// (at most cap are kept).  The stamps are kernels, so they can sit inside a
+// captured sweep graph: no host event is recorded between graph launches.
+__global__ void k_stamp(unsigned long long *buf, int cap)
+{
+    if (threadIdx.x != 0) return;
+    const unsigned long long n = buf[0];
+    if (n < (unsigned long long)cap) buf[1 + n] = __builtin_amdgcn_s_memrealtime();
+    buf[0] = n + 1;
+}
+
+__global__ void k_stamp_reset(unsigned long long *buf)
+{
+    if (threadIdx.x == 0) buf[0] = 0;
+}
+
+void launch_stamp(hipStream_t st, unsigned long long *buf, int cap)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, buf, cap);
+}
+
+void launch_stamp_reset(hipStream_t st, unsigned long long *buf)
+{
+    hipLaunchKernelGGL(k_stamp_reset, dim3(1), dim3(64), 0, st, buf);
 }
 
 // The tile table of one case, tiles (0, t) for t < nt, written on the device.

@@ -140,6 +140,9 @@ typedef struct {
     long long total_ad_iters, total_ppe_sweeps; /* over the call */
     double max_div;         /* max over the call's steps */
     int failed;             /* non-finite state detected */
+    int persistent_timeouts; /* steps whose persistent Poisson chunk timed out
+                                (blocks not all resident) and was solved again
+                                with one launch per stage */
 } if_step_stats;
 
 typedef struct {
@@ -260,8 +263,11 @@ if_status if_case_graphs(if_case *c, int enable);
  * sweeps runs as one launch whose tiles stay in registers, in stages of
  * sweeps_per_launch sweeps separated by a halo exchange with the
  * neighbouring tiles; otherwise one launch per stage.  Results are
- * identical.  A neighbour wait that times out makes the step return
- * IF_ERR_HIP and turns the form off. */
+ * identical.  A neighbour wait that times out (20 ms; the blocks were not
+ * all resident) ends that solve early with a finite pressure; the step then
+ * turns the form off, solves again from that pressure with one launch per
+ * stage, completes as usual (converged to the same stop level) and counts
+ * it in if_step_stats.persistent_timeouts. */
 if_status if_case_persistent(if_case *c, int enable);
 /* Monitors (the predecessor's probes, test/UTIL_PRE_SIM.f90:172-202, and a
  * body-force diagnostic with no counterpart there): after every step the

@@ -76,6 +76,11 @@ def test_c1_c3_c4_c5_grid_specs():
     ns = [len(c["xf"]) - 1 for c in c4]
     assert len(c4) == 256 and min(ns) >= 128 and max(ns) <= 1024
     assert all(c["steps"] == 200 and 50 <= c["re"] <= 500 for c in c4)
+    # dt: CFL 0.5 capped by the explicit predictor's stability limit
+    for c in c4:
+        h = 16.0 / (len(c["xf"]) - 1)
+        assert c["dt"] == min(0.5 * h, 1.0 / c["re"])
+    assert any(c["dt"] < 0.5 * 16.0 / (len(c["xf"]) - 1) for c in c4)
     tr = configs.c5_trace(500)
     ts = [t for t, _ in tr]
     assert len(tr) == 500 and all(b > a for a, b in zip(ts, ts[1:]))
@@ -83,6 +88,7 @@ def test_c1_c3_c4_c5_grid_specs():
     ns = [len(c["xf"]) - 1 for _, c in tr]
     assert min(ns) >= 64 and max(ns) <= 2048
     assert all(100 <= c["steps"] <= 1000 for _, c in tr)
+    assert all(c["dt"] == configs.stable_dt(20.0 / (len(c["xf"]) - 1), c["re"]) for _, c in tr)
     heave = sum(1 for _, c in tr if c["bodies"][0].get("motion", 0) == iff.IF_MOTION_HEAVE)
     assert 0.12 * 500 < heave < 0.28 * 500
 

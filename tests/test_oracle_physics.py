@@ -434,3 +434,34 @@ def test_star_polygon_inside_and_closest_point():
         d = math.hypot(qx - bx.value, qy - by.value)
         dmin = np.min(np.hypot(qx - px, qy - py))
         assert dmin - 1e-4 <= d <= dmin + 1e-12
+
+
+def test_time_step_rule_keeps_the_explicit_predictor_stable():
+    """The uniform-grid workloads take dt = min(0.5 h, 1/Re) (configs.stable_dt):
+    the convective term of the predictor is explicit (forward Euler, central
+    differences, factor 0.5 -- ADSolver.cu:46-79), so CFL 0.5 alone is not
+    enough at high cell Reynolds number.  A coarse serving request (70^2 in
+    20 x 20, Re 598, star body) diverges within 300 steps at dt = 0.5 h and
+    stays bounded at the capped step."""
+    from immerseflow_amd import configs
+    t, c = configs.c5_trace(6)[5]
+    n = len(c["xf"]) - 1
+    assert n == 70 and c["dt"] == configs.stable_dt(20.0 / n, c["re"]) < 0.5 * 20.0 / n
+    bodies = []
+    for b in c["bodies"]:
+        d = dict(b)
+        d["c"], d["s"] = b.get("cth", 1.0), b.get("sth", 0.0)
+        bodies.append(d)
+    out = {}
+    for dt in (0.5 * 20.0 / n, c["dt"]):
+        oc = O.PhysCase(c["xf"], c["yf"], bodies, prm(dt=dt, re=c["re"], ppe_omega=2 / (1 + math.sin(math.pi / n)),
+                                                      ppe_tol=1e-7, check_every=20))
+        umax = 0.0
+        for _ in range(300):
+            oc.step(1)
+            umax = float(np.max(np.abs(oc.u)))
+            if not np.isfinite(umax) or umax > 50.0:
+                break
+        out[dt] = umax
+    assert not (out[0.5 * 20.0 / n] <= 50.0)
+    assert out[c["dt"]] < 3.0
