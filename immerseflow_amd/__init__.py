@@ -277,7 +277,9 @@ class Case:
         register-tile with 4/8/16 waves), tile or strip height, red-black
         sweeps per launch, sweeps per check.  Every kernel and shape gives
         the same result.  Without a call the library uses 4/64/8 (mixed)
-        or 3/32/4 (FP64) and the case's check_every."""
+        or, in FP64, 4/48, 4/64 or 4/80 with 4 sweeps as persistent
+        chunks (the smallest height that fits on the device) else 3/32/4, and
+        the case's check_every."""
         _check(_lib().if_case_tune(self._h, kernel, tile_y, sweeps_per_launch, check_every), "tune")
 
     def restart(self, u, v, p, uf=None, vf=None):

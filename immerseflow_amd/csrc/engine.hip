@@ -707,12 +707,12 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
-    // FP64 default shape, by measurement: 16-wave tiles of 80 rows as
-    // persistent chunks when the running cases' tiles at that shape fill at
-    // least 3/4 of the device in one launch (the 2048 x 1024 case: 247
-    // tiles, 1.42x faster per sweep); else 48-row persistent chunks when
-    // they fit (1.1-1.3x faster than per-stage launches on 128^2..1024^2);
-    // else per-stage launches of 8-wave tiles of 32 rows
+    // FP64 default shape, by measurement: 16-wave tiles as persistent
+    // chunks of the smallest height (48, 64, 80 rows) whose tiles fit in
+    // one launch, i.e. the most tiles the device holds at once (1024^2:
+    // 48 rows, 3.5 us per sweep against 3.9 at 64 and 5.0 at 80; 2048 x
+    // 1024: only 80 rows fit, 4.6 us against 6.1 per-stage); per-stage
+    // launches of 8-wave tiles of 32 rows when none fits
     if (shape_auto && pprec_ == IF_FP64) {
         auto tiles_at = [&](int ty) {
             int nt = 0;
@@ -720,12 +720,13 @@ if_status Engine::refresh()
                 nt += ((slots_[id]->NX + reg_out_cols(4) - 1) / reg_out_cols(4)) * ((slots_[id]->NY + ty - 1) / ty);
             return nt;
         };
-        const int nt80 = tiles_at(80), cap80 = pers_blocks(IF_FP64, 16, 80, 4);
         int ty = 32;
-        if (use_persistent && cap80 > 0 && pers_fits(IF_FP64, 16, 80, 4, nt80) && 4 * nt80 >= 3 * cap80)
-            ty = 80;
-        else if (use_persistent && pers_fits(IF_FP64, 16, 48, 4, tiles_at(48)))
-            ty = 48;
+        if (use_persistent)
+            for (int cand : {48, 64, 80})
+                if (pers_fits(IF_FP64, 16, cand, 4, tiles_at(cand))) {
+                    ty = cand;
+                    break;
+                }
         sweep_kind = ty == 32 ? 3 : 4;
         sweep_ty = ty;
         sweep_m = 4;
@@ -1087,8 +1088,9 @@ if_status Engine::autotune(if_tune_result *out)
     };
     // register tiles of 16, 8 and 4 waves at 8, 4 and 2 sweeps per launch,
     // and the LDS-tiled kernel; shapes a precision lacks are skipped
-    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {4, 80, 4}, {3, 32, 8}, {3, 48, 8},
-                                  {3, 64, 8}, {3, 32, 4}, {3, 48, 4}, {3, 24, 2}, {2, 16, 4}, {0, 16, 2}};
+    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {4, 64, 4},
+                                  {4, 80, 4}, {3, 32, 8}, {3, 48, 8}, {3, 64, 8}, {3, 32, 4},
+                                  {3, 48, 4}, {3, 24, 2}, {2, 16, 4}, {0, 16, 2}};
     constexpr int kReps = 6;
     auto sweep_launch = [&](const Shape &c, int s, int K) {
         if (c.kind >= 2)

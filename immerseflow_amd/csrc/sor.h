@@ -62,9 +62,12 @@ __device__ __forceinline__ void gstore_vec(void *p, VT v)
 // face coefficients (zero on closed faces), sa = (aE + aW) + (aN + aS) =
 // -ap, r the right side, om the relaxation factor, i the cell's column.
 // FP64: the oracle's expression and order, no contraction, IEEE division:
-//   sum = (aE pE + aW pW) + (aN pN + aS pS);  gs = (r - sum) / ap;
-//   p' = pc + om (gs - pc)
-// so the FP64 sweeps are bit-identical to it.  FP32 (the error equation
+//   sum = (aE pE + aW pW) + (aN pN + aS pS);  w = om / sa;
+//   p' = (1 - om) pc + w (sum - r)
+// (= pc + om (gs - pc) with gs = (r - sum) / ap in exact arithmetic); the
+// weight w depends only on the cell's flags and the metric, so a kernel may
+// form it once per launch (sor_weight64) and apply sor_cell_w64 per sweep
+// with the same bits.  The FP64 sweeps are bit-identical to the oracle.  FP32 (the error equation
 // of the mixed path, refined in FP64 afterwards): the same update
 // rearranged for fused multiply-adds and the hardware reciprocal, with the
 // weight w = om / sa and wr = w r:
@@ -75,6 +78,13 @@ __device__ __forceinline__ void gstore_vec(void *p, VT v)
 // Every FP32 sweep kernel evaluates exactly this.
 // FP32 relaxation weight om / sa of a cell (sa != 0).
 __device__ __forceinline__ float sor_weight(float sa, float om) { return om * __builtin_amdgcn_rcpf(sa); }
+
+// FP64 relaxation weight and update (IEEE division; c1 = 1 - om).
+__device__ __forceinline__ double sor_weight64(double sa, double om) { return om / sa; }
+__device__ __forceinline__ double sor_cell_w64(double pc, double r, double sum, double w, double c1)
+{
+    return c1 * pc + w * (sum - r);
+}
 
 // FP32 update with w = sor_weight(sa, om), wr = w * r and c1 = 1 - om formed.
 __device__ __forceinline__ float sor_cell_w(float pc, float wr, float a1, float p1, float a2, float p2, float aN,
@@ -90,8 +100,7 @@ __device__ __forceinline__ T sor_cell(T pc, T r, T aE, T pE, T aW, T pW, T aN, T
 {
     if constexpr (sizeof(T) == 8) {
         const T sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
-        const T gs = (r - sum) / (-sa);
-        return pc + om * (gs - pc);
+        return sor_cell_w64(pc, r, sum, sor_weight64(sa, om), 1.0 - om);
     } else {
         const float w = sor_weight(sa, om);
         const bool odd = (i & 1) != 0;

@@ -40,9 +40,31 @@ struct RegTile {
     T pa[NRW], pb[NRW];  // unknown, slots a (even column) and b (odd column)
     T ra[NRW], rb[NRW];  // right side (FP32: w r once the weights are formed)
     T cyp[NRW], cym[NRW];
-    T cy[NRW];           // FP64: cyp + cym of the fast path
-    T wa[NRW], wb[NRW];  // FP32: the cell's weight om/sa, 0 for a cell not updated
+    T wa[NRW], wb[NRW];  // the cell's weight om/sa, 0 for a cell the sweep does not update
 };
+
+// A wave-uniform value held in scalar registers (the row coefficients are
+// the same in every lane; in SGPRs they cost no vector registers and enter
+// the multiplies as scalar operands).
+__device__ __forceinline__ float uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ void *uniform_ptr(void *p)
+{
+    const unsigned long long b = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return (void *)(((unsigned long long)hi << 32) | lo);
+}
 
 // a if face bit F of flags f is set, else +0: the bit, sign-extended to a
 // full mask, ANDed with a's bits (two VALU operations, no compare).
@@ -72,31 +94,35 @@ __device__ __forceinline__ float reg_cell_w(float pc, float wr, float a1, float 
     return sor_cell_w(pc, wr, a1, p1, a2, p2, cyp, pN, cym, pS, w, c1);
 }
 
-// One cell of a half-sweep, FP64.  Fast path (FAST): a fluid cell whose faces
-// are all open except those on the domain boundary; the coefficients are
-// the metric with the boundary faces' entries zeroed, so both paths give
-// the same bits for such a cell.  Flag path: the oracle's rule, coefficient
-// zero on a closed face, no update of a non-fluid cell or one with ap = 0.
-template <typename T, bool FAST>
-__device__ __forceinline__ T reg_cell(int f, T pc, T r, T pE, T pW, T pN, T pS, T cxp, T cxm, T cx,
-                                      T cyp, T cym, T cy, T om)
+// One cell of a half-sweep, FP64, with the cell's weight w = om / sa formed
+// once per launch (w = 0: a cell the sweep does not update).  Fast path
+// (FAST): a fluid cell whose faces are all open except those on the domain
+// boundary; the coefficients are the metric with the boundary faces' entries
+// zeroed, so both paths give the same bits for such a cell.  Flag path: the
+// oracle's rule, coefficient zero on a closed face, no update of a non-fluid
+// cell or one with ap = 0.
+template <bool FAST>
+__device__ __forceinline__ double reg_cell(int f, double pc, double r, double pE, double pW, double pN, double pS,
+                                           double cxp, double cxm, double cyp, double cym, double w, double c1)
 {
     if constexpr (FAST) {
-        return sor_cell<T>(pc, r, cxp, pE, cxm, pW, cyp, pN, cym, pS, cx + cy, om, 0);
+        const double sum = (cxp * pE + cxm * pW) + (cyp * pN + cym * pS);
+        return sor_cell_w64(pc, r, sum, w, c1);
     } else {
-        const T aE = face_coef<T, F_OPEN_E>(f, cxp);
-        const T aW = face_coef<T, F_OPEN_W>(f, cxm);
-        const T aN = face_coef<T, F_OPEN_N>(f, cyp);
-        const T aS = face_coef<T, F_OPEN_S>(f, cym);
-        const T sa = (aE + aW) + (aN + aS);
-        const T nv = sor_cell<T>(pc, r, aE, pE, aW, pW, aN, pN, aS, pS, sa, om, 0);
-        return ((f & F_FLUID) && sa != (T)0) ? nv : pc;
+        const double aE = face_coef<double, F_OPEN_E>(f, cxp);
+        const double aW = face_coef<double, F_OPEN_W>(f, cxm);
+        const double aN = face_coef<double, F_OPEN_N>(f, cyp);
+        const double aS = face_coef<double, F_OPEN_S>(f, cym);
+        const double sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
+        const double nv = sor_cell_w64(pc, r, sum, w, c1);
+        return w != 0.0 ? nv : pc;
     }
 }
 
+// One row's cells of a half-sweep.  c1 = 1 - om; the cells' weights are in R.
 template <typename T, int NRW, int Q, bool FAST, int k>
 __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb, T nva, T nvb, T cxpa,
-                                        T cxma, T cxa, T cxpb, T cxmb, T cxb, T om)
+                                        T cxma, T cxpb, T cxmb, T c1)
 {
     constexpr int ks = k > 0 ? k - 1 : 0, kn = k < NRW - 1 ? k + 1 : 0;
     if constexpr ((k & 1) == Q) {
@@ -104,21 +130,19 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
         const T pN = k == NRW - 1 ? nva : R.pa[kn];
         const T pE = R.pb[k], pW = from_west(R.pb[k]);
         if constexpr (sizeof(T) == 4)  // slot a: odd column, west neighbour in lane l-1
-            R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], cxma, pW, cxpa, pE, pN, pS, R.cyp[k], R.cym[k], R.wa[k],
-                                 1.0f - om);
+            R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], cxma, pW, cxpa, pE, pN, pS, R.cyp[k], R.cym[k], R.wa[k], c1);
         else
-            R.pa[k] = reg_cell<T, FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, cxa, R.cyp[k],
-                                        R.cym[k], R.cy[k], om);
+            R.pa[k] = reg_cell<FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, R.cyp[k], R.cym[k],
+                                     R.wa[k], c1);
     } else {
         const T pS = k == 0 ? svb : R.pb[ks];
         const T pN = k == NRW - 1 ? nvb : R.pb[kn];
         const T pE = from_east(R.pa[k]), pW = R.pa[k];
         if constexpr (sizeof(T) == 4)  // slot b: even column, east neighbour in lane l+1
-            R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], cxpb, pE, cxmb, pW, pN, pS, R.cyp[k], R.cym[k], R.wb[k],
-                                 1.0f - om);
+            R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], cxpb, pE, cxmb, pW, pN, pS, R.cyp[k], R.cym[k], R.wb[k], c1);
         else
-            R.pb[k] = reg_cell<T, FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, cxb, R.cyp[k],
-                                        R.cym[k], R.cy[k], om);
+            R.pb[k] = reg_cell<FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, R.cyp[k], R.cym[k],
+                                     R.wb[k], c1);
     }
 }
 
@@ -129,11 +153,11 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
 template <typename T, int NRW, int Q, bool ALLFAST>
 __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl)[64], int lane,
                                          unsigned smask, T sva, T svb, T nva, T nvb,
-                                         T cxpa, T cxma, T cxa, T cxpb, T cxmb, T cxb, T om)
+                                         T cxpa, T cxma, T cxpb, T cxmb, T c1)
 {
     if constexpr (ALLFAST) {
         [&]<int... K>(std::integer_sequence<int, K...>) {
-            (reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb, cxb, om), ...);
+            (reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxpb, cxmb, c1), ...);
         }(std::make_integer_sequence<int, NRW>{});
     } else {
         int f[NRW];  // flags of every row, read in one batch
@@ -142,11 +166,9 @@ __device__ __forceinline__ void reg_half(RegTile<T, NRW> &R, const uint16_t (*fl
         [&]<int... K>(std::integer_sequence<int, K...>) {
             (([&] {
                  if ((smask >> K) & 1u)
-                     reg_row<T, NRW, Q, false, K>(R, f[K], sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb,
-                                                  cxb, om);
+                     reg_row<T, NRW, Q, false, K>(R, f[K], sva, svb, nva, nvb, cxpa, cxma, cxpb, cxmb, c1);
                  else
-                     reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxa, cxpb, cxmb, cxb,
-                                                 om);
+                     reg_row<T, NRW, Q, true, K>(R, 0, sva, svb, nva, nvb, cxpa, cxma, cxpb, cxmb, c1);
              }()),
              ...);
         }(std::make_integer_sequence<int, NRW>{});
@@ -278,9 +300,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     const bool ina = ga >= 1 && ga <= nx - 2;
     const bool inb = ga + 1 >= 1 && ga + 1 <= nx - 2;
     // x coefficients of the fast path: zero on the domain boundary faces.
-    // A slot outside the interior gets cx = 1, so the fast path computes a
-    // finite value there (never stored, and only ever multiplied by the
-    // zero coefficient of a boundary face).
+    // A slot outside the interior gets cx = 1 (FP32: a finite weight; FP64:
+    // weight 0), so the fast path computes a finite value there (never
+    // stored, and only ever multiplied by the zero coefficient of a
+    // boundary face).
     const T cxpa = (ina && ga != nx - 2) ? G(cxp_)[ga] : (T)0;
     const T cxma = (ina && ga != 1) ? G(cxm_)[ga] : (T)0;
     const T cxpb = (inb && ga + 1 != nx - 2) ? G(cxp_)[ga + 1] : (T)0;
@@ -370,9 +393,9 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
         if constexpr (sizeof(T) == 8) sfl[w][k][lane] = (uint16_t)f;
         const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
         const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
-        R.cyp[k] = yp;
-        R.cym[k] = ym;
-        R.cy[k] = rowint ? yp + ym : (T)1;
+        R.cyp[k] = uniform(yp);
+        R.cym[k] = uniform(ym);
+        const T cy = rowint ? yp + ym : (T)1;
         bool offfast = false;
         if (rowint) {
             const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
@@ -387,8 +410,8 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 R.wb[k] = 0.0f;
             } else if (!offfast) {
                 // fast-path cells: sa = (cxp + cxm) + (cyp + cym)
-                R.wa[k] = sor_weight(cxa + R.cy[k], om);
-                R.wb[k] = sor_weight(cxb + R.cy[k], om);
+                R.wa[k] = sor_weight(cxa + cy, om);
+                R.wb[k] = sor_weight(cxb + cy, om);
             } else {
                 // sa over the open faces; the same bits as above for a
                 // fast-path cell
@@ -404,8 +427,19 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             R.rb[k] = R.wb[k] * R.rb[k];
         } else {
             if (offfast) smask |= 1u << k;
+            // the weight om / sa over the open faces (for a fast-path cell the
+            // same bits as om / (cx + cy)); 0 for a cell the sweep keeps:
+            // outside the interior, not fluid, or without an open face
+            const int fa = f & 0xff, fb = f >> 8;
+            const T sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
+                           (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
+            const T sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
+                           (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
+            R.wa[k] = (rowint && (fa & F_FLUID) && sa_a != (T)0) ? sor_weight64(sa_a, om) : (T)0;
+            R.wb[k] = (rowint && (fb & F_FLUID) && sa_b != (T)0) ? sor_weight64(sa_b, om) : (T)0;
         }
     }
+    const T c1 = (T)1 - om;
 
     const int Z = (c0 + jt0) & 1;  // parity of slot a in the wave's first row
     // Half-sweep h has colour h & 1, and slot a of row k has parity Z + k,
@@ -422,11 +456,11 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
         if (w > 0) sv = xch[b][w - 1][1][lane];
         if (w < NW - 1) nv = xch[b][w + 1][0][lane];
         [&]<int... K>(std::integer_sequence<int, K...>) {
-            (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxa, cxpb, cxmb, cxb, om),
+            (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1),
              ...);
         }(std::make_integer_sequence<int, NRW - 2>{});
-        reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, cxpb, cxmb, cxb, om);
-        reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxa, cxpb, cxmb, cxb, om);
+        reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
+        reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
         xch[b ^ 1][w][0][lane] = V2{R.pa[0], R.pb[0]};
         xch[b ^ 1][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         __syncthreads();
@@ -466,7 +500,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 if (w < NW - 1) nv = xch[b][w + 1][0][lane];
 #define IFF_HALF(Q_, F_)                                                                              \
     reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
-                             cxa, cxpb, cxmb, cxb, om)
+                             cxpb, cxmb, c1)
                 if (smask == 0) {
                     if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
                 } else {
@@ -510,6 +544,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             if (qx >= 0 && qx < ntxs && qy >= 0 && qy < ntys) nb = t - tl.y + qy * ntxs + qx;
         }
         const int nrec = P * ny * (int)sizeof(T);  // bytes of a field
+        // buffer offsets: the lane's column in the vector offset, the row
+        // (wave-uniform) in the scalar offset; every access below has
+        // ga >= 0 and 1 <= gj <= ny - 2, so both are non-negative
+        const int voff = ga * (int)sizeof(T);
         // the chunk's two buffers: stage l writes buffer (l + 1) & 1
         void *buf0, *buf1;
         if constexpr (sizeof(T) == 8) {
@@ -519,8 +557,11 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             buf0 = c.e[0];
             buf1 = c.e[1];
         }
-        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(buf0, 0, nrec, kRsrc3);
-        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(buf1, 0, nrec, kRsrc3);
+        // the descriptors are built from wave-uniform (readfirstlane)
+        // pointers: held in scalar registers, with no per-lane loop around
+        // the buffer accesses
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf0), 0, nrec, kRsrc3);
+        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf1), 0, nrec, kRsrc3);
         for (int l = 0; l < nexec; l++) {
             int me = kexec - l * M;
             if (me > M) me = M;
@@ -537,19 +578,20 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                     const int gj = jt0 + k;
                     if (rr < H || rr >= H + TY || gj > ny - 2) continue;
                     if (!last && !lb && rr >= 2 * H && rr < TY) continue;
-                    const int off = (gj * P + ga) * (int)sizeof(T);
+                    const int srow = gj * P * (int)sizeof(T);  // uniform: scalar offset
                     if constexpr (sizeof(T) == 4) {
                         if (ga + 1 <= nx - 2)
                             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro,
-                                                                  off, 0, kDev);
+                                                                  voff, srow, kDev);
                         else if (ga <= nx - 2)
-                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, off, 0, kDev);
+                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, voff, srow, kDev);
                     } else {
                         if (ga + 1 <= nx - 2)
                             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, V2{R.pa[k], R.pb[k]}), ro,
-                                                                   off, 0, kDev);
+                                                                   voff, srow, kDev);
                         else if (ga <= nx - 2)
-                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, R.pa[k]), ro, off, 0, kDev);
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, R.pa[k]), ro, voff, srow,
+                                                                  kDev);
                     }
                 }
             }
@@ -566,7 +608,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             // failure and the solve resumes from the current pressure
             const int bad =
                 threadIdx.x == 0 ? (int)__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            if (__syncthreads_or(bad)) break;
+            if (__builtin_amdgcn_readfirstlane(__syncthreads_or(bad))) break;
             // halo ring: the cells of the tile outside its stored region
             // that lie in the domain's interior (device-scope loads)
             const bool lh = lane < H / 2 || lane >= 64 - H / 2;
@@ -576,26 +618,31 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 const int gj = jt0 + k;
                 if (gj < 1 || gj > ny - 2) continue;
                 if (!lh && rr >= H && rr < H + TY) continue;
-                const int off = (gj * P + ga) * (int)sizeof(T);
+                const int srow = gj * P * (int)sizeof(T);
                 if constexpr (sizeof(T) == 4) {
                     if (ina && inb) {
-                        const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
+                        const V2 v =
+                            __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
                         R.pa[k] = v[0];
                         R.pb[k] = v[1];
                     } else {
-                        if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off, 0, kDev));
+                        if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff, srow, kDev));
                         if (inb)
-                            R.pb[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, off + 4, 0, kDev));
+                            R.pb[k] =
+                                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff + 4, srow, kDev));
                     }
                 } else {
                     if (ina && inb) {
-                        const V2 v = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(ro, off, 0, kDev));
+                        const V2 v =
+                            __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(ro, voff, srow, kDev));
                         R.pa[k] = v[0];
                         R.pb[k] = v[1];
                     } else {
-                        if (ina) R.pa[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, off, 0, kDev));
+                        if (ina)
+                            R.pa[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
                         if (inb)
-                            R.pb[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, off + 8, 0, kDev));
+                            R.pb[k] =
+                                __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff + 8, srow, kDev));
                     }
                 }
             }
@@ -648,7 +695,7 @@ static void pers_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int n
 // rows per wave), FP32 and FP64
 #define IFF_PERS_SHAPES32(X) \
     X(float, 16, 8, 96, 8) X(float, 16, 8, 64, 6) X(float, 16, 4, 48, 4) X(float, 8, 8, 32, 8) X(float, 8, 4, 32, 6)
-#define IFF_PERS_SHAPES64(X) X(double, 16, 4, 80, 6) X(double, 16, 4, 48, 4)
+#define IFF_PERS_SHAPES64(X) X(double, 16, 4, 80, 6) X(double, 16, 4, 64, 5) X(double, 16, 4, 48, 4)
 
 int pers_blocks(int prec, int waves, int ty, int m)
 {
@@ -705,7 +752,8 @@ bool reg_variant_ok(int prec, int waves, int ty, int m)
         return (m == 2 && (ty == 24 || ty == 40 || ty == 56)) || (m == 4 && (ty == 32 || ty == 48 || ty == 64)) ||
                (m == 8 && (ty == 32 || ty == 48 || ty == 64));
     if (waves == 16)
-        return (m == 4 && (ty == 48 || ty == 80)) || (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64)));
+        return (m == 4 && (ty == 48 || ty == 64 || ty == 80)) ||
+               (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64)));
     return false;
 }
 
@@ -726,7 +774,7 @@ void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, co
     IFF_RG(T, 8, 2, 24) IFF_RG(T, 8, 2, 40) IFF_RG(T, 8, 2, 56)                           \
     IFF_RG(T, 8, 4, 32) IFF_RG(T, 8, 4, 48) IFF_RG(T, 8, 4, 64)                           \
     IFF_RG(T, 8, 8, 32) IFF_RG(T, 8, 8, 48) IFF_RG(T, 8, 8, 64)                           \
-    IFF_RG(T, 16, 4, 48) IFF_RG(T, 16, 4, 80) IFF_RG(T, 16, 8, 32) IFF_RG(T, 16, 8, 64)
+    IFF_RG(T, 16, 4, 48) IFF_RG(T, 16, 4, 64) IFF_RG(T, 16, 4, 80) IFF_RG(T, 16, 8, 32) IFF_RG(T, 16, 8, 64)
     if (prec == IF_FP64) {
         IFF_RG_ALL(double)
     } else {

@@ -237,15 +237,14 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * registers), tile height 16/32/48 rows, 1, 2 or 4 sweeps per launch;
  * kernel 3: the same with eight wave64s per tile, tile height 24/40/56
  * (2 sweeps per launch), 32/48/64 (4) or 32/48/64 (8); kernel 4: sixteen
- * wave64s per tile, tile height 48/80 (4 sweeps per launch) or 32/64/96
+ * wave64s per tile, tile height 48/64/80 (4 sweeps per launch) or 32/64/96
  * (8; 96 in mixed precision only).  Several red-black sweeps per launch is
  * temporal blocking; check_every is the number of sweeps between
  * convergence checks.  Results do not depend on the kernel, the tile
  * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps
- * (mixed); FP64 (until tuned): kernel 4, 80 rows, 4 sweeps as persistent
- * chunks when the running cases' tiles at that shape fill at least 3/4 of
- * the device in one launch, else kernel 4, 48 rows, 4 sweeps as persistent
- * chunks when they fit, else kernel 3, 32 rows, 4 sweeps. */
+ * (mixed); FP64 (until tuned): kernel 4, 4 sweeps as persistent chunks of
+ * the smallest tile height of 48, 64 and 80 rows whose tiles fit on the
+ * device in one launch, else kernel 3, 32 rows, 4 sweeps. */
 if_status if_case_tune(if_case *c, int kernel, int tile_y, int sweeps_per_launch,
                        int check_every);
 /* Launch mode of the Poisson sweeps (no counterpart in the reference,

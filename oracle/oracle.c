@@ -987,6 +987,10 @@ static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
         line_sor_sweep(c, nx, ny, flags, rhs, om, p);
         return;
     }
+    /* p' = p + om (gs - p), gs = (rhs - sum) / ap, evaluated as
+     * (1 - om) p + w (sum - rhs) with the cell's weight w = om / sa,
+     * sa = -ap (the form whose weight a sweep kernel can hold per launch) */
+    const double c1 = 1.0 - om;
     for (int colour = 0; colour < 2; colour++) {
 #pragma omp parallel for schedule(static)
         for (int j = 1; j < ny - 1; j++)
@@ -997,8 +1001,8 @@ static void ppe_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
                 double ap, sum;
                 ppe_cell(c, nx, i, j, f, p, &ap, &sum);
                 if (ap == 0.0) continue;
-                const double gs = (rhs[id] - sum) / ap;
-                p[id] = p[id] + om * (gs - p[id]);
+                const double w = om / (-ap);
+                p[id] = c1 * p[id] + w * (sum - rhs[id]);
             }
     }
 }

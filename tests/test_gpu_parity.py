@@ -421,7 +421,7 @@ def test_autotune_leaves_state_and_matches_manual_tune(require_gpu, prec):
     kw = dict(precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8, ppe_itermax=200000,
               ppe_tol=1e-9, check_every=10)
     a, b = iff.Case(xf, yf, **kw), iff.Case(xf, yf, **kw)
-    ncand = 12 if prec == iff.IF_FP64 else 13   # the 16-wave x 8-row tile is FP32 only
+    ncand = 13 if prec == iff.IF_FP64 else 14   # the 16-wave x 96-row tile is FP32 only
     r0 = a.autotune()                       # before any step: interval unchanged
     assert r0["typical_sweeps"] == 0 and r0["check_every"] == 10 and r0["candidates"] == ncand
     a.step(2)
@@ -455,7 +455,7 @@ def test_batch_autotune(require_gpu):
         bt.step(1)
         bs.append((bt, sl))
     r = bs[0][0].autotune()
-    assert r["candidates"] == 13 and r["typical_sweeps"] > 0
+    assert r["candidates"] == 14 and r["typical_sweeps"] > 0
     bs[1][0].tune(r["kernel"], r["tile_y"], r["sweeps_per_launch"], r["check_every"])
     for bt, _ in bs:
         bt.step(2)
