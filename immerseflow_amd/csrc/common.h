@@ -131,6 +131,23 @@ struct Monitor {
     unsigned long long *count;         // records written since the last read
 };
 
+// Line solve partition of a row of n interior cells: S = ceil(n / B) <= 64
+// blocks of B = max(2, ceil(n / 64)) cells; the last cell of every block
+// but the last is a separator.  A row's factors take line_row_stride(B)
+// elements: five planes of B x 64 per-cell values and 13 x 64 for the
+// separators' cyclic reduction.  The sweep kernel stages a row in B x 65
+// LDS elements, which bounds B (and so n <= 64 kLineMaxB).
+constexpr int kLinePlanes = 5, kLinePcrLevels = 6, kLinePcr = 2 * kLinePcrLevels + 1;
+constexpr int kLineMaxB = 126;
+__host__ __device__ inline void line_partition(int n, int *B, int *S)
+{
+    int b = (n + 63) / 64;
+    if (b < 2) b = 2;
+    *B = b;
+    *S = (n + b - 1) / b;
+}
+__host__ __device__ inline size_t line_row_stride(int B) { return (size_t)(kLinePlanes * B + kLinePcr) * 64; }
+
 struct CaseDev {
     int nx, ny, P;
     int ntx, nty;          // tiles of the interior (kTileX x tile_y)
@@ -152,7 +169,8 @@ struct CaseDev {
     double *uf[2], *vf[2];         // face velocities: [0] = U^n, [1] = U*
     double *p[2], *rhs;
     float *e[2], *r32;             // mixed Poisson error / residual
-    void *lm, *ly;                 // line solve: pivots m (Poisson precision) and, FP32, 1/m; line cases only
+    void *lf;                      // line solve factors (Poisson precision), line_row_stride(lB) per row; line cases only
+    int lB, lS;                    // line solve partition of a row: S blocks of B cells
     uint8_t *flags;
     int8_t *bodyid;
     GhostDev *gh;

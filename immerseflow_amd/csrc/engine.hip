@@ -120,7 +120,7 @@ struct ArenaPlan {
     size_t off_u[2], off_v[2], off_sx, off_sy;
     size_t off_uf[2], off_vf[2], off_p[2], off_rhs;
     size_t off_e[2], off_r32;
-    size_t off_lm, off_ly;
+    size_t off_lf;
     size_t off_flags, off_bodyid;
     size_t off_metd, off_xc, off_yc, off_metf;
     size_t off_bodies, off_poly, off_part, off_gcount;
@@ -152,9 +152,11 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
     }
-    if (line) {  // the line solve's pivots (and, for FP32 sweeps, their reciprocals)
-        a.off_lm = fieldT(prec == IF_FP64 ? 8 : 4);
-        if (prec != IF_FP64) a.off_ly = fieldT(4);
+    if (line) {  // the line solve's factors, one stride per row, in the Poisson precision
+        int B, S;
+        line_partition(nx - 2, &B, &S);
+        a.off_lf = o;
+        o = align256(o + (prec == IF_FP64 ? 8 : 4) * line_row_stride(B) * (size_t)ny);
     }
     a.off_flags = fieldT(1);
     a.off_bodyid = fieldT(1);
@@ -276,6 +278,9 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
         return fail(IF_ERR_ARG, "case: bad body list");
     if (d.ppe_solver != IF_PPE_JACOBI && d.ppe_solver != IF_PPE_RBSOR && d.ppe_solver != IF_PPE_LINE)
         return fail(IF_ERR_ARG, "case: unknown Poisson solver");
+    if (d.ppe_solver == IF_PPE_LINE && d.nxf - 1 > 64 * kLineMaxB)
+        return fail(IF_ERR_ARG, "case: line SOR takes at most " + std::to_string(64 * kLineMaxB) +
+                                    " interior cells per row");
     if (d.ad_itermax < 0 || d.ppe_itermax < 0) return fail(IF_ERR_ARG, "case: negative itermax");
     s.desc = d;
     s.prec = prec_;
@@ -378,8 +383,8 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.rhs = (double *)(A + a.off_rhs);
     if (pprec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
     if (d.ppe_solver == IF_PPE_LINE) {
-        c.lm = A + a.off_lm;
-        if (pprec_ != IF_FP64) c.ly = A + a.off_ly;
+        c.lf = A + a.off_lf;
+        line_partition(s.nx - 2, &c.lB, &c.lS);
     }
     c.flags = (uint8_t *)(A + a.off_flags);
     c.bodyid = (int8_t *)(A + a.off_bodyid);

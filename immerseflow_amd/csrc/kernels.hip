@@ -631,97 +631,170 @@ __global__ __launch_bounds__(256) void k_jacobi(const CaseDev *cs, const int2 *t
 // colour of rows per launch: rows j odd (colour 1) first, then rows j even.
 // Each row's tridiagonal system
 //     -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs
-// (fluid cells with sa != 0; identity rows elsewhere) is solved by the
-// Thomas algorithm and relaxed, p + om (x - p).  FP64 is the oracle's
-// arithmetic (line_sor_sweep, oracle.c), to the bit.
+// (fluid cells with sa != 0; identity rows elsewhere) is solved and
+// relaxed, p + om (x - p).  FP64 is the oracle's arithmetic (line_sor_sweep,
+// oracle.c), to the bit.
 //
-// The elimination's pivots m[i] = b - l c'[i-1] and c'[i] = u / m[i] depend
-// on the cell flags and metric coefficients only: k_line_factor forms them
-// once per step (FP32 error sweeps: with y = 1/m from the hardware
-// reciprocal, c' = u y).  A sweep then runs only the d' recurrence,
-// d'[i] = (d - l d'[i-1]) / m[i] (FP32: times y), and the back
-// substitution.
-//
-// The recurrence is serial along a row, so a row costs the latency of its
-// dependence chain.  A block owns 64 rows of one colour and walks them in
-// chunks of kLineW columns staged in LDS, one lane per row: wave 0 runs the
-// recurrence on chunk t-1 while the "mover" waves stage chunk t with
-// coalesced row loads (issued a stage ahead, so they are in flight across
-// the barrier) and write chunk t-2's d' back; two LDS buffers, one barrier
-// per stage.  c' and d' live in the case's predictor source arrays (free
-// during the Poisson phase), m (and y) in the case's line-solve arrays.  The
-// back substitution stages c', d' right to left, wave 0 turns d' into x in
-// place, and the movers relax and store.
-// columns per chunk (a multiple of 8): 64 for FP32 cells, 32 for FP64
-// (three double-buffered chunk arrays must fit the LDS)
-template <typename T> constexpr int kLineW = sizeof(T) == 4 ? 64 : 32;
-constexpr int kLineMovers = 8;                      // mover waves per block
-constexpr int kLineThreads = 64 * (1 + kLineMovers);
-constexpr unsigned kLineSolved = 0x80;              // staged flag: sa != 0 fluid cell
+// One wave per row, one lane per segment of the row's partition
+// (line_partition: S <= 64 blocks of B cells, the last cell of each block
+// but the last a separator): a lane runs its segment's Thomas recurrences,
+// the separators' Schur complement is solved by cyclic reduction across
+// the lanes (shuffles), and each segment adds its separators' share.  The
+// factors depend on the flags and the metric only: k_line_factor forms
+// them once per step, in the segment layout [plane][t][lane] that a
+// lane-per-segment loop reads coalesced.  The sweep kernel stages the
+// row's right side into LDS in that layout (coalesced row loads, a
+// transposing store), runs the lanes' recurrences there, and relaxes and
+// stores the row with coalesced accesses.
+template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
 
-// The per-step factors of every interior row of every line-solve case: m,
-// c' (into the predictor source array sx) and, for FP32, y = 1/m.  One
-// lane per row; it runs once per step.
+enum { kLfR, kLfNlr, kLfNcp, kLfH, kLfQ };  // per-cell factor planes (kLinePlanes)
+
+template <typename T> struct LineMetric {
+    const T *xp, *xm, *yp, *ym;
+    __device__ LineMetric(const CaseDev &c)
+    {
+        if constexpr (sizeof(T) == 8) {
+            xp = c.x.cp; xm = c.x.cm; yp = c.y.cp; ym = c.y.cm;
+        } else {
+            xp = c.xf.cp; xm = c.xf.cm; yp = c.yf.cp; ym = c.yf.cm;
+        }
+    }
+    // row coefficients of a cell (x metric xpi, xmi of its column): l = -aW,
+    // b = sa, u = -aE when solved
+    __device__ static bool cell(unsigned f, T xpi, T xmi, T yN, T yS, T &l, T &b, T &u)
+    {
+        l = (T)0;
+        b = (T)1;
+        u = (T)0;
+        if (!(f & F_FLUID)) return false;
+        const T aE = (f & F_OPEN_E) ? xpi : (T)0;
+        const T aW = (f & F_OPEN_W) ? xmi : (T)0;
+        const T aN = (f & F_OPEN_N) ? yN : (T)0;
+        const T aS = (f & F_OPEN_S) ? yS : (T)0;
+        const T sa = (aE + aW) + (aN + aS);
+        if (sa == (T)0) return false;
+        l = -aW;
+        b = sa;
+        u = -aE;
+        return true;
+    }
+};
+
+template <typename T> __device__ __forceinline__ T shfl_up_t(T v, int d) { return __shfl_up(v, (unsigned)d, 64); }
+template <typename T> __device__ __forceinline__ T shfl_down_t(T v, int d) { return __shfl_down(v, (unsigned)d, 64); }
+
+// The factors of every interior row of every line-solve case (once per
+// step): lane k forms segment k's pivots r = 1/den, -l r, -u r and spikes
+// h, q; lanes 0..m-1 (m = S - 1) the separators' -l, -u and the cyclic
+// reduction's multipliers and final reciprocals.  Operation for operation
+// the oracle's line_factor_row.
 template <typename T>
 __global__ __launch_bounds__(64) void k_line_factor(const CaseDev *cs)
 {
     const CaseDev &c = cs[blockIdx.y];
     if (c.ppe_solver != IF_PPE_LINE || !c.ctrl->active) return;
-    const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = 1 + (int)blockIdx.x;
     if (j > c.ny - 2) return;
-    const T *cxp, *cxm, *cyp, *cym;
-    if constexpr (sizeof(T) == 8) {
-        cxp = c.x.cp; cxm = c.x.cm; cyp = c.y.cp; cym = c.y.cm;
-    } else {
-        cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
-    }
+    const int B = c.lB, S = c.lS, n = c.nx - 2, m = S - 1, k = (int)threadIdx.x;
     const size_t row = (size_t)j * c.P;
+    const LineMetric<T> mt(c);
+    const T yN = G(mt.yp)[j], yS = G(mt.ym)[j];
     const auto fl = G(c.flags) + row;
-    const auto gxp = G(cxp), gxm = G(cxm);
-    const auto cp = GW(fld<T>(c.sx) + row), pm = GW(fld<T>(c.lm) + row);
-    const T yp = G(cyp)[j], ym = G(cym)[j];
-    const int n = c.nx - 2;
-    T cprev = (T)0;
-    for (int i = 1; i <= n; i++) {
-        const unsigned f = fl[i];
-        T l = (T)0, b = (T)1, u = (T)0;
-        if (f & F_FLUID) {
-            const T aE = (f & F_OPEN_E) ? gxp[i] : (T)0;
-            const T aW = (f & F_OPEN_W) ? gxm[i] : (T)0;
-            const T aN = (f & F_OPEN_N) ? yp : (T)0;
-            const T aS = (f & F_OPEN_S) ? ym : (T)0;
-            const T sa = (aE + aW) + (aN + aS);
-            if (sa != (T)0) {
-                l = -aW;
-                b = sa;
-                u = -aE;
+    const auto F = GW(fld<T>(c.lf) + (size_t)j * line_row_stride(B));
+    auto at = [&](int pl, int t) { return ((size_t)pl * B + t) * 64 + k; };
+    const int len = k < S - 1 ? B - 1 : (k == S - 1 ? n - (S - 1) * B : 0);
+    T hlast = (T)0, qlast = (T)0, h0 = (T)0, q0 = (T)0;
+    if (len > 0) {
+        const int a = 1 + k * B;
+        T cpv = (T)0, hpv = (T)0;
+        for (int t = 0; t < len; t++) {
+            T l, b, u;
+            mt.cell(fl[a + t], G(mt.xp)[a + t], G(mt.xm)[a + t], yN, yS, l, b, u);
+            const T den = t == 0 ? b : b - l * cpv;
+            const T r = (T)1 / den, cp = u * r, nlr = -(l * r);
+            const T hp = t == 0 ? nlr : nlr * hpv;
+            F[at(kLfR, t)] = r;
+            F[at(kLfNlr, t)] = nlr;
+            F[at(kLfNcp, t)] = -cp;
+            cpv = cp;
+            hpv = hp;
+            if (t == len - 1) {
+                hlast = hp;
+                qlast = -cp;
+            } else {
+                F[at(kLfH, t)] = hp;
             }
         }
-        const T m = (i == 1) ? b : b - l * cprev;
-        T cpi;
-        if constexpr (sizeof(T) == 8) {
-            cpi = u / m;
-        } else {
-            const T y = __builtin_amdgcn_rcpf(m);
-            cpi = u * y;
-            GW(fld<T>(c.ly) + row)[i] = y;
+        // backward: h = fma(-u r, h', hp), q = (-u r) q' (this lane's stores
+        // above are read back by this lane only)
+        F[at(kLfH, len - 1)] = hlast;
+        F[at(kLfQ, len - 1)] = qlast;
+        T hx = hlast, qx = qlast;
+        for (int t = len - 2; t >= 0; t--) {
+            const T ncp = F[at(kLfNcp, t)];
+            hx = fmaT(ncp, hx, F[at(kLfH, t)]);
+            qx = ncp * qx;
+            F[at(kLfH, t)] = hx;
+            F[at(kLfQ, t)] = qx;
         }
-        cp[i] = cpi;
-        pm[i] = m;
-        cprev = cpi;
+        h0 = hx;
+        q0 = qx;
     }
+    // separators: the Schur complement's rows A x[k-1] + Bv x[k] + Cv x[k+1]
+    const T hR = shfl_down_t(h0, 1), qR = shfl_down_t(q0, 1);
+    T A = (T)0, Bv = (T)1, Cv = (T)0;
+    if (k < m) {
+        const int sp = (k + 1) * B;
+        T l, b, u;
+        mt.cell(fl[sp], G(mt.xp)[sp], G(mt.xm)[sp], yN, yS, l, b, u);
+        A = l * hlast;
+        Cv = u * qR;
+        Bv = (b + l * qlast) + u * hR;
+        F[at(kLfR, B - 1)] = -l;
+        F[at(kLfNlr, B - 1)] = -u;
+    }
+    // cyclic reduction with strides 1, 2, 4, ... < m
+    const auto pcr = F + (size_t)kLinePlanes * B * 64;
+    int lev = 0;
+    for (int sg = 1; sg < m; sg *= 2, lev++) {
+        const T Am_ = shfl_up_t(A, sg), Cm_ = shfl_up_t(Cv, sg), Bm_ = shfl_up_t(Bv, sg);
+        const T Ap_ = shfl_down_t(A, sg), Cp_ = shfl_down_t(Cv, sg), Bp_ = shfl_down_t(Bv, sg);
+        const bool lo = k >= sg, hi = k + sg < m;
+        T alpha = (T)0, gamma = (T)0;
+        if (k < m) {
+            alpha = lo ? -(A / Bm_) : (T)0;
+            gamma = hi ? -(Cv / Bp_) : (T)0;
+            const T Am = lo ? Am_ : (T)0, Cm = lo ? Cm_ : (T)0;
+            const T Ap = hi ? Ap_ : (T)0, Cp = hi ? Cp_ : (T)0;
+            const T nA = alpha * Am, nC = gamma * Cp, nB = (Bv + alpha * Cm) + gamma * Ap;
+            A = nA;
+            Cv = nC;
+            Bv = nB;
+        }
+        pcr[(2 * lev) * 64 + k] = alpha;
+        pcr[(2 * lev + 1) * 64 + k] = gamma;
+    }
+    pcr[(2 * kLinePcrLevels) * 64 + k] = k < m ? (T)1 / Bv : (T)0;
 }
 
-template <typename T> struct LineStage {
-    T a[2][64][kLineW<T> + 1];        // forward: y = 1/m (FP32); backward: c'
-    T d[2][64][kLineW<T> + 1];        // forward: d, then d'; backward: d', then x
-    T m[2][64][kLineW<T> + 1];        // forward: the pivots m
-    uint8_t f[2][64][kLineW<T> + 4];  // forward: cell flags | kLineSolved
-    T yp[64], ym[64];                 // the rows' y coefficients
-};
-
+// One colour of one sweep: one wave per row (blocks of 64 threads), LDS of
+// B x 65 elements (the row's values in the segment layout, padded so that
+// the transposing store and the lanes' column walks are conflict-free).
+// A colour has few rows (a wave or two per CU), so the kernel is bound by
+// the latency of its dependent steps: the row passes load kLineU cells per
+// lane at once, and the lanes' factor loads run a group of 8 cells ahead
+// of the recurrences (two register buffers), with the first groups of the
+// later passes and the reduction's multipliers loaded at the start.
+constexpr int kLineU = 8;
 template <typename T>
-__global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
+__global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
 {
     const CaseDev &c = cs[blockIdx.y];
     const Ctrl *ct = c.ctrl;
@@ -730,260 +803,259 @@ __global__ __launch_bounds__(kLineThreads) void k_line_sor(const CaseDev *cs, in
     if (kexec > K) kexec = K;
     if (s >= kexec) return;
     const int nx = c.nx, ny = c.ny, P = c.P;
-    const int jlast = ny - 2;
-    const int j0 = 2 - colour + 128 * (int)blockIdx.x;  // local row r is j0 + 2 r
-    if (j0 > jlast) return;
+    const int j = 2 - colour + 2 * (int)blockIdx.x;
+    if (j > ny - 2) return;
     const T *src;
     T *dst;
     const T *rhs;
-    const T *cxp, *cxm, *cyp, *cym;
     bool zero_src = false;
     T om;
     if constexpr (sizeof(T) == 8) {
         src = c.p[(ct->ppe_base + s) & 1];
         dst = c.p[(ct->ppe_base + s + 1) & 1];
         rhs = c.rhs;
-        cxp = c.x.cp; cxm = c.x.cm; cyp = c.y.cp; cym = c.y.cm;
         om = c.omega;
     } else {
         src = c.e[s & 1];
         dst = c.e[(s + 1) & 1];
         zero_src = (s == 0);
         rhs = c.r32;
-        cxp = c.xf.cp; cxm = c.xf.cm; cyp = c.yf.cp; cym = c.yf.cm;
         om = c.omega_f;
     }
     const bool zero_nbr = colour == 1 && zero_src;
-    // global address space (the case descriptor's pointers are generic)
-    const auto gsrc = G(src), gnbr = G(colour == 1 ? src : dst), grhs = G(rhs);  // nbr: rows above and below
-    const auto gcxp = G(cxp), gcxm = G(cxm), gcyp = G(cyp), gcym = G(cym);
-    const auto gflags = G(c.flags);
-    const auto gdst = GW(dst), gcpv = GW(fld<T>(c.sx)), gdpv = GW(fld<T>(c.sy));
-    const auto gm = G((const T *)c.lm), gy = G((const T *)c.ly);  // gy: FP32 only
+    // rows above and below: the old values (colour 1) or this sweep's (colour 0)
+    const auto gsrc = G(src), gnbr = G(colour == 1 ? src : dst), grhs = G(rhs);
+    const auto gdst = GW(dst);
+    const LineMetric<T> mt(c);
+    const auto gxp = G(mt.xp), gxm = G(mt.xm);
+    const T yN = G(mt.yp)[j], yS = G(mt.ym)[j];
+    const auto gfl = G(c.flags);
+    const int B = c.lB, S = c.lS, n = nx - 2, m = S - 1;
+    const int lane = (int)threadIdx.x;
+    const int len = lane < S - 1 ? B - 1 : (lane == S - 1 ? n - (S - 1) * B : 0);
+    const auto F = G((const T *)fld<T>(c.lf) + (size_t)j * line_row_stride(B));
+    auto fat = [&](int pl, int t) { return F[((size_t)pl * B + t) * 64 + lane]; };
+    const auto pcr = F + (size_t)kLinePlanes * B * 64;
+    extern __shared__ __align__(16) unsigned char line_lds[];
+    T *g = (T *)line_lds;  // [t * 65 + segment]
+    // segment and position of cell i: (i - 1) / B by a multiply-high
+    // (exact for i - 1 < 2^32 / B), the remainder by subtraction
+    const unsigned mg = 0xffffffffu / (unsigned)B + 1u;
+    auto slot = [&](int i) {
+        const unsigned x = (unsigned)(i - 1), kk = __umulhi(x, mg);
+        return (int)((x - kk * (unsigned)B) * 65u + kk);
+    };
+    const int tl = len > 0 ? len - 1 : 0;  // clamp for the lanes' loads
 
-    __shared__ LineStage<T> st;
-    constexpr int kW = kLineW<T>;
-    const int n = nx - 2;  // a row's cells are i = 1..n
-    const int nch = (n + kW - 1) / kW;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr int kMv = 64 * kLineMovers;
-    constexpr int kPer = 64 * kW / kMv;  // cells per mover thread and chunk
-    static_assert(kMv % kW == 0 && (64 * kW) % kMv == 0 && kW % 8 == 0, "line chunk shape");
-    // mover thread mt owns column mt % kW of every chunk and local rows
-    // mt / kW + q kMv / kW (q < kPer)
-    const int mt = (int)threadIdx.x - 64;
-    const int mcol = mt % kW, mrow = mt / kW;
-    if (threadIdx.x < 64) {
-        const int j = min(j0 + 2 * (int)threadIdx.x, jlast);
-        st.yp[threadIdx.x] = gcyp[j];
-        st.ym[threadIdx.x] = gcym[j];
+    // data-independent loads, issued first: the reduction's multipliers,
+    // the separator's -l, -u and the first groups of the backward and
+    // final passes
+    T pal[kLinePcrLevels], pga[kLinePcrLevels];
+#pragma unroll
+    for (int q = 0; q < kLinePcrLevels; q++) {
+        pal[q] = pcr[(2 * q) * 64 + lane];
+        pga[q] = pcr[(2 * q + 1) * 64 + lane];
+    }
+    const T prb = pcr[(2 * kLinePcrLevels) * 64 + lane];
+    const T snl = fat(kLfR, B - 1), snu = fat(kLfNlr, B - 1);
+    T bc0[8], fh0[8], fq0[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        bc0[q] = fat(kLfNcp, max(len - 2 - q, 0));
+        fh0[q] = fat(kLfH, min(q, tl));
+        fq0[q] = fat(kLfQ, min(q, tl));
+    }
+    T fra[8], fna[8], frb[8], fnb[8];
+    auto ld_fwd = [&](T(&fr)[8], T(&fn)[8], int t0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = min(t0 + q, tl);
+            fr[q] = fat(kLfR, t);
+            fn[q] = fat(kLfNlr, t);
+        }
+    };
+    ld_fwd(fra, fna, 0);
+
+    // ---- the right side d, staged in the segment layout ----------------
+    const size_t row = (size_t)j * P;
+    for (int i0 = 1; i0 <= n; i0 += 64 * kLineU) {
+        unsigned fv[kLineU];
+        T pv[kLineU], vn[kLineU], vs[kLineU], rh[kLineU], xa[kLineU], xb[kLineU];
+#pragma unroll
+        for (int q = 0; q < kLineU; q++) {
+            const int i = min(i0 + 64 * q + lane, n);
+            const size_t id = row + i;
+            fv[q] = gfl[id];
+            pv[q] = gsrc[id];
+            vn[q] = gnbr[id + P];
+            vs[q] = gnbr[id - P];
+            rh[q] = grhs[id];
+            xa[q] = gxp[i];
+            xb[q] = gxm[i];
+        }
+#pragma unroll
+        for (int q = 0; q < kLineU; q++) {
+            const int i = i0 + 64 * q + lane;
+            if (i <= n) {
+                const unsigned f = fv[q];
+                T d = zero_src ? (T)0 : pv[q];
+                T l, b, u;
+                if (LineMetric<T>::cell(f, xa[q], xb[q], yN, yS, l, b, u)) {
+                    const T aN = (f & F_OPEN_N) ? yN : (T)0, aS = (f & F_OPEN_S) ? yS : (T)0;
+                    const T un = zero_nbr ? (T)0 : vn[q], us = zero_nbr ? (T)0 : vs[q];
+                    d = (aN * un + aS * us) - rh[q];
+                }
+                g[slot(i)] = d;
+            }
+        }
     }
     __syncthreads();
 
-    // ---- forward elimination (the d' recurrence) ------------------------
-    // The movers load a chunk's inputs one stage before they stage it, so
-    // the loads are in flight across a barrier (indices clamped into the
-    // row).
-    unsigned fv[kPer];
-    T po[kPer], pn[kPer], ps[kPer], rh[kPer], mv[kPer], yv[kPer], xp = (T)0, xm = (T)0;
-    auto fwd_load = [&](int ch) {
-        const int i = min(1 + ch * kW + mcol, n);
-        xp = gcxp[i];
-        xm = gcxm[i];
+    // ---- segments: gp = fma(-l r, gp, d r) forward ------------------------
+    T gp = (T)0;
+    auto fwd = [&](const T(&fr)[8], const T(&fn)[8], int t0) {
+        T dv[8];
 #pragma unroll
-        for (int q = 0; q < kPer; q++) {
-            const int j = min(j0 + 2 * (mrow + q * (kMv / kW)), jlast);
-            const size_t id = (size_t)j * P + i;
-            fv[q] = gflags[id];
-            po[q] = gsrc[id];
-            pn[q] = gnbr[id + P];
-            ps[q] = gnbr[id - P];
-            rh[q] = grhs[id];
-            mv[q] = gm[id];
-            if constexpr (sizeof(T) == 4) yv[q] = gy[id];
+        for (int q = 0; q < 8; q++) dv[q] = g[min(t0 + q, tl) * 65 + lane];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = t0 + q;
+            if (t < len) {
+                const T dr = dv[q] * fr[q];
+                gp = t == 0 ? dr : fmaT(fn[q], gp, dr);
+                g[t * 65 + lane] = gp;
+            }
         }
     };
-    if (wave >= 1) fwd_load(0);
-    T dprev = (T)0;
-    for (int t = 0; t < nch + 2; t++) {
-        if (wave == 0) {  // the d' recurrence on chunk t-1
-            if (t >= 1 && t <= nch) {
-                const int b = (t - 1) & 1;
-                const int i0 = 1 + (t - 1) * kW;
-                const int kn = min(kW, n - i0 + 1);
-                // groups of 8 cells: the group's LDS and coefficient loads
-                // are issued ahead of its recurrence.  Cells past the row's
-                // end (last chunk only) run on clamped inputs and are never
-                // stored.
-                for (int k8 = 0; k8 < kn; k8 += 8) {
-                    T dd[8], mm[8], yy[8], xw[8];
-                    unsigned ff[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        xw[q] = gcxm[min(i0 + k8 + q, n)];
-                        ff[q] = st.f[b][lane][k8 + q];
-                        dd[q] = st.d[b][lane][k8 + q];
-                        mm[q] = st.m[b][lane][k8 + q];
-                        if constexpr (sizeof(T) == 4) yy[q] = st.a[b][lane][k8 + q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const unsigned f = ff[q];
-                        const T aW = (f & F_OPEN_W) ? xw[q] : (T)0;
-                        const T l = (f & kLineSolved) ? -aW : (T)0;
-                        // the first cell: m is b, and d' = d / b
-                        const T num = (i0 + k8 + q == 1) ? dd[q] : dd[q] - l * dprev;
-                        T dpi;
-                        if constexpr (sizeof(T) == 8)
-                            dpi = num / mm[q];  // IEEE division, as the oracle
-                        else
-                            dpi = num * yy[q];
-                        dprev = dpi;
-                        st.d[b][lane][k8 + q] = dpi;
-                    }
-                }
-            }
-        } else {
-            const int b = t & 1;
-            if (t >= 2) {  // d' of chunk t-2
-                const int i = 1 + (t - 2) * kW + mcol;
-#pragma unroll
-                for (int q = 0; q < kPer; q++) {
-                    const int rl = mrow + q * (kMv / kW), j = j0 + 2 * rl;
-                    if (i <= n && j <= jlast) gdpv[(size_t)j * P + i] = st.d[b][rl][mcol];
-                }
-            }
-            if (t < nch) {  // stage chunk t from the registers loaded a stage ago
-#pragma unroll
-                for (int q = 0; q < kPer; q++) {
-                    const int rl = mrow + q * (kMv / kW);
-                    const unsigned f = fv[q];
-                    T d = zero_src ? (T)0 : po[q];
-                    unsigned fs = f;
-                    if (f & F_FLUID) {
-                        const T aE = (f & F_OPEN_E) ? xp : (T)0;
-                        const T aW = (f & F_OPEN_W) ? xm : (T)0;
-                        const T aN = (f & F_OPEN_N) ? st.yp[rl] : (T)0;
-                        const T aS = (f & F_OPEN_S) ? st.ym[rl] : (T)0;
-                        const T sa = (aE + aW) + (aN + aS);
-                        if (sa != (T)0) {
-                            const T vn = zero_nbr ? (T)0 : pn[q], vs = zero_nbr ? (T)0 : ps[q];
-                            d = (aN * vn + aS * vs) - rh[q];
-                            fs |= kLineSolved;
-                        }
-                    }
-                    st.f[b][rl][mcol] = (uint8_t)fs;
-                    st.d[b][rl][mcol] = d;
-                    st.m[b][rl][mcol] = mv[q];
-                    if constexpr (sizeof(T) == 4) st.a[b][rl][mcol] = yv[q];
-                }
-            }
-            if (t + 1 < nch) fwd_load(t + 1);
-        }
-        __syncthreads();
+    for (int t0 = 0; t0 < len; t0 += 16) {
+        if (t0 + 8 < len) ld_fwd(frb, fnb, t0 + 8);
+        fwd(fra, fna, t0);
+        if (t0 + 8 >= len) break;
+        if (t0 + 16 < len) ld_fwd(fra, fna, t0 + 16);
+        fwd(frb, fnb, t0 + 8);
     }
-
-    // ---- back substitution (right to left), relaxation, store -----------
-    // Mover registers for step t, loaded at step t-1: the cell flags and old
-    // values of the chunk staged at step t-2 (relaxed and stored at step t:
-    // fv, po, xp, xm) and c', d' of the chunk staged at step t (pn, ps).
-    auto bwd_load = [&](int t) {
-        if (t >= 2 && t <= nch + 1) {
-            const int i = min(1 + (nch + 1 - t) * kW + mcol, n);
-            xp = gcxp[i];
-            xm = gcxm[i];
+    const T glast = gp;
+    // ---- backward g = fma(-u r, g, gp) ------------------------------------
+    T gx = gp;
+    auto ld_bwd = [&](T(&fc)[8], int t0) {
 #pragma unroll
-            for (int q = 0; q < kPer; q++) {
-                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kW)), jlast) * P + i;
-                fv[q] = gflags[id];
-                po[q] = gsrc[id];
-            }
-        }
-        if (t < nch) {
-            const int i = max(1, min(1 + (nch - 1 - t) * kW + mcol, n));
+        for (int q = 0; q < 8; q++) fc[q] = fat(kLfNcp, max(t0 - q, 0));
+    };
+    auto bwd = [&](const T(&fc)[8], int t0) {
+        T gv[8];
 #pragma unroll
-            for (int q = 0; q < kPer; q++) {
-                const size_t id = (size_t)min(j0 + 2 * (mrow + q * (kMv / kW)), jlast) * P + i;
-                pn[q] = gcpv[id];
-                ps[q] = gdpv[id];
+        for (int q = 0; q < 8; q++) gv[q] = g[max(t0 - q, 0) * 65 + lane];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = t0 - q;
+            if (t >= 0) {
+                gx = fmaT(fc[q], gx, gv[q]);
+                g[t * 65 + lane] = gx;
             }
         }
     };
-    if (wave >= 1) bwd_load(0);
-    T x = (T)0;
-    for (int t = 0; t < nch + 2; t++) {
-        if (wave == 0) {
-            if (t >= 1 && t <= nch) {
-                const int b = (t - 1) & 1;
-                const int i0 = 1 + (nch - t) * kW;
-                const int kn = min(kW, n - i0 + 1);
-                // groups of 8 cells from the right; in the row's last
-                // chunk the cells past its end come first and x restarts
-                // at i = n
-                for (int k8 = ((kn - 1) & ~7); k8 >= 0; k8 -= 8) {
-                    T ca[8], da[8];
+    // fra/frb are free now: the backward pass's second buffer is fra
+    for (int t0 = len - 2; t0 >= 0; t0 -= 16) {
+        if (t0 - 8 >= 0) ld_bwd(fra, t0 - 8);
+        bwd(bc0, t0);
+        if (t0 - 8 < 0) break;
+        if (t0 - 16 >= 0) ld_bwd(bc0, t0 - 16);
+        bwd(fra, t0 - 8);
+    }
+    // ---- separators: D = fma(-u, g right, fma(-l, g left, d)); cyclic
+    // reduction D = fma(gamma, D[k + s], fma(alpha, D[k - s], D)) with 0
+    // outside 0..m-1; x = D times the reciprocal of the reduced diagonal
+    const T gR = shfl_down_t(gx, 1);
+    T D = (T)0;
+    if (lane < m) D = fmaT(snu, gR, fmaT(snl, glast, g[(B - 1) * 65 + lane]));
 #pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        ca[q] = st.a[b][lane][k8 + q];
-                        da[q] = st.d[b][lane][k8 + q];
-                    }
+    for (int q = 0; q < kLinePcrLevels; q++) {
+        const int sg = 1 << q;
+        if (sg >= m) break;
+        const T Dm_ = shfl_up_t(D, sg), Dp_ = shfl_down_t(D, sg);
+        const T Dm = lane >= sg ? Dm_ : (T)0, Dp = lane + sg < m ? Dp_ : (T)0;
+        if (lane < m) D = fmaT(pga[q], Dp, fmaT(pal[q], Dm, D));
+    }
+    const T xs = lane < m ? D * prb : (T)0;
+    const T xl_ = shfl_up_t(xs, 1);
+    const T xL = lane >= 1 ? xl_ : (T)0, xR = lane < m ? xs : (T)0;
+    // ---- x = fma(q, xR, fma(h, xL, g)); separators take xs ---------------
+    auto ld_fin = [&](T(&fh)[8], T(&fq)[8], int t0) {
 #pragma unroll
-                    for (int q = 7; q >= 0; q--) {
-                        x = (i0 + k8 + q == n) ? da[q] : da[q] - ca[q] * x;
-                        st.d[b][lane][k8 + q] = x;
-                    }
-                }
-            }
-        } else {
-            const int b = t & 1;
-            const int is = 1 + (nch + 1 - t) * kW + mcol;  // chunk staged at step t-2
-#pragma unroll
-            for (int q = 0; q < kPer; q++) {
-                const int rl = mrow + q * (kMv / kW), j = j0 + 2 * rl;
-                if (t >= 2) {
-                    const unsigned f = fv[q];
-                    const T pold = zero_src ? (T)0 : po[q];
-                    const T xv = st.d[b][rl][mcol];
-                    T val = pold;
-                    if (f & F_FLUID) {
-                        const T aE = (f & F_OPEN_E) ? xp : (T)0;
-                        const T aW = (f & F_OPEN_W) ? xm : (T)0;
-                        const T aN = (f & F_OPEN_N) ? st.yp[rl] : (T)0;
-                        const T aS = (f & F_OPEN_S) ? st.ym[rl] : (T)0;
-                        if ((aE + aW) + (aN + aS) != (T)0) val = pold + om * (xv - pold);
-                    }
-                    if (is <= n && j <= jlast) gdst[(size_t)j * P + is] = val;
-                }
-                if (t < nch) {
-                    st.a[b][rl][mcol] = pn[q];
-                    st.d[b][rl][mcol] = ps[q];
-                }
-            }
-            bwd_load(t + 1);
+        for (int q = 0; q < 8; q++) {
+            const int t = min(t0 + q, tl);
+            fh[q] = fat(kLfH, t);
+            fq[q] = fat(kLfQ, t);
         }
-        __syncthreads();
+    };
+    auto fin = [&](const T(&fh)[8], const T(&fq)[8], int t0) {
+        T gv[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) gv[q] = g[min(t0 + q, tl) * 65 + lane];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (t0 + q < len) g[(t0 + q) * 65 + lane] = fmaT(fq[q], xR, fmaT(fh[q], xL, gv[q]));
+    };
+    for (int t0 = 0; t0 < len; t0 += 16) {
+        if (t0 + 8 < len) ld_fin(frb, fnb, t0 + 8);
+        fin(fh0, fq0, t0);
+        if (t0 + 8 >= len) break;
+        if (t0 + 16 < len) ld_fin(fh0, fq0, t0 + 16);
+        fin(frb, fnb, t0 + 8);
+    }
+    if (lane < m) g[(B - 1) * 65 + lane] = xs;
+    __syncthreads();
+
+    // ---- relax the solved cells, store the row ---------------------------
+    for (int i0 = 1; i0 <= n; i0 += 64 * kLineU) {
+        unsigned fv[kLineU];
+        T pv[kLineU], xa[kLineU], xb[kLineU];
+#pragma unroll
+        for (int q = 0; q < kLineU; q++) {
+            const int i = min(i0 + 64 * q + lane, n);
+            fv[q] = gfl[row + i];
+            pv[q] = gsrc[row + i];
+            xa[q] = gxp[i];
+            xb[q] = gxm[i];
+        }
+#pragma unroll
+        for (int q = 0; q < kLineU; q++) {
+            const int i = i0 + 64 * q + lane;
+            if (i <= n) {
+                const T pold = zero_src ? (T)0 : pv[q];
+                T val = pold;
+                T l, b, u;
+                if (LineMetric<T>::cell(fv[q], xa[q], xb[q], yN, yS, l, b, u)) val = pold + om * (g[slot(i)] - pold);
+                gdst[row + i] = val;
+            }
+        }
     }
 }
 
-
 void launch_line_factor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny)
 {
-    const dim3 grid((unsigned)((max_ny - 2 + 63) / 64), (unsigned)ncase);
+    const dim3 grid((unsigned)(max_ny - 2), (unsigned)ncase);
     if (prec == IF_FP64)
         hipLaunchKernelGGL(k_line_factor<double>, grid, dim3(64), 0, st, cs);
     else
         hipLaunchKernelGGL(k_line_factor<float>, grid, dim3(64), 0, st, cs);
 }
 
-void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny, int s, int K)
+void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_n, int s, int K)
 {
-    const int rows = (max_ny - 1) / 2;  // rows of one colour, at most
-    const dim3 grid((unsigned)((rows + 63) / 64), (unsigned)ncase);
+    // max_n bounds every case's row length and row count: the LDS holds the
+    // largest partition among line cases (at most kLineMaxB, checked at
+    // case creation), and each colour has at most (max_n - 1) / 2 rows
+    int B, S;
+    line_partition(max_n - 2, &B, &S);
+    B = std::min(B, kLineMaxB);
+    const size_t lds = (size_t)B * 65 * (prec == IF_FP64 ? 8 : 4);
+    const dim3 grid((unsigned)((max_n - 1) / 2), (unsigned)ncase);
     for (int colour = 1; colour >= 0; colour--) {
         if (prec == IF_FP64)
-            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(kLineThreads), 0, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(64), lds, st, cs, s, K, colour);
         else
-            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(kLineThreads), 0, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(64), lds, st, cs, s, K, colour);
     }
 }
 

@@ -887,65 +887,227 @@ double of_phys_residual(int nx, int ny, const double *dx, const double *dy,
 
 /* Zebra line SOR, the deck's PPE_Solver 2 "Line SOR" (inputs.txt:14-15):
  * rows j odd first, then rows j even; each row's cells are solved together
- * (tridiagonal in i, the Thomas algorithm) with the rows above and below
- * held, then relaxed, p <- p + om (x - p).  Row equation of a fluid cell
- * with sa = (aE + aW) + (aN + aS) != 0:
+ * with the rows above and below held, then relaxed, p <- p + om (x - p).
+ * Row equation of a fluid cell with sa = (aE + aW) + (aN + aS) != 0:
  *     -aW x[i-1] + sa x[i] - aE x[i+1] = (aN pN + aS pS) - rhs;
- * any other cell keeps its value (an identity row). */
+ * any other cell keeps its value (an identity row, l = u = 0, b = 1,
+ * d = p).
+ *
+ * The row's tridiagonal system (cells i = 1..n, n = nx - 2) is solved by a
+ * partition (Schur-complement) method, the form a GPU runs with one lane
+ * per segment:
+ *   - the row is cut into S blocks of B cells, B = max(2, ceil(n / 64)),
+ *     S = ceil(n / B) <= 64; the last cell of each block but the last is a
+ *     separator, the block's other cells its segment (at least one cell);
+ *   - within a segment, x = g + h xL + q xR, with xL, xR the separators on
+ *     either side (0 where there is none): g solves the segment's system
+ *     with the row's right side, h and q with -l of the first cell and -u
+ *     of the last cell (the Thomas algorithm, pivots as reciprocals r);
+ *   - the separators' Schur complement is tridiagonal (m = S - 1
+ *     unknowns) and is solved by parallel cyclic reduction with strides
+ *     1, 2, 4, ... < m.
+ * Every quantity that depends only on the flags and the metric (r, the
+ * negated products -l r and -u r, h, q, the reduction's multipliers and
+ * final reciprocals) is formed once per sweep by line_factor_row; a row
+ * solve applies them to the right side with the exact operations listed
+ * there (fma is the fused multiply-add, one rounding).  The result equals
+ * the Thomas algorithm's to round-off (pinned by the row equations). */
+enum { LF_R, LF_NLR, LF_NCP, LF_H, LF_Q, LF_PLANES };  /* per-cell planes [B][64] */
+enum { LF_PCR_LEVELS = 6, LF_PCR = 2 * LF_PCR_LEVELS + 1 };  /* alpha, gamma per level; 1 / B */
+
+static void line_partition(int n, int *B, int *S)
+{
+    int b = (n + 63) / 64;
+    if (b < 2) b = 2;
+    *B = b;
+    *S = (n + b - 1) / b;
+}
+
+/* Row coefficients of cell (i, j): l = -aW, b = sa, u = -aE for a solved
+ * cell (returns 1), the identity row otherwise. */
+static int line_cell(const ppe_coef *c, int nx, int i, int j, uint8_t f, double *l, double *b, double *u)
+{
+    *l = 0.0;
+    *b = 1.0;
+    *u = 0.0;
+    (void)nx;
+    if (!(f & OF_FLUID)) return 0;
+    const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+    const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+    const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+    const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+    const double sa = (aE + aW) + (aN + aS);
+    if (sa == 0.0) return 0;
+    *l = -aW;
+    *b = sa;
+    *u = -aE;
+    return 1;
+}
+
+/* Factors of row j: F[(plane * B + t) * 64 + k] for segment k's t-th cell
+ * (plane LF_R at t = B - 1 holds the separator's -l, LF_NLR its -u), then
+ * the reduction's LF_PCR planes of 64 at F + LF_PLANES * B * 64. */
+static void line_factor_row(const ppe_coef *c, int nx, int j, const uint8_t *flags, int B, int S, double *F)
+{
+    const int n = nx - 2, m = S - 1;
+#define LFP(pl, t, k) F[((size_t)(pl) * B + (t)) * 64 + (k)]
+    for (int k = 0; k < S; k++) {
+        const int a = 1 + k * B, len = k < S - 1 ? B - 1 : n - (S - 1) * B;
+        double cpv = 0.0, hpv = 0.0;
+        for (int t = 0; t < len; t++) {
+            const int i = a + t;
+            double l, b, u;
+            line_cell(c, nx, i, j, flags[IDX(i, j, nx)], &l, &b, &u);
+            const double den = t == 0 ? b : b - l * cpv;
+            const double r = 1.0 / den, cp = u * r, nlr = -(l * r);
+            const double hp = t == 0 ? nlr : nlr * hpv;
+            LFP(LF_R, t, k) = r;
+            LFP(LF_NLR, t, k) = nlr;
+            LFP(LF_NCP, t, k) = -cp;
+            LFP(LF_H, t, k) = hp;
+            cpv = cp;
+            hpv = hp;
+        }
+        LFP(LF_Q, len - 1, k) = LFP(LF_NCP, len - 1, k);
+        for (int t = len - 2; t >= 0; t--) {
+            LFP(LF_H, t, k) = fma(LFP(LF_NCP, t, k), LFP(LF_H, t + 1, k), LFP(LF_H, t, k));
+            LFP(LF_Q, t, k) = LFP(LF_NCP, t, k) * LFP(LF_Q, t + 1, k);
+        }
+    }
+    double A[64], Bv[64], Cv[64], nA[64], nB[64], nC[64];
+    double *pcr = F + (size_t)LF_PLANES * B * 64;
+    for (int q = 0; q < LF_PCR * 64; q++) pcr[q] = 0.0;
+    for (int k = 0; k < m; k++) {
+        const int sp = (k + 1) * B;
+        double l, b, u;
+        line_cell(c, nx, sp, j, flags[IDX(sp, j, nx)], &l, &b, &u);
+        const double hL = LFP(LF_H, B - 2, k), qL = LFP(LF_Q, B - 2, k);
+        const double hR = LFP(LF_H, 0, k + 1), qR = LFP(LF_Q, 0, k + 1);
+        A[k] = l * hL;
+        Cv[k] = u * qR;
+        Bv[k] = (b + l * qL) + u * hR;
+        LFP(LF_R, B - 1, k) = -l;
+        LFP(LF_NLR, B - 1, k) = -u;
+    }
+    int lev = 0;
+    for (int sg = 1; sg < m; sg *= 2, lev++) {
+        for (int k = 0; k < m; k++) {
+            const int lo = k >= sg, hi = k + sg < m;
+            const double alpha = lo ? -(A[k] / Bv[k - sg]) : 0.0;
+            const double gamma = hi ? -(Cv[k] / Bv[k + sg]) : 0.0;
+            const double Am = lo ? A[k - sg] : 0.0, Cm = lo ? Cv[k - sg] : 0.0;
+            const double Ap = hi ? A[k + sg] : 0.0, Cp = hi ? Cv[k + sg] : 0.0;
+            nA[k] = alpha * Am;
+            nC[k] = gamma * Cp;
+            nB[k] = (Bv[k] + alpha * Cm) + gamma * Ap;
+            pcr[(2 * lev) * 64 + k] = alpha;
+            pcr[(2 * lev + 1) * 64 + k] = gamma;
+        }
+        memcpy(A, nA, sizeof(double) * m);
+        memcpy(Bv, nB, sizeof(double) * m);
+        memcpy(Cv, nC, sizeof(double) * m);
+    }
+    for (int k = 0; k < m; k++) pcr[(2 * LF_PCR_LEVELS) * 64 + k] = 1.0 / Bv[k];
+#undef LFP
+}
+
+/* Solve row j with the held rows above and below and relax it (p in
+ * place).  d, g, x: scratch of B * 64. */
+static void line_solve_row(const ppe_coef *c, int nx, int j, const uint8_t *flags, const double *rhs, double om,
+                           int B, int S, const double *F, double *p, double *g)
+{
+    const int n = nx - 2, m = S - 1;
+#define LFP(pl, t, k) F[((size_t)(pl) * B + (t)) * 64 + (k)]
+#define GV(t, k) g[(size_t)(t) * 64 + (k)]
+    /* the right side d (in g), segment layout */
+    for (int i = 1; i <= n; i++) {
+        const int id = IDX(i, j, nx);
+        const uint8_t f = flags[id];
+        double d = p[id];
+        if (f & OF_FLUID) {
+            const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+            const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+            const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+            const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+            if ((aE + aW) + (aN + aS) != 0.0) d = (aN * p[id + nx] + aS * p[id - nx]) - rhs[id];
+        }
+        GV((i - 1) % B, (i - 1) / B) = d;
+    }
+    /* segments: forward gp = fma(-l r, gp, d r), backward g = fma(-u r, g, gp) */
+    for (int k = 0; k < S; k++) {
+        const int len = k < S - 1 ? B - 1 : n - (S - 1) * B;
+        double gp = 0.0;
+        for (int t = 0; t < len; t++) {
+            const double dr = GV(t, k) * LFP(LF_R, t, k);
+            gp = t == 0 ? dr : fma(LFP(LF_NLR, t, k), gp, dr);
+            GV(t, k) = gp;
+        }
+        double gx = GV(len - 1, k);
+        for (int t = len - 2; t >= 0; t--) {
+            gx = fma(LFP(LF_NCP, t, k), gx, GV(t, k));
+            GV(t, k) = gx;
+        }
+    }
+    /* separators: D = fma(-u, g right, fma(-l, g left, d)), then the
+     * reduction D = fma(gamma, D[k + s], fma(alpha, D[k - s], D)) (0
+     * outside 0..m-1) and x = D / B as D times the reciprocal */
+    double D[64], nD[64], xs[64];
+    for (int k = 0; k < m; k++)
+        D[k] = fma(LFP(LF_NLR, B - 1, k), GV(0, k + 1), fma(LFP(LF_R, B - 1, k), GV(B - 2, k), GV(B - 1, k)));
+    const double *pcr = F + (size_t)LF_PLANES * B * 64;
+    int lev = 0;
+    for (int sg = 1; sg < m; sg *= 2, lev++) {
+        for (int k = 0; k < m; k++) {
+            const double Dm = k >= sg ? D[k - sg] : 0.0, Dp = k + sg < m ? D[k + sg] : 0.0;
+            nD[k] = fma(pcr[(2 * lev + 1) * 64 + k], Dp, fma(pcr[(2 * lev) * 64 + k], Dm, D[k]));
+        }
+        memcpy(D, nD, sizeof(double) * m);
+    }
+    for (int k = 0; k < m; k++) xs[k] = D[k] * pcr[(2 * LF_PCR_LEVELS) * 64 + k];
+    /* x = fma(q, xR, fma(h, xL, g)); separators take xs */
+    for (int k = 0; k < S; k++) {
+        const int len = k < S - 1 ? B - 1 : n - (S - 1) * B;
+        const double xL = k >= 1 ? xs[k - 1] : 0.0, xR = k < m ? xs[k] : 0.0;
+        for (int t = 0; t < len; t++) GV(t, k) = fma(LFP(LF_Q, t, k), xR, fma(LFP(LF_H, t, k), xL, GV(t, k)));
+        if (k < m) GV(B - 1, k) = xs[k];
+    }
+    /* relax the solved cells only */
+    for (int i = 1; i <= n; i++) {
+        const int id = IDX(i, j, nx);
+        const uint8_t f = flags[id];
+        if (f & OF_FLUID) {
+            const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
+            const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
+            const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
+            const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
+            if ((aE + aW) + (aN + aS) != 0.0) {
+                const double x = GV((i - 1) % B, (i - 1) / B);
+                p[id] = p[id] + om * (x - p[id]);
+            }
+        }
+    }
+#undef GV
+#undef LFP
+}
+
 static void line_sor_sweep(const ppe_coef *c, int nx, int ny, const uint8_t *flags,
                            const double *rhs, double om, double *p)
 {
+    int B, S;
+    line_partition(nx - 2, &B, &S);
+    const size_t fsz = ((size_t)LF_PLANES * B + LF_PCR) * 64;
     for (int colour = 1; colour >= 0; colour--) {
 #pragma omp parallel
         {
-            double *cp = malloc(sizeof(double) * nx), *dp = malloc(sizeof(double) * nx);
-            double *x = malloc(sizeof(double) * nx);
+            double *F = malloc(sizeof(double) * fsz), *g = malloc(sizeof(double) * (size_t)B * 64);
 #pragma omp for schedule(static)
             for (int j = 1; j < ny - 1; j++) {
                 if ((j & 1) != colour) continue;
-                for (int i = 1; i < nx - 1; i++) {
-                    const int id = IDX(i, j, nx);
-                    const uint8_t f = flags[id];
-                    double l = 0.0, b = 1.0, u = 0.0, d = p[id];
-                    if (f & OF_FLUID) {
-                        const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
-                        const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
-                        const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
-                        const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
-                        const double sa = (aE + aW) + (aN + aS);
-                        if (sa != 0.0) {
-                            l = -aW;
-                            b = sa;
-                            u = -aE;
-                            d = (aN * p[id + nx] + aS * p[id - nx]) - rhs[id];
-                        }
-                    }
-                    if (i == 1) {
-                        cp[i] = u / b;
-                        dp[i] = d / b;
-                    } else {
-                        const double m = b - l * cp[i - 1];
-                        cp[i] = u / m;
-                        dp[i] = (d - l * dp[i - 1]) / m;
-                    }
-                }
-                x[nx - 2] = dp[nx - 2];
-                for (int i = nx - 3; i >= 1; i--) x[i] = dp[i] - cp[i] * x[i + 1];
-                for (int i = 1; i < nx - 1; i++) {  /* relax the solved cells only */
-                    const int id = IDX(i, j, nx);
-                    const uint8_t f = flags[id];
-                    if (f & OF_FLUID) {
-                        const double aE = (f & OF_OPEN_E) ? c->xp[i] : 0.0;
-                        const double aW = (f & OF_OPEN_W) ? c->xm[i] : 0.0;
-                        const double aN = (f & OF_OPEN_N) ? c->yp[j] : 0.0;
-                        const double aS = (f & OF_OPEN_S) ? c->ym[j] : 0.0;
-                        if ((aE + aW) + (aN + aS) != 0.0) p[id] = p[id] + om * (x[i] - p[id]);
-                    }
-                }
+                line_factor_row(c, nx, j, flags, B, S, F);
+                line_solve_row(c, nx, j, flags, rhs, om, B, S, F, p, g);
             }
-            free(cp);
-            free(dp);
-            free(x);
+            free(F);
+            free(g);
         }
     }
 }

@@ -239,7 +239,7 @@ def test_weighted_jacobi_matches_oracle(require_gpu):
     gc.close()
 
 
-# a row of 81 cells: the line solve's last chunk of 8 cells is ragged
+# rows of 81 cells: the line solve's partition ends in a block of a single cell
 RAGGED = (stretched_faces(81, 4.0, 1.005), np.linspace(0, 2.0, 40), [iff.circle(1.7, 0.9, 0.4)])
 
 
@@ -262,6 +262,19 @@ def test_line_sor_matches_oracle(require_gpu, name):
     gm.step(2)
     _compare_fields(oc, gm, 1e-4)
     gm.close()
+
+
+def test_line_sor_row_width_limit(require_gpu):
+    """The line solve stages a row in LDS: rows of more than 64 x 126 = 8064
+    interior cells are refused at case creation, and 8064 is accepted."""
+    yf = np.linspace(0.0, 1.0, 5)
+    with pytest.raises(iff.ImmerseFlowError) as e:
+        iff.Case(np.linspace(0.0, 1.0, 8066), yf, dt=1e-3, re=100.0, ppe_solver=iff.IF_PPE_LINE)
+    assert e.value.code == iff._abi.IF_ERR_ARG and "8064" in str(e.value)
+    c = iff.Case(np.linspace(0.0, 1.0, 8065), yf, dt=1e-3, re=100.0, ppe_solver=iff.IF_PPE_LINE,
+                 ppe_tol=1e-8, ppe_itermax=50, check_every=10)
+    c.step(1)
+    c.close()
 
 
 def test_batch_mixes_poisson_solvers(require_gpu):

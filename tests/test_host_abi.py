@@ -215,12 +215,15 @@ def test_footprint_by_precision():
 
 
 def test_footprint_of_a_line_solve_case():
-    """A zebra line-SOR case holds its row pivots (FP64: m; FP32 error
-    sweeps: m and 1/m) for the whole step: one 8-byte or two 4-byte fields
-    more than the same case with red-black SOR."""
+    """A zebra line-SOR case holds its row factors for the whole step: per
+    row, five per-cell planes of B x 64 and 13 x 64 reduction values in the
+    Poisson precision (256 interior cells: B = 4, S = 64 blocks), beyond
+    what the same case needs with red-black SOR."""
     xf = np.linspace(0.0, 16.0, 257)
-    pitched = 320 * 258  # rows of 258 cells padded to a multiple of 64
-    for prec in (iff.IF_FP64, iff.IF_MIXED):
+    B, S = 4, 64
+    per_row = (5 * B + 13) * 64
+    for prec, es in ((iff.IF_FP64, 8), (iff.IF_MIXED, 4)):
         rb, line = (iff.Batch.footprint(xf, xf, precision=prec, dt=0.01, re=100.0, ppe_solver=sv)
                     for sv in (iff.IF_PPE_RBSOR, iff.IF_PPE_LINE))
-        assert 8 * pitched <= line - rb <= 8 * pitched + 2 * 1024, (prec, line - rb)
+        want = es * per_row * 258
+        assert want <= line - rb <= want + 256, (prec, line - rb)
