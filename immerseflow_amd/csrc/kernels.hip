@@ -784,17 +784,18 @@ __global__ __launch_bounds__(64) void k_line_factor(const CaseDev *cs)
     pcr[(2 * kLinePcrLevels) * 64 + k] = k < m ? (T)1 / Bv : (T)0;
 }
 
-// One colour of one sweep: one wave per row (blocks of 64 threads), LDS of
+// One colour of one sweep: one block of kLineWaves waves per row, LDS of
 // B x 65 elements (the row's values in the segment layout, padded so that
 // the transposing store and the lanes' column walks are conflict-free).
-// A colour has few rows (a wave or two per CU), so the kernel is bound by
-// the latency of its dependent steps: the row passes load kLineU cells per
-// lane at once, and the lanes' factor loads run a group of 8 cells ahead
-// of the recurrences (two register buffers), with the first groups of the
-// later passes and the reduction's multipliers loaded at the start.
-constexpr int kLineU = 8;
+// A colour has few rows (a block or two per CU), so the kernel is bound by
+// the latency of its dependent steps: the row passes (all waves) load
+// kLineU cells per thread at once; wave 0 runs the segments, with its
+// factor loads a group of 8 cells ahead of the recurrences (two register
+// buffers) and the first groups of the later passes and the reduction's
+// multipliers loaded at the start.
+constexpr int kLineU = 8, kLineWaves = 4;
 template <typename T>
-__global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
+__global__ __launch_bounds__(64 * kLineWaves) void k_line_sor(const CaseDev *cs, int s, int K, int colour)
 {
     const CaseDev &c = cs[blockIdx.y];
     const Ctrl *ct = c.ctrl;
@@ -831,7 +832,8 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
     const T yN = G(mt.yp)[j], yS = G(mt.ym)[j];
     const auto gfl = G(c.flags);
     const int B = c.lB, S = c.lS, n = nx - 2, m = S - 1;
-    const int lane = (int)threadIdx.x;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const bool w0 = tid < 64;  // the wave that runs the segments
     const int len = lane < S - 1 ? B - 1 : (lane == S - 1 ? n - (S - 1) * B : 0);
     const auto F = G((const T *)fld<T>(c.lf) + (size_t)j * line_row_stride(B));
     auto fat = [&](int pl, int t) { return F[((size_t)pl * B + t) * 64 + lane]; };
@@ -850,21 +852,8 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
     // data-independent loads, issued first: the reduction's multipliers,
     // the separator's -l, -u and the first groups of the backward and
     // final passes
-    T pal[kLinePcrLevels], pga[kLinePcrLevels];
-#pragma unroll
-    for (int q = 0; q < kLinePcrLevels; q++) {
-        pal[q] = pcr[(2 * q) * 64 + lane];
-        pga[q] = pcr[(2 * q + 1) * 64 + lane];
-    }
-    const T prb = pcr[(2 * kLinePcrLevels) * 64 + lane];
-    const T snl = fat(kLfR, B - 1), snu = fat(kLfNlr, B - 1);
+    T pal[kLinePcrLevels], pga[kLinePcrLevels], prb = (T)0, snl = (T)0, snu = (T)0;
     T bc0[8], fh0[8], fq0[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        bc0[q] = fat(kLfNcp, max(len - 2 - q, 0));
-        fh0[q] = fat(kLfH, min(q, tl));
-        fq0[q] = fat(kLfQ, min(q, tl));
-    }
     T fra[8], fna[8], frb[8], fnb[8];
     auto ld_fwd = [&](T(&fr)[8], T(&fn)[8], int t0) {
 #pragma unroll
@@ -874,16 +863,33 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
             fn[q] = fat(kLfNlr, t);
         }
     };
-    ld_fwd(fra, fna, 0);
+    if (w0) {
+#pragma unroll
+        for (int q = 0; q < kLinePcrLevels; q++) {
+            pal[q] = pcr[(2 * q) * 64 + lane];
+            pga[q] = pcr[(2 * q + 1) * 64 + lane];
+        }
+        prb = pcr[(2 * kLinePcrLevels) * 64 + lane];
+        snl = fat(kLfR, B - 1);
+        snu = fat(kLfNlr, B - 1);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            bc0[q] = fat(kLfNcp, max(len - 2 - q, 0));
+            fh0[q] = fat(kLfH, min(q, tl));
+            fq0[q] = fat(kLfQ, min(q, tl));
+        }
+        ld_fwd(fra, fna, 0);
+    }
 
-    // ---- the right side d, staged in the segment layout ----------------
+    // ---- the right side d, staged in the segment layout (all waves) -----
     const size_t row = (size_t)j * P;
-    for (int i0 = 1; i0 <= n; i0 += 64 * kLineU) {
+    constexpr int kT = 64 * kLineWaves;
+    for (int i0 = 1; i0 <= n; i0 += kT * kLineU) {
         unsigned fv[kLineU];
         T pv[kLineU], vn[kLineU], vs[kLineU], rh[kLineU], xa[kLineU], xb[kLineU];
 #pragma unroll
         for (int q = 0; q < kLineU; q++) {
-            const int i = min(i0 + 64 * q + lane, n);
+            const int i = min(i0 + kT * q + tid, n);
             const size_t id = row + i;
             fv[q] = gfl[id];
             pv[q] = gsrc[id];
@@ -895,7 +901,7 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
         }
 #pragma unroll
         for (int q = 0; q < kLineU; q++) {
-            const int i = i0 + 64 * q + lane;
+            const int i = i0 + kT * q + tid;
             if (i <= n) {
                 const unsigned f = fv[q];
                 T d = zero_src ? (T)0 : pv[q];
@@ -911,108 +917,110 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
     }
     __syncthreads();
 
-    // ---- segments: gp = fma(-l r, gp, d r) forward ------------------------
-    T gp = (T)0;
-    auto fwd = [&](const T(&fr)[8], const T(&fn)[8], int t0) {
-        T dv[8];
+    if (w0) {
+        // ---- segments: gp = fma(-l r, gp, d r) forward ------------------------
+        T gp = (T)0;
+        auto fwd = [&](const T(&fr)[8], const T(&fn)[8], int t0) {
+            T dv[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) dv[q] = g[min(t0 + q, tl) * 65 + lane];
+            for (int q = 0; q < 8; q++) dv[q] = g[min(t0 + q, tl) * 65 + lane];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = t0 + q;
-            if (t < len) {
-                const T dr = dv[q] * fr[q];
-                gp = t == 0 ? dr : fmaT(fn[q], gp, dr);
-                g[t * 65 + lane] = gp;
+            for (int q = 0; q < 8; q++) {
+                const int t = t0 + q;
+                if (t < len) {
+                    const T dr = dv[q] * fr[q];
+                    gp = t == 0 ? dr : fmaT(fn[q], gp, dr);
+                    g[t * 65 + lane] = gp;
+                }
             }
+        };
+        for (int t0 = 0; t0 < len; t0 += 16) {
+            if (t0 + 8 < len) ld_fwd(frb, fnb, t0 + 8);
+            fwd(fra, fna, t0);
+            if (t0 + 8 >= len) break;
+            if (t0 + 16 < len) ld_fwd(fra, fna, t0 + 16);
+            fwd(frb, fnb, t0 + 8);
         }
-    };
-    for (int t0 = 0; t0 < len; t0 += 16) {
-        if (t0 + 8 < len) ld_fwd(frb, fnb, t0 + 8);
-        fwd(fra, fna, t0);
-        if (t0 + 8 >= len) break;
-        if (t0 + 16 < len) ld_fwd(fra, fna, t0 + 16);
-        fwd(frb, fnb, t0 + 8);
-    }
-    const T glast = gp;
-    // ---- backward g = fma(-u r, g, gp) ------------------------------------
-    T gx = gp;
-    auto ld_bwd = [&](T(&fc)[8], int t0) {
+        const T glast = gp;
+        // ---- backward g = fma(-u r, g, gp) ------------------------------------
+        T gx = gp;
+        auto ld_bwd = [&](T(&fc)[8], int t0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) fc[q] = fat(kLfNcp, max(t0 - q, 0));
-    };
-    auto bwd = [&](const T(&fc)[8], int t0) {
-        T gv[8];
+            for (int q = 0; q < 8; q++) fc[q] = fat(kLfNcp, max(t0 - q, 0));
+        };
+        auto bwd = [&](const T(&fc)[8], int t0) {
+            T gv[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) gv[q] = g[max(t0 - q, 0) * 65 + lane];
+            for (int q = 0; q < 8; q++) gv[q] = g[max(t0 - q, 0) * 65 + lane];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = t0 - q;
-            if (t >= 0) {
-                gx = fmaT(fc[q], gx, gv[q]);
-                g[t * 65 + lane] = gx;
+            for (int q = 0; q < 8; q++) {
+                const int t = t0 - q;
+                if (t >= 0) {
+                    gx = fmaT(fc[q], gx, gv[q]);
+                    g[t * 65 + lane] = gx;
+                }
             }
+        };
+        // fra/frb are free now: the backward pass's second buffer is fra
+        for (int t0 = len - 2; t0 >= 0; t0 -= 16) {
+            if (t0 - 8 >= 0) ld_bwd(fra, t0 - 8);
+            bwd(bc0, t0);
+            if (t0 - 8 < 0) break;
+            if (t0 - 16 >= 0) ld_bwd(bc0, t0 - 16);
+            bwd(fra, t0 - 8);
         }
-    };
-    // fra/frb are free now: the backward pass's second buffer is fra
-    for (int t0 = len - 2; t0 >= 0; t0 -= 16) {
-        if (t0 - 8 >= 0) ld_bwd(fra, t0 - 8);
-        bwd(bc0, t0);
-        if (t0 - 8 < 0) break;
-        if (t0 - 16 >= 0) ld_bwd(bc0, t0 - 16);
-        bwd(fra, t0 - 8);
-    }
-    // ---- separators: D = fma(-u, g right, fma(-l, g left, d)); cyclic
-    // reduction D = fma(gamma, D[k + s], fma(alpha, D[k - s], D)) with 0
-    // outside 0..m-1; x = D times the reciprocal of the reduced diagonal
-    const T gR = shfl_down_t(gx, 1);
-    T D = (T)0;
-    if (lane < m) D = fmaT(snu, gR, fmaT(snl, glast, g[(B - 1) * 65 + lane]));
+        // ---- separators: D = fma(-u, g right, fma(-l, g left, d)); cyclic
+        // reduction D = fma(gamma, D[k + s], fma(alpha, D[k - s], D)) with 0
+        // outside 0..m-1; x = D times the reciprocal of the reduced diagonal
+        const T gR = shfl_down_t(gx, 1);
+        T D = (T)0;
+        if (lane < m) D = fmaT(snu, gR, fmaT(snl, glast, g[(B - 1) * 65 + lane]));
 #pragma unroll
-    for (int q = 0; q < kLinePcrLevels; q++) {
-        const int sg = 1 << q;
-        if (sg >= m) break;
-        const T Dm_ = shfl_up_t(D, sg), Dp_ = shfl_down_t(D, sg);
-        const T Dm = lane >= sg ? Dm_ : (T)0, Dp = lane + sg < m ? Dp_ : (T)0;
-        if (lane < m) D = fmaT(pga[q], Dp, fmaT(pal[q], Dm, D));
-    }
-    const T xs = lane < m ? D * prb : (T)0;
-    const T xl_ = shfl_up_t(xs, 1);
-    const T xL = lane >= 1 ? xl_ : (T)0, xR = lane < m ? xs : (T)0;
-    // ---- x = fma(q, xR, fma(h, xL, g)); separators take xs ---------------
-    auto ld_fin = [&](T(&fh)[8], T(&fq)[8], int t0) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = min(t0 + q, tl);
-            fh[q] = fat(kLfH, t);
-            fq[q] = fat(kLfQ, t);
+        for (int q = 0; q < kLinePcrLevels; q++) {
+            const int sg = 1 << q;
+            if (sg >= m) break;
+            const T Dm_ = shfl_up_t(D, sg), Dp_ = shfl_down_t(D, sg);
+            const T Dm = lane >= sg ? Dm_ : (T)0, Dp = lane + sg < m ? Dp_ : (T)0;
+            if (lane < m) D = fmaT(pga[q], Dp, fmaT(pal[q], Dm, D));
         }
-    };
-    auto fin = [&](const T(&fh)[8], const T(&fq)[8], int t0) {
-        T gv[8];
+        const T xs = lane < m ? D * prb : (T)0;
+        const T xl_ = shfl_up_t(xs, 1);
+        const T xL = lane >= 1 ? xl_ : (T)0, xR = lane < m ? xs : (T)0;
+        // ---- x = fma(q, xR, fma(h, xL, g)); separators take xs ---------------
+        auto ld_fin = [&](T(&fh)[8], T(&fq)[8], int t0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) gv[q] = g[min(t0 + q, tl) * 65 + lane];
+            for (int q = 0; q < 8; q++) {
+                const int t = min(t0 + q, tl);
+                fh[q] = fat(kLfH, t);
+                fq[q] = fat(kLfQ, t);
+            }
+        };
+        auto fin = [&](const T(&fh)[8], const T(&fq)[8], int t0) {
+            T gv[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (t0 + q < len) g[(t0 + q) * 65 + lane] = fmaT(fq[q], xR, fmaT(fh[q], xL, gv[q]));
-    };
-    for (int t0 = 0; t0 < len; t0 += 16) {
-        if (t0 + 8 < len) ld_fin(frb, fnb, t0 + 8);
-        fin(fh0, fq0, t0);
-        if (t0 + 8 >= len) break;
-        if (t0 + 16 < len) ld_fin(fh0, fq0, t0 + 16);
-        fin(frb, fnb, t0 + 8);
-    }
-    if (lane < m) g[(B - 1) * 65 + lane] = xs;
+            for (int q = 0; q < 8; q++) gv[q] = g[min(t0 + q, tl) * 65 + lane];
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (t0 + q < len) g[(t0 + q) * 65 + lane] = fmaT(fq[q], xR, fmaT(fh[q], xL, gv[q]));
+        };
+        for (int t0 = 0; t0 < len; t0 += 16) {
+            if (t0 + 8 < len) ld_fin(frb, fnb, t0 + 8);
+            fin(fh0, fq0, t0);
+            if (t0 + 8 >= len) break;
+            if (t0 + 16 < len) ld_fin(fh0, fq0, t0 + 16);
+            fin(frb, fnb, t0 + 8);
+        }
+        if (lane < m) g[(B - 1) * 65 + lane] = xs;
+    }  // wave 0
     __syncthreads();
 
-    // ---- relax the solved cells, store the row ---------------------------
-    for (int i0 = 1; i0 <= n; i0 += 64 * kLineU) {
+    // ---- relax the solved cells, store the row (all waves) --------------
+    for (int i0 = 1; i0 <= n; i0 += kT * kLineU) {
         unsigned fv[kLineU];
         T pv[kLineU], xa[kLineU], xb[kLineU];
 #pragma unroll
         for (int q = 0; q < kLineU; q++) {
-            const int i = min(i0 + 64 * q + lane, n);
+            const int i = min(i0 + kT * q + tid, n);
             fv[q] = gfl[row + i];
             pv[q] = gsrc[row + i];
             xa[q] = gxp[i];
@@ -1020,7 +1028,7 @@ __global__ __launch_bounds__(64) void k_line_sor(const CaseDev *cs, int s, int K
         }
 #pragma unroll
         for (int q = 0; q < kLineU; q++) {
-            const int i = i0 + 64 * q + lane;
+            const int i = i0 + kT * q + tid;
             if (i <= n) {
                 const T pold = zero_src ? (T)0 : pv[q];
                 T val = pold;
@@ -1053,9 +1061,9 @@ void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int
     const dim3 grid((unsigned)((max_n - 1) / 2), (unsigned)ncase);
     for (int colour = 1; colour >= 0; colour--) {
         if (prec == IF_FP64)
-            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(64), lds, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<double>, grid, dim3(64 * kLineWaves), lds, st, cs, s, K, colour);
         else
-            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(64), lds, st, cs, s, K, colour);
+            hipLaunchKernelGGL(k_line_sor<float>, grid, dim3(64 * kLineWaves), lds, st, cs, s, K, colour);
     }
 }
 
