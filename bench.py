@@ -75,6 +75,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-line", action="store_true")
+    ap.add_argument("--line-cases", default="c1,c2", help="line-SOR workloads: c1 (256^2), c2 (2048x1024)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=2000)
     ap.add_argument("--c2-fp64-steps", type=int, default=3)
@@ -399,6 +400,29 @@ def c2_workload(iff, configs, a):
     return out
 
 
+def persistent_tail(cases, sweeps, tr, step):
+    """How much of a batch step could run as one persistent launch: the
+    running cases' tiles fit on the device (one 16-wave block per CU, 256)
+    only once every case with more tiles has converged.  From each case's
+    Poisson sweeps in one step (FP64, alone) and its tiles at the batch's
+    tuned shape: the sweeps, and the share of the step's tile-sweeps (its
+    work), during which the running tiles number at most 256."""
+    m = tr["sweeps_per_launch"]
+    wo = 128 - 4 * m if tr["kernel"] >= 2 else 128
+    tiles = [-(-(len(c["xf"]) - 1) // wo) * -(-(len(c["yf"]) - 1) // tr["tile_y"]) for c in cases]
+    cap, smax = 256, max(sweeps)
+    work = tail_work = tail_sweeps = 0
+    for sw in range(smax):
+        act = sum(t for t, n in zip(tiles, sweeps) if n > sw)
+        work += act
+        if act <= cap:
+            tail_work += act
+            tail_sweeps += 1
+    return dict(step=step, capacity_tiles=cap, tiles_all_cases=sum(tiles), max_sweeps=smax,
+                median_sweeps=sorted(sweeps)[len(sweeps) // 2], sweeps_with_tiles_fitting=tail_sweeps,
+                work_share_fitting=round(tail_work / max(work, 1), 4))
+
+
 def batched_c4(iff, configs, a):
     """The 256 seeded uneven cases (128^2..1024^2) resident in one batch,
     mixed precision, for all --batch-steps steps: step 1 untimed, autotune,
@@ -445,6 +469,7 @@ def batched_c4(iff, configs, a):
     cells = sum((len(c["xf"]) - 1) * (len(c["yf"]) - 1) for c in cases)
     tot = 0.0
     err = {"u": 0.0, "v": 0.0, "p": 0.0, "p_mod_const": 0.0}
+    last_sweeps = []
     for k, c in enumerate(cases):
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-7, check_every=160)
@@ -452,7 +477,7 @@ def batched_c4(iff, configs, a):
         s.persistent(not a.no_persistent)
         s.step(1)
         t0 = time.perf_counter()
-        s.step(W)
+        last_sweeps.append(s.step(W)["ppe_sweeps"])
         tot += time.perf_counter() - t0
         if k in keep and W + 1 == a.batch_steps:
             e = fields_err(lambda f: keep[k][f], s.get)
@@ -476,6 +501,7 @@ def batched_c4(iff, configs, a):
                       "%d steps); window: steps 2..%d of the batch against the FP64 single-case path on all 256 "
                       "cases one after another, same steps, rates per %d-step case"
                       % (a.batch_steps, a.batch_steps, 1 + W, a.batch_steps))
+    out["persistent_tail"] = persistent_tail(cases, last_sweeps, tr, 1 + W)
     if W + 1 != a.batch_steps:
         out["error_note"] = "fields compared only when the window is the whole run (--batch-window 199)"
     else:
@@ -487,18 +513,17 @@ def fp64_single_full(iff, configs, a):
     """--full: the FP64 single-case path on all 256 batch cases for all 200
     steps (one case at a time)."""
     cases = configs.c4_cases(256)
-    t0 = time.perf_counter()
+    dt = 0.0
     for k, c in enumerate(cases):
         s = iff.Case(c["xf"], c["yf"], precision=iff.IF_FP64, dt=c["dt"], re=c["re"], bodies=c["bodies"],
                      ppe_tol=1e-7, check_every=160)
         s.step(1)
-        if k == 0:
-            t0 = time.perf_counter()
+        t0 = time.perf_counter()
         s.step(c["steps"] - 1)
+        dt += time.perf_counter() - t0
         s.close()
         if k % 16 == 15:
-            log("fp64 single full: %d cases, %.1f s" % (k + 1, time.perf_counter() - t0))
-    dt = time.perf_counter() - t0
+            log("fp64 single full: %d cases, %.1f s timed" % (k + 1, dt))
     return dict(value=round(len(cases) / (dt * 200 / 199), 4), unit="cases/s", seconds=round(dt, 1),
                 sample="all 256 cases, steps 2..200 each (step 1 untimed), one case at a time")
 
@@ -550,13 +575,17 @@ def line_sor_f2(iff, configs, a):
     2048x1024 workload, from the sweep events of a solve capped at 400
     sweeps."""
     out = {}
+    want = set(a.line_cases.split(","))
     for cname, mk, steps, cap in (("c1", configs.c1, 2, 200000), ("c2", configs.c2, 1, 400)):
+        if cname not in want:
+            continue
         cfg = mk()
         NX, NY = len(cfg["xf"]) - 1, len(cfg["yf"]) - 1
         for pname, prec in (("fp64", iff.IF_FP64), ("mixed", iff.IF_MIXED)):
             c = iff.Case(cfg["xf"], cfg["yf"], precision=prec, dt=cfg["dt"], re=cfg["re"], bodies=cfg["bodies"],
                          ad_itermax=25, ad_tol=1e-6, ppe_solver=iff.IF_PPE_LINE, ppe_itermax=cap,
                          ppe_tol=cfg["ppe_tol"], check_every=40)
+            c.graphs(not a.no_graphs)
             c.step(1)
             c.sweep_timing(reset=True)
             t0 = time.perf_counter()

@@ -169,6 +169,7 @@ struct CaseDev {
     double *uf[2], *vf[2];         // face velocities: [0] = U^n, [1] = U*
     double *p[2], *rhs;
     float *e[2], *r32;             // mixed Poisson error / residual
+    float *w32;                    // mixed: each cell's FP32 SOR weight om / sa (0: not updated), formed once per step
     void *lf;                      // line solve factors (Poisson precision), line_row_stride(lB) per row; line cases only
     int lB, lS;                    // line solve partition of a row: S blocks of B cells
     uint8_t *flags;

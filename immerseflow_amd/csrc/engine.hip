@@ -119,7 +119,7 @@ int Engine::resident() const
 struct ArenaPlan {
     size_t off_u[2], off_v[2], off_sx, off_sy;
     size_t off_uf[2], off_vf[2], off_p[2], off_rhs;
-    size_t off_e[2], off_r32;
+    size_t off_e[2], off_r32, off_w32;
     size_t off_lf;
     size_t off_flags, off_bodyid;
     size_t off_metd, off_xc, off_yc, off_metf;
@@ -151,6 +151,7 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
     if (prec != IF_FP64) {
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
+        a.off_w32 = fieldT(4);
     }
     if (line) {  // the line solve's factors, one stride per row, in the Poisson precision
         int B, S;
@@ -381,7 +382,10 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     c.sx = A + a.off_sx;
     c.sy = A + a.off_sy;
     c.rhs = (double *)(A + a.off_rhs);
-    if (pprec_ == IF_MIXED) c.r32 = (float *)(A + a.off_r32);
+    if (pprec_ == IF_MIXED) {
+        c.r32 = (float *)(A + a.off_r32);
+        c.w32 = (float *)(A + a.off_w32);
+    }
     if (d.ppe_solver == IF_PPE_LINE) {
         c.lf = A + a.off_lf;
         line_partition(s.nx - 2, &c.lB, &c.lS);
@@ -905,6 +909,7 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         // ---- P3: faces + RHS
         launch_faces(st_, vprec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
         if (any_line) launch_line_factor(st_, pprec_, d_cases_, n, max_ring_);
+        if (pprec_ == IF_MIXED) launch_weights32(st_, d_cases_, n, max_ring_);
 
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;

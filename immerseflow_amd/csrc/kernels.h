@@ -40,6 +40,7 @@ void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase);
 // zebra line SOR: the per-step factors (pivots, c', 1/m) of every line
 // case, then one sweep (both colours) at chunk position s
 void launch_line_factor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny);
+void launch_weights32(hipStream_t st, const CaseDev *cs, int ncase, int max_n);
 void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_n, int s, int K);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);

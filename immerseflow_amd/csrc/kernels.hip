@@ -1213,6 +1213,39 @@ __global__ void k_ppe_rearm(const CaseDev *cs, int ncase)
     c->ppe_sweeps = 0;
 }
 
+// The FP32 red-black SOR weight of every interior cell, om / sa over the
+// open faces (sor_weight, the expression every FP32 sweep kernel uses), 0
+// for a cell the sweep keeps (not fluid, or no open face).  Once per step,
+// after the flags are final: the register-tile kernel loads it instead of
+// forming it from the flags on every launch.  One block per row.
+__global__ __launch_bounds__(256) void k_weights32(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.y];
+    if (!c.w32 || !c.ctrl->active) return;  // FP64 Poisson cases hold no FP32 weights
+    const int j = 1 + (int)blockIdx.x;
+    if (j > c.ny - 2) return;
+    const size_t row = (size_t)j * c.P;
+    const float yp = G(c.yf.cp)[j], ym = G(c.yf.cm)[j], om = c.omega_f;
+    for (int i = 1 + (int)threadIdx.x; i <= c.nx - 2; i += (int)blockDim.x) {
+        const unsigned f = G(c.flags)[row + i];
+        float w = 0.0f;
+        if (f & F_FLUID) {
+            const float aE = (f & F_OPEN_E) ? G(c.xf.cp)[i] : 0.0f;
+            const float aW = (f & F_OPEN_W) ? G(c.xf.cm)[i] : 0.0f;
+            const float aN = (f & F_OPEN_N) ? yp : 0.0f;
+            const float aS = (f & F_OPEN_S) ? ym : 0.0f;
+            const float sa = (aE + aW) + (aN + aS);
+            if (sa != 0.0f) w = sor_weight(sa, om);
+        }
+        GW(c.w32)[row + i] = w;
+    }
+}
+
+void launch_weights32(hipStream_t st, const CaseDev *cs, int ncase, int max_n)
+{
+    hipLaunchKernelGGL(k_weights32, dim3((unsigned)(max_n - 2), (unsigned)ncase), dim3(256), 0, st, cs);
+}
+
 void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase)
 {
     hipLaunchKernelGGL(k_ppe_rearm, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase);

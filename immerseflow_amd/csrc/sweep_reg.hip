@@ -299,23 +299,23 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     const int ga = c0 + 2 * lane;               // slot a column; slot b is ga + 1
     const bool ina = ga >= 1 && ga <= nx - 2;
     const bool inb = ga + 1 >= 1 && ga + 1 <= nx - 2;
-    // x coefficients of the fast path: zero on the domain boundary faces.
-    // A slot outside the interior gets cx = 1 (FP32: a finite weight; FP64:
-    // weight 0), so the fast path computes a finite value there (never
-    // stored, and only ever multiplied by the zero coefficient of a
-    // boundary face).
+    // x coefficients of the fast path: zero on the domain boundary faces
+    // and outside the interior.  A slot outside the interior has weight 0:
+    // its value stays finite, is never stored and is only ever multiplied
+    // by the zero coefficient of a boundary face.
     const T cxpa = (ina && ga != nx - 2) ? G(cxp_)[ga] : (T)0;
     const T cxma = (ina && ga != 1) ? G(cxm_)[ga] : (T)0;
     const T cxpb = (inb && ga + 1 != nx - 2) ? G(cxp_)[ga + 1] : (T)0;
     const T cxmb = (inb && ga + 1 != 1) ? G(cxm_)[ga + 1] : (T)0;
-    const T cxa = ina ? cxpa + cxma : (T)1, cxb = inb ? cxpb + cxmb : (T)1;
     // faces a fluid cell of each slot has open when it is a fast-path cell
     const int xa = (ga != nx - 2 ? F_OPEN_E : 0) | (ga != 1 ? F_OPEN_W : 0);
     const int xb = (ga + 1 != nx - 2 ? F_OPEN_E : 0) | (ga + 1 != 1 ? F_OPEN_W : 0);
     constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
 
     RegTile<T, NRW> R;
-    int lf[NRW];  // flags: slot a in bits 0-7, slot b in bits 8-15
+    int lf[NRW];  // FP64: flags, slot a in bits 0-7, slot b in bits 8-15
+    // FP32: the cells' weights om / sa, formed once per step (k_weights32)
+    const float *w32 = c.w32;
     // Loads.  A tile lying inside the interior (the common case) loads
     // every row with aligned 8-byte vectors and no per-lane tests.
     const bool tile_in = c0 >= 1 && c0 + 127 <= nx - 2 && jt0 >= 1 && jt0 + NRW - 1 <= ny - 2;
@@ -325,12 +325,19 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             const size_t ro = (size_t)(jt0 + k) * P;  // wave-uniform row offset
             const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + ro + ga);
             const V2 rv = gload_vec<V2>(rhs + ro + ga);
-            const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
             R.pa[k] = pv[0];
             R.pb[k] = pv[1];
             R.ra[k] = rv[0];
             R.rb[k] = rv[1];
-            lf[k] = (int)fb[0] | ((int)fb[1] << 8);
+            if constexpr (sizeof(T) == 4) {
+                const V2 wv = gload_vec<V2>(w32 + ro + ga);
+                R.wa[k] = wv[0];
+                R.wb[k] = wv[1];
+                lf[k] = 0;
+            } else {
+                const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
+                lf[k] = (int)fb[0] | ((int)fb[1] << 8);
+            }
         }
     } else {
 #pragma unroll
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             const int gj = jt0 + k;
             const bool rowin = gj >= 0 && gj < ny;
             const bool rowint = gj >= 1 && gj <= ny - 2;
-            T a = (T)0, b = (T)0, x = (T)0, y = (T)0;
+            T a = (T)0, b = (T)0, x = (T)0, y = (T)0, wa = (T)0, wb = (T)0;
             int f = 0;
             if (rowin) {
                 const size_t o = (size_t)gj * P + ga;
@@ -355,18 +362,30 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
                 if (rowint) {
                     if (ina && inb) {
                         const V2 v = gload_vec<V2>(rhs + o);
-                        const u8x2 fb = gload_vec<u8x2>(c.flags + o);
                         x = v[0];
                         y = v[1];
-                        f = (int)fb[0] | ((int)fb[1] << 8);
+                        if constexpr (sizeof(T) == 4) {
+                            const V2 wv = gload_vec<V2>(w32 + o);
+                            wa = wv[0];
+                            wb = wv[1];
+                        } else {
+                            const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+                            f = (int)fb[0] | ((int)fb[1] << 8);
+                        }
                     } else {
                         if (ina) {
                             x = G(rhs)[o];
-                            f = G(c.flags)[o];
+                            if constexpr (sizeof(T) == 4)
+                                wa = G(w32)[o];
+                            else
+                                f = G(c.flags)[o];
                         }
                         if (inb) {
                             y = G(rhs)[o + 1];
-                            f |= (int)G(c.flags)[o + 1] << 8;
+                            if constexpr (sizeof(T) == 4)
+                                wb = G(w32)[o + 1];
+                            else
+                                f |= (int)G(c.flags)[o + 1] << 8;
                         }
                     }
                 }
@@ -375,57 +394,43 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             R.pb[k] = b;
             R.ra[k] = x;
             R.rb[k] = y;
+            if constexpr (sizeof(T) == 4) {
+                R.wa[k] = wa;
+                R.wb[k] = wb;
+            }
             lf[k] = f;
         }
     }
 
-    // Per-row coefficients, fast-path test and (FP32) per-cell weights.
-    // Rows outside the interior (ghost rows, rows beyond the array) take
-    // the fast path with zero y coefficients and cy = 1 (FP32: weight 0):
-    // their values stay finite, are never stored and reach interior cells
-    // only through the zero coefficients of the domain's boundary faces.
+    // Per-row coefficients and (FP64) the fast-path test and per-cell
+    // weights.  Rows outside the interior take the fast path with zero y
+    // coefficients and weight 0: their values stay finite, are never stored
+    // and reach interior cells only through the zero coefficients of the
+    // domain's boundary faces.  FP32 weights come from k_weights32 (0
+    // outside the interior and for kept cells); wr = w r is formed here.
     unsigned smask = 0;  // FP64: rows holding a cell off the fast path
 #pragma unroll
     for (int k = 0; k < NRW; k++) {
         const int gj = jt0 + k;
         const bool rowint = gj >= 1 && gj <= ny - 2;
-        const int f = lf[k];
-        if constexpr (sizeof(T) == 8) sfl[w][k][lane] = (uint16_t)f;
         const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
         const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
         R.cyp[k] = uniform(yp);
         R.cym[k] = uniform(ym);
-        const T cy = rowint ? yp + ym : (T)1;
-        bool offfast = false;
-        if (rowint) {
-            const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
-            const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
-            const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
-            const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
-            offfast = __any(!(fa && fb));
-        }
         if constexpr (sizeof(T) == 4) {
-            if (!rowint) {
-                R.wa[k] = 0.0f;
-                R.wb[k] = 0.0f;
-            } else if (!offfast) {
-                // fast-path cells: sa = (cxp + cxm) + (cyp + cym)
-                R.wa[k] = sor_weight(cxa + cy, om);
-                R.wb[k] = sor_weight(cxb + cy, om);
-            } else {
-                // sa over the open faces; the same bits as above for a
-                // fast-path cell
-                const int fa = f & 0xff, fb = f >> 8;
-                const float sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
-                                   (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
-                const float sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
-                                   (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
-                R.wa[k] = ((fa & F_FLUID) && sa_a != 0.0f) ? sor_weight(sa_a, om) : 0.0f;
-                R.wb[k] = ((fb & F_FLUID) && sa_b != 0.0f) ? sor_weight(sa_b, om) : 0.0f;
-            }
             R.ra[k] = R.wa[k] * R.ra[k];
             R.rb[k] = R.wb[k] * R.rb[k];
         } else {
+            const int f = lf[k];
+            sfl[w][k][lane] = (uint16_t)f;
+            bool offfast = false;
+            if (rowint) {
+                const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
+                const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
+                const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
+                const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
+                offfast = __any(!(fa && fb));
+            }
             if (offfast) smask |= 1u << k;
             // the weight om / sa over the open faces (for a fast-path cell the
             // same bits as om / (cx + cy)); 0 for a cell the sweep keeps:

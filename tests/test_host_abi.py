@@ -201,15 +201,16 @@ def test_case_descriptor_validation_without_device():
 
 def test_footprint_by_precision():
     """Device bytes a case needs (host-side plan, no device): the mixed mode
-    adds the FP32 error and residual fields to the FP64 fields; the
-    FP32-velocity mode stores u, v and the sources in 4 bytes."""
+    adds the FP32 error, residual and SOR-weight fields to the FP64 fields;
+    the FP32-velocity mode stores u, v and the sources in 4 bytes."""
     xf = np.linspace(0.0, 16.0, 257)
     fp = {p: iff.Batch.footprint(xf, xf, precision=p, dt=0.01, re=100.0, bodies=[iff.circle(5.0, 8.0, 0.5)])
           for p in (iff.IF_FP64, iff.IF_MIXED, iff.IF_MIXED_V32)}
-    cells = 258 * 258
     assert fp[iff.IF_MIXED] > fp[iff.IF_FP64] > fp[iff.IF_MIXED_V32]
-    # three FP32 fields more than FP64 (e twice, r32), to within the padding
-    assert 10 * cells <= fp[iff.IF_MIXED] - fp[iff.IF_FP64] <= 16 * cells
+    # four FP32 fields more than FP64 (e twice, r32, w32): rows of 258 cells
+    # pitched to 320, each field aligned to 256 bytes
+    pitched = 320 * 258
+    assert 16 * pitched <= fp[iff.IF_MIXED] - fp[iff.IF_FP64] <= 16 * pitched + 4 * 1024
     with pytest.raises(iff.ImmerseFlowError):
         iff.Batch.footprint(xf, xf, precision=7, dt=0.01, re=100.0)
 
