@@ -26,12 +26,26 @@ def _add(batch, c):
     return batch.add(c["xf"], c["yf"], c["steps"], **_kwargs(c))
 
 
+def pressure_err(pm, pf, flags):
+    """max |pm - pf - c| / max(1, max |pf|) over the cells of the Poisson
+    operator (fluid with an open face), with c the mean of pm - pf over
+    them.  The pressure of the all-Neumann problem is fixed only up to a
+    constant there: the iteration never changes that mode and rounding
+    moves it differently in each precision, while the solid cells and the
+    corners of the ghost ring keep 0 in both, so a mean over all cells does
+    not remove it.  (The workloads here have one connected fluid region.)"""
+    f = np.asarray(flags).astype(np.int64)
+    fluid = ((f & 1) != 0) & ((f & 0x3c) != 0)
+    d = np.asarray(pm)[fluid] - np.asarray(pf)[fluid]
+    return float(np.max(np.abs(d - d.mean()))) / max(1.0, float(np.max(np.abs(pf))))
+
+
 def error_vs_fp64(rec, c):
     """The error of a served (mixed-precision) result against the FP64 path:
     runs request c alone in FP64 for the same number of steps and returns
-    max |served - FP64| / max(1, max |FP64|) for u, v and p, and for p with
-    each field's mean removed (p_mod_const).  rec must hold the served
-    fields (serve(..., keep_fields=...))."""
+    max |served - FP64| / max(1, max |FP64|) for u, v and p, and for p
+    modulo the constant of the fluid region (p_mod_const, pressure_err).
+    rec must hold the served fields (serve(..., keep_fields=...))."""
     s = Case(c["xf"], c["yf"], precision=IF_FP64, **_kwargs(c))
     try:
         s.step(rec["steps"])
@@ -39,9 +53,7 @@ def error_vs_fp64(rec, c):
         rel = lambda a, b: float(np.max(np.abs(a - b))) / max(1.0, float(np.max(np.abs(b))))
         for f in ("u", "v", "p"):
             out[f] = rel(rec[f], s.get(f))
-        # pressure of the all-Neumann problem: fixed up to a constant
-        pf = s.get("p")
-        out["p_mod_const"] = rel(rec["p"] - rec["p"].mean(), pf - pf.mean())
+        out["p_mod_const"] = pressure_err(rec["p"], s.get("p"), s.get("flags"))
         return out
     finally:
         s.close()

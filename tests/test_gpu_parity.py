@@ -515,6 +515,34 @@ def test_heaving_cylinder_matches_oracle(require_gpu):
     gc.close()
 
 
+def test_mixed_weights_follow_moving_flags(require_gpu):
+    """Mixed precision with a heaving body: the register-tile kernel loads
+    the FP32 SOR weights formed once per step from the flags, the LDS-tiled
+    kernel forms them from the flags itself.  Both must give the same bits
+    step after step while the body moves and cells change status (stale
+    weights would also pass an accuracy test: the FP64 defect correction
+    hides them)."""
+    xf = np.linspace(0, 4, 81)
+    kw = dict(precision=iff.IF_MIXED, bodies=[iff.heaving_circle(1.5, 2.0, 0.4, 0.5, 0.4)], dt=0.02, re=100.0,
+              ppe_omega=1.8, ppe_itermax=100000, ppe_tol=1e-8, check_every=16)
+    _say("mixed weights with a heaving body")
+    a, b = iff.Case(xf, xf, **kw), iff.Case(xf, xf, **kw)
+    a.tune(4, 64, 8, 16)
+    b.tune(0, 16, 2, 16)
+    moved = 0
+    f0 = a.get("flags")
+    for n in range(6):
+        sa, sb = a.step(1), b.step(1)
+        assert sa["ppe_sweeps"] == sb["ppe_sweeps"], (n, sa, sb)
+        for f in ("u", "v", "p", "uf", "vf", "flags"):
+            assert np.array_equal(a.get(f), b.get(f)), (n, f)
+        moved += int(np.count_nonzero(a.get("flags") != f0))
+        f0 = a.get("flags")
+    assert moved > 0  # cells changed status along the way
+    a.close()
+    b.close()
+
+
 def test_batch_moving_and_static_alone_equals_mix(require_gpu):
     xs = np.linspace(0, 4, 65)
     kw = dict(dt=0.02, re=150.0, ppe_tol=1e-8, check_every=10, ppe_omega=1.8)
