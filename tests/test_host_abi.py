@@ -228,3 +228,27 @@ def test_footprint_of_a_line_solve_case():
                     for sv in (iff.IF_PPE_RBSOR, iff.IF_PPE_LINE))
         want = es * per_row * 258
         assert want <= line - rb <= want + 256, (prec, line - rb)
+
+
+def test_pressure_error_modulo_the_fluid_constant():
+    """serve.pressure_err compares pressures modulo the constant of the
+    Poisson operator's cells (fluid with an open face): a constant offset
+    there is no error, the never-written ring corners and solid cells (0 in
+    both fields) do not count, and a non-constant difference is reported at
+    its size relative to max(1, max|p|)."""
+    from immerseflow_amd import serve
+    ny, nx = 6, 7
+    flags = np.zeros((ny, nx), dtype=np.int64)
+    flags[1:-1, 1:-1] = 1 | 4 | 8 | 16 | 32     # operator cells
+    flags[0, :] = flags[-1, :] = flags[:, 0] = flags[:, -1] = 1  # ring: fluid, no open face
+    flags[3, 3] = 2                              # a ghost (solid) cell
+    rng = np.random.default_rng(5)
+    pf = rng.standard_normal((ny, nx)) * 3.0
+    pf[0, 0] = pf[0, -1] = pf[-1, 0] = pf[-1, -1] = 0.0
+    pf[3, 3] = 0.0
+    pm = pf + 2.5e-5 * (flags & 0x3c != 0)      # the operator cells' constant only
+    assert serve.pressure_err(pm.ravel(), pf.ravel(), flags.ravel()) <= 1e-15 * 4
+    pm2 = pm.copy()
+    pm2[2, 4] += 1e-6 * np.abs(pf).max()         # one cell off the constant
+    e = serve.pressure_err(pm2.ravel(), pf.ravel(), flags.ravel())
+    assert 0.9e-6 <= e <= 1.0e-6
