@@ -116,7 +116,8 @@ typedef struct {
     double ubc, vbc;        /* Dirichlet velocity on the four sides */
     int ad_itermax;
     double ad_tol;          /* L1 sum of |dq| (ADSolver.cu:315), 1e-6 */
-    int ppe_solver;         /* IF_PPE_JACOBI / IF_PPE_RBSOR / IF_PPE_LINE (zebra line SOR) */
+    int ppe_solver;         /* IF_PPE_JACOBI / IF_PPE_RBSOR / IF_PPE_LINE (zebra line SOR; rows of at most
+                               8064 interior cells, IF_ERR_ARG beyond) */
     double ppe_omega;
     int ppe_itermax;
     double ppe_tol;
