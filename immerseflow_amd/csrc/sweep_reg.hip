@@ -232,7 +232,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     constexpr int TY = NW * NRW - 2 * H;  // stored rows per tile
     static_assert(TY > 0 && NRW <= 32, "tile shape");
     using V2 = typename VecT<T, 2>::type;
-    __shared__ V2 xch[2][NW][2][64];  // [parity][wave][first row, last row][lane]
+    // [parity][1 + wave][first row, last row][lane]; slots 0 and NW + 1 stay
+    // zero (the rows beyond the first and last wave), so every wave reads its
+    // neighbour rows unconditionally
+    __shared__ V2 xch[2][NW + 2][2][64];
     // Face flags (slot a in bits 0-7, slot b in 8-15), read by the rows that
     // take the flag path; kept in LDS rather than registers.
     constexpr int kFlw = sizeof(T) == 8 ? NW : 1;  // FP64 only
@@ -455,26 +458,33 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // the first and last row, and ends by publishing those two rows for
     // the next half-sweep.  Within a half-sweep every updated cell reads
     // only cells of the other colour, so the row order changes no result.
+    if (w == 0) {
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                xch[b][0][r][lane] = V2{(T)0, (T)0};
+                xch[b][NW + 1][r][lane] = V2{(T)0, (T)0};
+            }
+    }
     auto half32 = [&]<int Q>(int h) {
         const int b = h & 1;
-        V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
-        if (w > 0) sv = xch[b][w - 1][1][lane];
-        if (w < NW - 1) nv = xch[b][w + 1][0][lane];
+        const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
         [&]<int... K>(std::integer_sequence<int, K...>) {
             (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1),
              ...);
         }(std::make_integer_sequence<int, NRW - 2>{});
         reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
         reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
-        xch[b ^ 1][w][0][lane] = V2{R.pa[0], R.pb[0]};
-        xch[b ^ 1][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        xch[b ^ 1][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[b ^ 1][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         __syncthreads();
     };
     // nst is even: the half-sweeps run in pairs whose colour order is fixed
     // per wave, so each loop body is straight-line code
     auto pairs = [&](int n) {
-        xch[0][w][0][lane] = V2{R.pa[0], R.pb[0]};
-        xch[0][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        xch[0][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[0][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         __syncthreads();
         if (Z == 0) {
             for (int h = 0; h < n; h += 2) {
@@ -497,12 +507,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
         } else {
             for (int h = 0; h < n; h++) {
                 const int b = h & 1;
-                xch[b][w][0][lane] = V2{R.pa[0], R.pb[0]};
-                xch[b][w][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+                xch[b][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+                xch[b][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
                 __syncthreads();
-                V2 sv = V2{(T)0, (T)0}, nv = V2{(T)0, (T)0};
-                if (w > 0) sv = xch[b][w - 1][1][lane];
-                if (w < NW - 1) nv = xch[b][w + 1][0][lane];
+                const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
 #define IFF_HALF(Q_, F_)                                                                              \
     reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
                              cxpb, cxmb, c1)
