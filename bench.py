@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-line", action="store_true")
     ap.add_argument("--line-cases", default="c1,c2", help="line-SOR workloads: c1 (256^2), c2 (2048x1024)")
+    ap.add_argument("--c3-tune", default="",
+                    help="kernel,tile_y,sweeps_per_launch,check_every for the headline instead of the autotuner "
+                         "(profiling runs: the autotuner's choice between close shapes can change under a profiler)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=2000)
     ap.add_argument("--c2-fp64-steps", type=int, default=3)
@@ -191,7 +194,12 @@ def headline_c3(iff, configs, a):
     st1 = c.step(1)
     log("c3 step 1: sweeps", st1["ppe_sweeps"], "div %.2e" % st1["div_max"])
     t0 = time.perf_counter()
-    tr = c.autotune()
+    if a.c3_tune:
+        kind, ty, m, k = (int(x) for x in a.c3_tune.split(","))
+        c.tune(kind, ty, m, k)
+        tr = dict(kernel=kind, tile_y=ty, sweeps_per_launch=m, check_every=k)
+    else:
+        tr = c.autotune()
     t_tune = time.perf_counter() - t0
     log("c3 autotune", tr, "%.3f s" % t_tune)
     for k in range(max(0, a.warmup - 1)):
