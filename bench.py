@@ -94,23 +94,35 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def kernel_name(kind, ty, m, t, pers=False):
-    """The Poisson sweep kernel a tune() choice launches (as rocprofv3 names it)."""
+def kernel_name(kind, ty, m, t, pers=False, wt=False):
+    """The Poisson sweep kernel a tune() choice launches (as rocprofv3 names
+    it); wt: the FP32 weights are looked up by flag pattern."""
     if kind == 0:
         return "k_rbsor<%s, %d, %d>" % (t, ty, m)
     if kind == 1:
         return "k_rbsor_stream<%s, %d, %d>" % (t, m, ty)
     nw = {2: 4, 3: 8, 4: 16}[kind]
-    return "k_rbsor_reg<%s, %d, %d, %d, %s>" % (t, m, nw, (ty + 4 * m) // nw, "true" if pers else "false")
+    b = lambda x: "true" if x else "false"
+    return "k_rbsor_reg<%s, %d, %d, %d, %s, %s>" % (t, m, nw, (ty + 4 * m) // nw, b(pers), b(wt))
 
 
-def reg_updates_per_sweep(NX, NY, kind, ty, m):
+def reg_updates_per_sweep(NX, NY, kind, ty, m, pers=False):
     """Cell updates one red-black sweep of the register-tile kernel executes
-    over an NX x NY interior: every tile computes its whole 128-column,
-    (ty + 4m)-row block (the stored region plus the 2m halo it recomputes)."""
+    over an NX x NY interior, averaged over the 2m half-sweeps of a launch:
+    every tile computes its whole 128-column, (ty + 4m)-row block (the
+    stored region plus the 2m halo it recomputes), except that in a launch
+    per stage a wave whose rows all lie within h + 1 rows of the block's top
+    or bottom edge skips half-sweep h (no row of it can still be exact)."""
     tw = 128 - 4 * m
     ntiles = -(-NX // tw) * -(-NY // ty)
-    return ntiles * 128 * (ty + 4 * m)
+    nw = {2: 4, 3: 8, 4: 16}[kind]
+    rows = (ty + 4 * m) // nw
+    live = 0
+    for h in range(2 * m):
+        for w in range(nw):
+            if pers or (w * rows + rows - 1 > h and nw * rows - 1 - w * rows > h):
+                live += rows
+    return ntiles * 128 * live / (2 * m)
 
 
 def rel_err(a, b):
@@ -216,9 +228,9 @@ def headline_c3(iff, configs, a):
     log("c3 timed %d steps in %.3f s, last sweeps %d, total %d, max div %.2e"
         % (a.steps, dt, st["ppe_sweeps"], st["total_ppe_sweeps"], st["max_div"]))
     avg_s = ms * 1e-3 / max(1, launches)
-    upd = reg_updates_per_sweep(NX, NY, tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"]) * spl
+    upd = int(reg_updates_per_sweep(NX, NY, tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], pers) * spl)
     useful = cells * spl
-    kname = kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float", pers)
+    kname = kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float", pers, st["weight_table"])
     achieved = upd * FLOP_PER_UPDATE / avg_s / 1e12 if avg_s > 0 else 0.0
     roof = dict(bound="valu", achieved=round(achieved, 2), peak=round(FP32_NONPACKED_TFLOPS, 2), unit="TFLOP/s",
                 frac=round(achieved / FP32_NONPACKED_TFLOPS, 4), traffic=None, kernel=kname,
@@ -235,10 +247,12 @@ def headline_c3(iff, configs, a):
                      "the FP32 rate without packed instructions, which this build does not emit (the "
                      "published 157.3 TFLOP/s counts packed FP32)")
     tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c3.json")
+    tj = {}
     if os.path.exists(tp):
-        with open(tp) as f:
-            tj = json.load(f)
-        if tj.get("kernel") == kname and tj.get("sweeps_per_launch") == spl:
+        with open(tp) as f:  # one entry per profiled shape of the headline kernel
+            tj = next((e for e in json.load(f)["kernels"]
+                       if e.get("kernel") == kname and e.get("sweeps_per_launch") == spl), {})
+        if tj:
             roof["traffic"] = tj.get("bytes_per_launch")
             hb = tj["bytes_per_launch"] / avg_s / 1e9 if avg_s > 0 else None
             roof["hbm"] = dict(achieved=round(hb, 1) if hb else None, peak=HBM_PEAK_GBS, unit="GB/s",
@@ -481,6 +495,7 @@ def run_batch(iff, cases, prec, a, window, tune=None):
     ups = (ib["cell_sweeps"] - ia["cell_sweeps"]) / (ms * 1e-3) if ms > 0 else None
     return dict(cps=len(cases) / (t_all * a.batch_steps / timed_steps), t_all=t_all, t_win=t_win, tr=tr,
                 t_tune=t_tune, finished=sum(1 for st in states if st[0] == 1),
+                wt=all(st[2]["weight_table"] for st in states),
                 max_div=max(st[2]["max_div"] for st in states), keep=keep, ups=ups)
 
 
@@ -585,7 +600,8 @@ def batched_c4(iff, configs, a):
                            speedup_vs_fp64_single=round(batch_cps_win / single_cps_win, 2),
                            fp64_cases=len(cases)),
                sweep_gcell_sweeps_per_s=round(mx["ups"] / 1e9, 1) if mx["ups"] else None,
-               autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float"),
+               autotune=dict(kernel=kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float",
+                                                wt=mx["wt"]),
                              check_every=tr["check_every"], candidates=tr["candidates"],
                              seconds=round(mx["t_tune"], 3)),
                fp32_stencils=dict(value=round(v32["cps"], 3), unit="cases/s", finished=v32["finished"],

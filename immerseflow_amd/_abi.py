@@ -56,7 +56,8 @@ class IfStepStats(C.Structure):
     _fields_ = [("steps", C.c_int), ("ad_iters", C.c_int), ("ad_res", C.c_double),
                 ("ppe_sweeps", C.c_int), ("ppe_res", C.c_double), ("div_max", C.c_double),
                 ("total_ad_iters", C.c_longlong), ("total_ppe_sweeps", C.c_longlong),
-                ("max_div", C.c_double), ("failed", C.c_int), ("persistent_timeouts", C.c_int)]
+                ("max_div", C.c_double), ("failed", C.c_int), ("persistent_timeouts", C.c_int),
+                ("weight_table", C.c_int)]
 
 
 class IfGhost(C.Structure):

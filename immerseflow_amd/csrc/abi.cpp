@@ -211,7 +211,8 @@ static if_status tune(Engine &e, int kind, int ty, int m, int k)
                                 "4 sweeps with tile_y 32/48/64 or 8 sweeps with tile_y 32/48/64");
     if (kind == 4 && !reg_variant_ok(pr, 16, ty, m))
         return fail(IF_ERR_ARG, "tune: 16-wave register-tile sweep needs 4 sweeps with tile_y 48/64/80 or 8 "
-                                "sweeps with tile_y 32/64 (96: mixed precision only)");
+                                "sweeps with tile_y 32/64; mixed precision also 8 sweeps with tile_y 96, 9 "
+                                "with 92 or 10 with 88");
     if (kind < 0 || kind > 4) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
     if (k < 1) return fail(IF_ERR_ARG, "tune: check_every must be >= 1");
     e.sweep_kind = kind;

@@ -119,7 +119,7 @@ int Engine::resident() const
 struct ArenaPlan {
     size_t off_u[2], off_v[2], off_sx, off_sy;
     size_t off_uf[2], off_vf[2], off_p[2], off_rhs;
-    size_t off_e[2], off_r32, off_w32;
+    size_t off_e[2], off_r32, off_w32, off_wtab;
     size_t off_lf;
     size_t off_flags, off_bodyid;
     size_t off_metd, off_xc, off_yc, off_metf;
@@ -152,6 +152,8 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
         a.off_w32 = fieldT(4);
+        a.off_wtab = o;
+        o = align256(o + sizeof(float) * 64);
     }
     if (line) {  // the line solve's factors, one stride per row, in the Poisson precision
         int B, S;
@@ -402,6 +404,20 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     std::vector<float> FX, FY;
     axis_vectors(s.dx, d.dt, d.re, d.conv_factor, true, VX, FX);
     axis_vectors(s.dy, d.dt, d.re, d.conv_factor, false, VY, FY);
+    // A mixed case whose FP32 Poisson coefficients are each one value over
+    // the faces that can open (east and north: cells 1..n-3; west and
+    // south: cells 2..n-2) has one weight per pattern of face flags: the
+    // sweep reads the cell's flag byte and looks the weight up instead of
+    // loading a per-cell weight.
+    auto flat = [](const std::vector<float> &F, int n) {
+        const float *cp = &F[(size_t)7 * n], *cm = &F[(size_t)8 * n];
+        for (int i = 2; i <= n - 3; i++)
+            if (cp[i] != cp[1]) return false;
+        for (int i = 3; i <= n - 2; i++)
+            if (cm[i] != cm[2]) return false;
+        return true;
+    };
+    c.wtab = (pprec_ == IF_MIXED && flat(FX, s.nx) && flat(FY, s.ny)) ? (float *)(A + a.off_wtab) : nullptr;
     double *md = (double *)(A + a.off_metd);
     float *mf = (float *)(A + a.off_metf);
     IFF_HIP(hipMemcpyAsync(md, VX.data(), VX.size() * 8, hipMemcpyHostToDevice, st_));
@@ -627,12 +643,12 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
     if (stamp) launch_stamp(st_, d_stamps_, kStamps);
     if (any_rb && pers)  // the whole chunk in one launch
         launch_ppe_sweep_pers(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
-                              K, d_bar_, d_flags_);
+                              K, d_bar_, d_flags_, wt_all_);
     for (int s = 0; s < K; s++) {
         if (any_rb && !pers && s % M == 0) {
             if (sweep_kind >= 2)
                 launch_ppe_sweep_reg(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
-                                     nt_sweep_, s, K);
+                                     nt_sweep_, s, K, wt_all_);
             else if (sweep_kind == 1)
                 launch_ppe_sweep_stream(st_, pprec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
             else
@@ -716,6 +732,11 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
+    wt_all_ = pprec_ == IF_MIXED;
+    for (int id : order_) {
+        const Slot &sl = *slots_[id];
+        if (sl.desc.ppe_solver == IF_PPE_RBSOR && !sl.dev.wtab) wt_all_ = 0;
+    }
     // FP64 default shape, by measurement: 16-wave tiles as persistent
     // chunks of the smallest height (48, 64, 80 rows) whose tiles fit in
     // one launch, i.e. the most tiles the device holds at once (1024^2:
@@ -1018,7 +1039,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             s.last.total_ppe_sweeps = c.tot_ppe;
             s.last.max_div = c.max_div;
             s.last.failed = c.failed;
+            s.last.weight_table = wt_all_;
             if (agg) {
+                agg->weight_table = wt_all_;
                 agg->steps = std::max(agg->steps, 1 + it);
                 agg->ad_iters = c.ad_iter;
                 agg->ad_res = c.ad_res;
@@ -1098,14 +1121,15 @@ if_status Engine::autotune(if_tune_result *out)
     };
     // register tiles of 16, 8 and 4 waves at 8, 4 and 2 sweeps per launch,
     // and the LDS-tiled kernel; shapes a precision lacks are skipped
-    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 32, 8}, {4, 48, 4}, {4, 64, 4},
-                                  {4, 80, 4}, {3, 32, 8}, {3, 48, 8}, {3, 64, 8}, {3, 32, 4},
-                                  {3, 48, 4}, {3, 24, 2}, {2, 16, 4}, {0, 16, 2}};
+    static const Shape cands[] = {{4, 64, 8}, {4, 96, 8}, {4, 92, 9}, {4, 88, 10}, {4, 32, 8},
+                                  {4, 48, 4}, {4, 64, 4}, {4, 80, 4}, {3, 32, 8}, {3, 48, 8},
+                                  {3, 64, 8}, {3, 32, 4}, {3, 48, 4}, {3, 24, 2}, {2, 16, 4},
+                                  {0, 16, 2}};
     constexpr int kReps = 6;
     auto sweep_launch = [&](const Shape &c, int s, int K) {
         if (c.kind >= 2)
             launch_ppe_sweep_reg(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
-                                 s, K);
+                                 s, K, wt_all_);
         else if (c.kind == 1)
             launch_ppe_sweep_stream(st_, pprec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
         else
@@ -1154,7 +1178,7 @@ if_status Engine::autotune(if_tune_result *out)
         auto timed = [&]() {
             if (pc)
                 launch_ppe_sweep_pers(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
-                                      d_bar_, d_flags_);
+                                      d_bar_, d_flags_, wt_all_);
             else
                 sweep_launch(c, s0, 2 * c.m);
         };

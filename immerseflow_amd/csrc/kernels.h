@@ -21,8 +21,9 @@ void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const Case
 // register-tile red-black SOR (sweep_reg.hip)
 bool reg_variant_ok(int prec, int waves, int ty, int m);
 int reg_out_cols(int m);
+// wt (FP32): every case of the tile table has a weight table (CaseDev::wtab)
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
-                          const int2 *tiles, int nt, int s, int K);
+                          const int2 *tiles, int nt, int s, int K, int wt);
 // The K sweeps of a chunk in one launch of the register-tile kernel, in
 // stages of m sweeps with a halo exchange between neighbouring tiles after
 // each (prec: the Poisson sweep precision).  pers_fits: the shape has a
@@ -33,7 +34,7 @@ void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, co
 bool pers_fits(int prec, int waves, int ty, int m, int nt);
 int pers_blocks(int prec, int waves, int ty, int m);  // 0: no persistent form
 void launch_ppe_sweep_pers(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
-                           const int2 *tiles, int nt, int K, GridBar *bar, unsigned *flags);
+                           const int2 *tiles, int nt, int K, GridBar *bar, unsigned *flags, int wt);
 void launch_ppe_check(hipStream_t st, int prec, const CaseDev *cs, int ncase, const int2 *tiles,
                       int nt, int K, const GridBar *bar = nullptr);
 void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase);

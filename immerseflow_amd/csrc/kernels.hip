@@ -1217,11 +1217,31 @@ __global__ void k_ppe_rearm(const CaseDev *cs, int ncase)
 // open faces (sor_weight, the expression every FP32 sweep kernel uses), 0
 // for a cell the sweep keeps (not fluid, or no open face).  Once per step,
 // after the flags are final: the register-tile kernel loads it instead of
-// forming it from the flags on every launch.  One block per row.
+// forming it from the flags on every launch.  One block per row.  On a grid
+// whose coefficients are each one value (wtab) the weight depends on the
+// flags alone: the first block also writes the weight of each pattern of
+// flag bits 0-5, with the same expression, for launches whose cases all
+// have such a table.
 __global__ __launch_bounds__(256) void k_weights32(const CaseDev *cs)
 {
     const CaseDev &c = cs[blockIdx.y];
     if (!c.w32 || !c.ctrl->active) return;  // FP64 Poisson cases hold no FP32 weights
+    if (c.wtab && blockIdx.x == 0) {
+        if (threadIdx.x < 64) {
+            const unsigned f = threadIdx.x;
+            const float om = c.omega_f;
+            float w = 0.0f;
+            if (f & F_FLUID) {
+                const float aE = (f & F_OPEN_E) ? G(c.xf.cp)[1] : 0.0f;
+                const float aW = (f & F_OPEN_W) ? G(c.xf.cm)[2] : 0.0f;
+                const float aN = (f & F_OPEN_N) ? G(c.yf.cp)[1] : 0.0f;
+                const float aS = (f & F_OPEN_S) ? G(c.yf.cm)[2] : 0.0f;
+                const float sa = (aE + aW) + (aN + aS);
+                if (sa != 0.0f) w = sor_weight(sa, om);
+            }
+            GW(c.wtab)[f] = w;
+        }
+    }
     const int j = 1 + (int)blockIdx.x;
     if (j > c.ny - 2) return;
     const size_t row = (size_t)j * c.P;

@@ -144,6 +144,11 @@ typedef struct {
     int persistent_timeouts; /* steps whose persistent Poisson chunk timed out
                                 (blocks not all resident) and was solved again
                                 with one launch per stage */
+    int weight_table;       /* last step: the FP32 red-black sweeps looked each
+                               cell's weight up by its face flags (every
+                               red-black case resident has a grid whose FP32
+                               coefficients are each one value) instead of
+                               loading a per-cell weight */
 } if_step_stats;
 
 typedef struct {
@@ -239,7 +244,9 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * kernel 3: the same with eight wave64s per tile, tile height 24/40/56
  * (2 sweeps per launch), 32/48/64 (4) or 32/48/64 (8); kernel 4: sixteen
  * wave64s per tile, tile height 48/64/80 (4 sweeps per launch) or 32/64/96
- * (8; 96 in mixed precision only).  Several red-black sweeps per launch is
+ * (8), and in mixed precision only also 96 (8), 92 (9) and 88 (10; the
+ * tile computes 128 x 128 cells, stores 128 - 4m of them in each direction
+ * and recomputes the rest).  Several red-black sweeps per launch is
  * temporal blocking; check_every is the number of sweeps between
  * convergence checks.  Results do not depend on the kernel, the tile
  * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps

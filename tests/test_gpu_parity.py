@@ -192,18 +192,23 @@ def test_sweep_variants_are_identical(require_gpu, kind, ty, m):
         gc.close()
 
 
+FP32_ONLY_SHAPES = [(4, 96, 8), (4, 92, 9), (4, 88, 10), (4, 80, 12)]
+
+
 @pytest.mark.parametrize("kind,ty,m", [(2, 16, 2), (2, 32, 4), (2, 48, 1), (3, 40, 2), (3, 64, 4), (3, 32, 8),
-                                         (4, 64, 8)])
+                                         (4, 64, 8)] + FP32_ONLY_SHAPES)
 def test_register_tile_across_tiles_and_edges(require_gpu, kind, ty, m):
     """Register-tile sweep on a grid several tiles wide: bodies cut by tile
     boundaries and lying next to the domain edge, partial tiles at the east
-    and north edges; FP64 equal to the oracle, mixed equal to the LDS kernel."""
+    and north edges; FP64 equal to the oracle, mixed equal to the LDS kernel
+    (the 16-wave tiles of 96 rows and more sweeps per launch are mixed
+    precision only)."""
     xf = stretched_faces(300, 6.0, 1.002)
     yf = stretched_faces(140, 2.8, 0.997)
     bodies = [iff.ellipse(2.4, 1.3, 0.45, 0.1, np.radians(20.0)),
               iff.circle(5.72, 0.35, 0.2)]
     _say("register tile edges", kind, ty, m)
-    for prec in (iff.IF_FP64, iff.IF_MIXED):
+    for prec in (iff.IF_MIXED,) if (kind, ty, m) in FP32_ONLY_SHAPES else (iff.IF_FP64, iff.IF_MIXED):
         oc, gc = _pair(xf, yf, bodies, prec=prec, dt=0.005, check_every=9, ppe_tol=1e-8)
         gc.tune(kind, ty, m, 9)
         ref = None
@@ -434,7 +439,7 @@ def test_autotune_leaves_state_and_matches_manual_tune(require_gpu, prec):
     kw = dict(precision=prec, bodies=bodies, dt=0.01, re=100.0, ppe_omega=1.8, ppe_itermax=200000,
               ppe_tol=1e-9, check_every=10)
     a, b = iff.Case(xf, yf, **kw), iff.Case(xf, yf, **kw)
-    ncand = 13 if prec == iff.IF_FP64 else 14   # the 16-wave x 96-row tile is FP32 only
+    ncand = 13 if prec == iff.IF_FP64 else 16   # the 16-wave tiles of 96/92/88 rows are FP32 only
     r0 = a.autotune()                       # before any step: interval unchanged
     assert r0["typical_sweeps"] == 0 and r0["check_every"] == 10 and r0["candidates"] == ncand
     a.step(2)
@@ -468,7 +473,7 @@ def test_batch_autotune(require_gpu):
         bt.step(1)
         bs.append((bt, sl))
     r = bs[0][0].autotune()
-    assert r["candidates"] == 14 and r["typical_sweeps"] > 0
+    assert r["candidates"] == 16 and r["typical_sweeps"] > 0
     bs[1][0].tune(r["kernel"], r["tile_y"], r["sweeps_per_launch"], r["check_every"])
     for bt, _ in bs:
         bt.step(2)
@@ -541,6 +546,42 @@ def test_mixed_weights_follow_moving_flags(require_gpu):
     assert moved > 0  # cells changed status along the way
     a.close()
     b.close()
+
+
+def test_mixed_weight_table_equals_per_cell_weights(require_gpu):
+    """On a grid whose FP32 Poisson coefficients are each one value (spacing
+    1/16) the register-tile kernel looks each cell's weight up by its face
+    flags.  Run alone, a static and a heaving case take that form; in a
+    batch with a stretched case every case loads per-cell weights.  Both
+    forms, and the LDS-tiled kernel (which forms the weights itself), must
+    give the same bits step after step."""
+    xu = np.linspace(0.0, 4.0, 65)
+    xs = stretched_faces(70, 4.0, 1.01)
+    kw = dict(dt=0.02, re=100.0, ppe_omega=1.8, ppe_itermax=100000, ppe_tol=1e-8, check_every=16)
+    bodies = {"static": [iff.circle(1.5, 2.0, 0.4)], "heave": [iff.heaving_circle(1.5, 2.0, 0.4, 0.5, 0.4)]}
+    _say("mixed weight table vs per-cell weights")
+    for name, bd in bodies.items():
+        alone = iff.Case(xu, xu, precision=iff.IF_MIXED, bodies=bd, **kw)
+        lds = iff.Case(xu, xu, precision=iff.IF_MIXED, bodies=bd, **kw)
+        alone.tune(4, 64, 8, 16)
+        lds.tune(0, 16, 2, 16)
+        b = iff.Batch(iff.IF_MIXED)
+        b.tune(4, 64, 8, 16)
+        s0 = b.add(xu, xu, 4, bodies=bd, **kw)
+        b.add(xs, xs, 4, bodies=[iff.circle(2.0, 2.0, 0.5)], **kw)
+        for n in range(4):
+            sa, sl = alone.step(1), lds.step(1)
+            assert sa["weight_table"] == 1 and sl["weight_table"] == 1, (name, n)
+            assert sa["ppe_sweeps"] == sl["ppe_sweeps"], (name, n)
+        b.step(4)
+        st = b.state(s0)[2]
+        assert st["weight_table"] == 0 and st["total_ppe_sweeps"] > 0, name
+        for f in ("u", "v", "p", "uf", "vf", "flags"):
+            assert np.array_equal(alone.get(f), lds.get(f)), (name, f)
+            assert np.array_equal(alone.get(f), b.get(s0, f)), (name, f)
+        alone.close()
+        lds.close()
+        b.close()
 
 
 def test_batch_moving_and_static_alone_equals_mix(require_gpu):
