@@ -320,389 +320,386 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // every case of the launch has) instead of loaded per cell (w32).  Two
     // kernels rather than a runtime branch, which would hold both forms'
     // registers at once.
-    auto body = [&](auto wt_) {
-        RegTile<T, NRW> R;
-        int lf[NRW];  // FP64: flags, slot a in bits 0-7, slot b in bits 8-15
-        // FP32: the cells' weights om / sa, formed once per step (k_weights32),
-        // or on a grid with one value per coefficient the weight of the cell's
-        // flag pattern: lane q holds the table's entry q, and each cell reads
-        // its entry across the wave (ds_bpermute)
-        const float *w32 = c.w32;
-        constexpr bool wuni = decltype(wt_)::value;
-        int wtv = 0;
-        if constexpr (wuni) wtv = __float_as_int(G(c.wtab)[lane]);
-        // Loads.  A tile lying inside the interior (the common case) loads
-        // every row with aligned 8-byte vectors and no per-lane tests.
-        const bool tile_in = c0 >= 1 && c0 + 127 <= nx - 2 && jt0 >= 1 && jt0 + NRW - 1 <= ny - 2;
-        if (tile_in) {
-    #pragma unroll
-            for (int k = 0; k < NRW; k++) {
-                const size_t ro = (size_t)(jt0 + k) * P;  // wave-uniform row offset
-                const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + ro + ga);
-                const V2 rv = gload_vec<V2>(rhs + ro + ga);
-                R.pa[k] = pv[0];
-                R.pb[k] = pv[1];
-                R.ra[k] = rv[0];
-                R.rb[k] = rv[1];
-                if constexpr (sizeof(T) == 4) {
-                    if constexpr (wuni) {
-                        const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
-                        lf[k] = (int)fb[0] | ((int)fb[1] << 8);
-                    } else {
-                        const V2 wv = gload_vec<V2>(w32 + ro + ga);
-                        R.wa[k] = wv[0];
-                        R.wb[k] = wv[1];
-                        lf[k] = 0;
-                    }
-                } else {
+    static_assert(!WT || sizeof(T) == 4, "weight tables are FP32");
+    RegTile<T, NRW> R;
+    int lf[NRW];  // FP64: flags, slot a in bits 0-7, slot b in bits 8-15
+    // FP32: the cells' weights om / sa, formed once per step (k_weights32),
+    // or on a grid with one value per coefficient the weight of the cell's
+    // flag pattern: lane q holds the table's entry q, and each cell reads
+    // its entry across the wave (ds_bpermute)
+    const float *w32 = c.w32;
+    constexpr bool wuni = WT;
+    int wtv = 0;
+    if constexpr (wuni) wtv = __float_as_int(G(c.wtab)[lane]);
+    // Loads.  A tile lying inside the interior (the common case) loads
+    // every row with aligned 8-byte vectors and no per-lane tests.
+    const bool tile_in = c0 >= 1 && c0 + 127 <= nx - 2 && jt0 >= 1 && jt0 + NRW - 1 <= ny - 2;
+    if (tile_in) {
+#pragma unroll
+        for (int k = 0; k < NRW; k++) {
+            const size_t ro = (size_t)(jt0 + k) * P;  // wave-uniform row offset
+            const V2 pv = zero_src ? V2{(T)0, (T)0} : gload_vec<V2>(src + ro + ga);
+            const V2 rv = gload_vec<V2>(rhs + ro + ga);
+            R.pa[k] = pv[0];
+            R.pb[k] = pv[1];
+            R.ra[k] = rv[0];
+            R.rb[k] = rv[1];
+            if constexpr (sizeof(T) == 4) {
+                if constexpr (wuni) {
                     const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
                     lf[k] = (int)fb[0] | ((int)fb[1] << 8);
+                } else {
+                    const V2 wv = gload_vec<V2>(w32 + ro + ga);
+                    R.wa[k] = wv[0];
+                    R.wb[k] = wv[1];
+                    lf[k] = 0;
                 }
-            }
-        } else {
-    #pragma unroll
-            for (int k = 0; k < NRW; k++) {
-                const int gj = jt0 + k;
-                const bool rowin = gj >= 0 && gj < ny;
-                const bool rowint = gj >= 1 && gj <= ny - 2;
-                T a = (T)0, b = (T)0, x = (T)0, y = (T)0, wa = (T)0, wb = (T)0;
-                int f = 0;
-                if (rowin) {
-                    const size_t o = (size_t)gj * P + ga;
-                    if (!zero_src) {
-                        if (ga >= 0 && ga + 1 < nx) {
-                            const V2 v = gload_vec<V2>(src + o);
-                            a = v[0];
-                            b = v[1];
-                        } else {
-                            if (ga >= 0 && ga < nx) a = G(src)[o];
-                            if (ga + 1 >= 0 && ga + 1 < nx) b = G(src)[o + 1];
-                        }
-                    }
-                    if (rowint) {
-                        if (ina && inb) {
-                            const V2 v = gload_vec<V2>(rhs + o);
-                            x = v[0];
-                            y = v[1];
-                            if (sizeof(T) == 4 && !wuni) {
-                                const V2 wv = gload_vec<V2>(w32 + o);
-                                wa = wv[0];
-                                wb = wv[1];
-                            } else {
-                                const u8x2 fb = gload_vec<u8x2>(c.flags + o);
-                                f = (int)fb[0] | ((int)fb[1] << 8);
-                            }
-                        } else {
-                            if (ina) {
-                                x = G(rhs)[o];
-                                if (sizeof(T) == 4 && !wuni)
-                                    wa = G(w32)[o];
-                                else
-                                    f = G(c.flags)[o];
-                            }
-                            if (inb) {
-                                y = G(rhs)[o + 1];
-                                if (sizeof(T) == 4 && !wuni)
-                                    wb = G(w32)[o + 1];
-                                else
-                                    f |= (int)G(c.flags)[o + 1] << 8;
-                            }
-                        }
-                    }
-                }
-                R.pa[k] = a;
-                R.pb[k] = b;
-                R.ra[k] = x;
-                R.rb[k] = y;
-                if constexpr (sizeof(T) == 4) {
-                    R.wa[k] = wa;
-                    R.wb[k] = wb;
-                }
-                lf[k] = f;
+            } else {
+                const u8x2 fb = gload_vec<u8x2>(c.flags + ro + ga);
+                lf[k] = (int)fb[0] | ((int)fb[1] << 8);
             }
         }
-
-        // Per-row coefficients and (FP64) the fast-path test and per-cell
-        // weights.  Rows outside the interior take the fast path with zero y
-        // coefficients and weight 0: their values stay finite, are never stored
-        // and reach interior cells only through the zero coefficients of the
-        // domain's boundary faces.  FP32 weights come from k_weights32 (0
-        // outside the interior and for kept cells); wr = w r is formed here.
-        unsigned smask = 0;  // FP64: rows holding a cell off the fast path
-    #pragma unroll
+    } else {
+#pragma unroll
         for (int k = 0; k < NRW; k++) {
             const int gj = jt0 + k;
+            const bool rowin = gj >= 0 && gj < ny;
             const bool rowint = gj >= 1 && gj <= ny - 2;
-            const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
-            const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
-            R.cyp[k] = uniform(yp);
-            R.cym[k] = uniform(ym);
-            if constexpr (sizeof(T) == 4) {
-                if constexpr (wuni) {  // a cell outside the interior has flags 0: weight 0
-                    R.wa[k] = __int_as_float(__builtin_amdgcn_ds_bpermute((lf[k] & 63) << 2, wtv));
-                    R.wb[k] = __int_as_float(__builtin_amdgcn_ds_bpermute(((lf[k] >> 8) & 63) << 2, wtv));
-                }
-                R.ra[k] = R.wa[k] * R.ra[k];
-                R.rb[k] = R.wb[k] * R.rb[k];
-            } else {
-                const int f = lf[k];
-                sfl[w][k][lane] = (uint16_t)f;
-                bool offfast = false;
-                if (rowint) {
-                    const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
-                    const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
-                    const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
-                    const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
-                    offfast = __any(!(fa && fb));
-                }
-                if (offfast) smask |= 1u << k;
-                // the weight om / sa over the open faces (for a fast-path cell the
-                // same bits as om / (cx + cy)); 0 for a cell the sweep keeps:
-                // outside the interior, not fluid, or without an open face
-                const int fa = f & 0xff, fb = f >> 8;
-                const T sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
-                               (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
-                const T sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
-                               (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
-                R.wa[k] = (rowint && (fa & F_FLUID) && sa_a != (T)0) ? sor_weight64(sa_a, om) : (T)0;
-                R.wb[k] = (rowint && (fb & F_FLUID) && sa_b != (T)0) ? sor_weight64(sa_b, om) : (T)0;
-            }
-        }
-        const T c1 = (T)1 - om;
-
-        const int Z = (c0 + jt0) & 1;  // parity of slot a in the wave's first row
-        // Half-sweep h has colour h & 1, and slot a of row k has parity Z + k,
-        // so rows with (k & 1) == Q update slot a where Q = (Z ^ h) & 1.  A
-        // wave of fast-path rows (FP32: every wave) runs branch-free.
-        // FP32: each half-sweep first updates the rows that do not touch the
-        // neighbouring waves (hiding the LDS read of the neighbour rows), then
-        // the first and last row, and ends by publishing those two rows for
-        // the next half-sweep.  Within a half-sweep every updated cell reads
-        // only cells of the other colour, so the row order changes no result.
-        if (w == 0) {
-    #pragma unroll
-            for (int b = 0; b < 2; b++)
-    #pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    xch[b][0][r][lane] = V2{(T)0, (T)0};
-                    xch[b][NW + 1][r][lane] = V2{(T)0, (T)0};
-                }
-        }
-        // Tile row rr holds the value of the full sweep after half-sweep h only
-        // while rr > h and NW * NRW - 1 - rr > h: each half-sweep loses one
-        // row at each edge of the tile.  A wave none of whose rows can still
-        // be valid (all of them in the halo, which is H = 2M deep) skips the
-        // update and the publish and only joins the barrier; the rows next to
-        // it read its stale values only when they are no longer valid either.
-        // That frees issue slots for the other workgroup on the CU; the
-        // persistent form (one workgroup per CU, every wave waiting at the
-        // barrier for the slowest SIMD) runs every wave.
-        auto live = [&](int h) { return PERS || (w * NRW + NRW - 1 > h && NW * NRW - 1 - w * NRW > h); };
-        auto half32 = [&]<int Q>(int h) {
-            const int b = h & 1;
-            if (live(h)) {
-                const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
-                [&]<int... K>(std::integer_sequence<int, K...>) {
-                    (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1),
-                     ...);
-                }(std::make_integer_sequence<int, NRW - 2>{});
-                reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
-                reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
-                xch[b ^ 1][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
-                xch[b ^ 1][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
-            }
-            __syncthreads();
-        };
-        // nst is even: the half-sweeps run in pairs whose colour order is fixed
-        // per wave, so each loop body is straight-line code
-        auto pairs = [&](int n) {
-            xch[0][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
-            xch[0][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
-            __syncthreads();
-            if (Z == 0) {
-                for (int h = 0; h < n; h += 2) {
-                    half32.template operator()<0>(h);
-                    half32.template operator()<1>(h + 1);
-                }
-            } else {
-                for (int h = 0; h < n; h += 2) {
-                    half32.template operator()<1>(h);
-                    half32.template operator()<0>(h + 1);
-                }
-            }
-        };
-        // n half-sweeps (n even).  FP64: one loop body picks the colour order
-        // per half-sweep (two copies of the unrolled pair would exceed the
-        // register budget)
-        auto sweeps = [&](int n) {
-            if constexpr (sizeof(T) == 4) {
-                pairs(n);
-            } else {
-                for (int h = 0; h < n; h++) {
-                    const int b = h & 1;
-                    xch[b][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
-                    xch[b][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
-                    __syncthreads();
-                    const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
-    #define IFF_HALF(Q_, F_)                                                                              \
-        reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
-                                 cxpb, cxmb, c1)
-                    if (smask == 0) {
-                        if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+            T a = (T)0, b = (T)0, x = (T)0, y = (T)0, wa = (T)0, wb = (T)0;
+            int f = 0;
+            if (rowin) {
+                const size_t o = (size_t)gj * P + ga;
+                if (!zero_src) {
+                    if (ga >= 0 && ga + 1 < nx) {
+                        const V2 v = gload_vec<V2>(src + o);
+                        a = v[0];
+                        b = v[1];
                     } else {
-                        if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+                        if (ga >= 0 && ga < nx) a = G(src)[o];
+                        if (ga + 1 >= 0 && ga + 1 < nx) b = G(src)[o + 1];
                     }
-    #undef IFF_HALF
+                }
+                if (rowint) {
+                    if (ina && inb) {
+                        const V2 v = gload_vec<V2>(rhs + o);
+                        x = v[0];
+                        y = v[1];
+                        if (sizeof(T) == 4 && !wuni) {
+                            const V2 wv = gload_vec<V2>(w32 + o);
+                            wa = wv[0];
+                            wb = wv[1];
+                        } else {
+                            const u8x2 fb = gload_vec<u8x2>(c.flags + o);
+                            f = (int)fb[0] | ((int)fb[1] << 8);
+                        }
+                    } else {
+                        if (ina) {
+                            x = G(rhs)[o];
+                            if (sizeof(T) == 4 && !wuni)
+                                wa = G(w32)[o];
+                            else
+                                f = G(c.flags)[o];
+                        }
+                        if (inb) {
+                            y = G(rhs)[o + 1];
+                            if (sizeof(T) == 4 && !wuni)
+                                wb = G(w32)[o + 1];
+                            else
+                                f |= (int)G(c.flags)[o + 1] << 8;
+                        }
+                    }
                 }
             }
-        };
-        // store the stored region of the tile (rows H .. H+TY-1, columns H ..
-        // H+TW-1), or only its border strips of depth H (the halo of the
-        // neighbouring tiles)
-        auto store = [&](T *out, bool border) {
+            R.pa[k] = a;
+            R.pb[k] = b;
+            R.ra[k] = x;
+            R.rb[k] = y;
+            if constexpr (sizeof(T) == 4) {
+                R.wa[k] = wa;
+                R.wb[k] = wb;
+            }
+            lf[k] = f;
+        }
+    }
+
+    // Per-row coefficients and (FP64) the fast-path test and per-cell
+    // weights.  Rows outside the interior take the fast path with zero y
+    // coefficients and weight 0: their values stay finite, are never stored
+    // and reach interior cells only through the zero coefficients of the
+    // domain's boundary faces.  FP32 weights come from k_weights32 (0
+    // outside the interior and for kept cells); wr = w r is formed here.
+    unsigned smask = 0;  // FP64: rows holding a cell off the fast path
+#pragma unroll
+    for (int k = 0; k < NRW; k++) {
+        const int gj = jt0 + k;
+        const bool rowint = gj >= 1 && gj <= ny - 2;
+        const T yp = (rowint && gj != ny - 2) ? G(cyp_)[gj] : (T)0;
+        const T ym = (rowint && gj != 1) ? G(cym_)[gj] : (T)0;
+        R.cyp[k] = uniform(yp);
+        R.cym[k] = uniform(ym);
+        if constexpr (sizeof(T) == 4) {
+            if constexpr (wuni) {  // a cell outside the interior has flags 0: weight 0
+                R.wa[k] = __int_as_float(__builtin_amdgcn_ds_bpermute((lf[k] & 63) << 2, wtv));
+                R.wb[k] = __int_as_float(__builtin_amdgcn_ds_bpermute(((lf[k] >> 8) & 63) << 2, wtv));
+            }
+            R.ra[k] = R.wa[k] * R.ra[k];
+            R.rb[k] = R.wb[k] * R.rb[k];
+        } else {
+            const int f = lf[k];
+            sfl[w][k][lane] = (uint16_t)f;
+            bool offfast = false;
+            if (rowint) {
+                const int y4 = F_FLUID | (gj != ny - 2 ? F_OPEN_N : 0) | (gj != 1 ? F_OPEN_S : 0);
+                const int ea = y4 | xa, eb = y4 | xb;  // the fast-path pattern of each slot
+                const bool fa = !ina || (ea != F_FLUID && (f & kFaces) == ea);
+                const bool fb = !inb || (eb != F_FLUID && ((f >> 8) & kFaces) == eb);
+                offfast = __any(!(fa && fb));
+            }
+            if (offfast) smask |= 1u << k;
+            // the weight om / sa over the open faces (for a fast-path cell the
+            // same bits as om / (cx + cy)); 0 for a cell the sweep keeps:
+            // outside the interior, not fluid, or without an open face
+            const int fa = f & 0xff, fb = f >> 8;
+            const T sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
+                           (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
+            const T sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
+                           (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
+            R.wa[k] = (rowint && (fa & F_FLUID) && sa_a != (T)0) ? sor_weight64(sa_a, om) : (T)0;
+            R.wb[k] = (rowint && (fb & F_FLUID) && sa_b != (T)0) ? sor_weight64(sa_b, om) : (T)0;
+        }
+    }
+    const T c1 = (T)1 - om;
+
+    const int Z = (c0 + jt0) & 1;  // parity of slot a in the wave's first row
+    // Half-sweep h has colour h & 1, and slot a of row k has parity Z + k,
+    // so rows with (k & 1) == Q update slot a where Q = (Z ^ h) & 1.  A
+    // wave of fast-path rows (FP32: every wave) runs branch-free.
+    // FP32: each half-sweep first updates the rows that do not touch the
+    // neighbouring waves (hiding the LDS read of the neighbour rows), then
+    // the first and last row, and ends by publishing those two rows for
+    // the next half-sweep.  Within a half-sweep every updated cell reads
+    // only cells of the other colour, so the row order changes no result.
+    if (w == 0) {
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                xch[b][0][r][lane] = V2{(T)0, (T)0};
+                xch[b][NW + 1][r][lane] = V2{(T)0, (T)0};
+            }
+    }
+    // Tile row rr holds the value of the full sweep after half-sweep h only
+    // while rr > h and NW * NRW - 1 - rr > h: each half-sweep loses one
+    // row at each edge of the tile.  A wave none of whose rows can still
+    // be valid (all of them in the halo, which is H = 2M deep) skips the
+    // update and the publish and only joins the barrier; the rows next to
+    // it read its stale values only when they are no longer valid either.
+    // That frees issue slots for the other workgroup on the CU; the
+    // persistent form (one workgroup per CU, every wave waiting at the
+    // barrier for the slowest SIMD) runs every wave.
+    auto live = [&](int h) { return PERS || (w * NRW + NRW - 1 > h && NW * NRW - 1 - w * NRW > h); };
+    auto half32 = [&]<int Q>(int h) {
+        const int b = h & 1;
+        if (live(h)) {
+            const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
+            [&]<int... K>(std::integer_sequence<int, K...>) {
+                (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1),
+                 ...);
+            }(std::make_integer_sequence<int, NRW - 2>{});
+            reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
+            reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
+            xch[b ^ 1][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+            xch[b ^ 1][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        }
+        __syncthreads();
+    };
+    // nst is even: the half-sweeps run in pairs whose colour order is fixed
+    // per wave, so each loop body is straight-line code
+    auto pairs = [&](int n) {
+        xch[0][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+        xch[0][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+        __syncthreads();
+        if (Z == 0) {
+            for (int h = 0; h < n; h += 2) {
+                half32.template operator()<0>(h);
+                half32.template operator()<1>(h + 1);
+            }
+        } else {
+            for (int h = 0; h < n; h += 2) {
+                half32.template operator()<1>(h);
+                half32.template operator()<0>(h + 1);
+            }
+        }
+    };
+    // n half-sweeps (n even).  FP64: one loop body picks the colour order
+    // per half-sweep (two copies of the unrolled pair would exceed the
+    // register budget)
+    auto sweeps = [&](int n) {
+        if constexpr (sizeof(T) == 4) {
+            pairs(n);
+        } else {
+            for (int h = 0; h < n; h++) {
+                const int b = h & 1;
+                xch[b][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
+                xch[b][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
+                __syncthreads();
+                const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
+#define IFF_HALF(Q_, F_)                                                                              \
+    reg_half<T, NRW, Q_, F_>(R, sfl[kFlw > 1 ? w : 0], lane, smask, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, \
+                             cxpb, cxmb, c1)
+                if (smask == 0) {
+                    if (((Z ^ h) & 1) == 0) IFF_HALF(0, true); else IFF_HALF(1, true);
+                } else {
+                    if (((Z ^ h) & 1) == 0) IFF_HALF(0, false); else IFF_HALF(1, false);
+                }
+#undef IFF_HALF
+            }
+        }
+    };
+    // store the stored region of the tile (rows H .. H+TY-1, columns H ..
+    // H+TW-1), or only its border strips of depth H (the halo of the
+    // neighbouring tiles)
+    auto store = [&](T *out, bool border) {
+        if (lane >= H / 2 && lane < 64 - H / 2) {
+            const bool lb = lane < H || lane >= 64 - H;
+#pragma unroll
+            for (int k = 0; k < NRW; k++) {
+                const int rr = w * NRW + k;
+                const int gj = jt0 + k;
+                if (rr < H || rr >= H + TY || gj > ny - 2) continue;
+                if (border && !lb && rr >= 2 * H && rr < TY) continue;
+                T *row = out + (size_t)gj * P;  // wave-uniform
+                if (ga + 1 <= nx - 2)
+                    gstore_vec<V2>(row + ga, V2{R.pa[k], R.pb[k]});
+                else if (ga <= nx - 2)
+                    GW(row)[ga] = R.pa[k];
+            }
+        }
+    };
+    if constexpr (PERS) {
+        using U2 = unsigned __attribute__((ext_vector_type(2)));
+        using U4 = unsigned __attribute__((ext_vector_type(4)));
+        const int nexec = (kexec + M - 1) / M;  // stages with sweeps for this case
+        const unsigned f0 = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the neighbouring tiles of the case (lanes 0-7 of wave 0)
+        int nb = -1;
+        if (w == 0 && lane < 8) {
+            const int d = lane < 4 ? lane : lane + 1;  // 3 x 3 without the centre
+            const int ntys = (ny - 2 + TY - 1) / TY;
+            const int qx = tx + d % 3 - 1, qy = ty + d / 3 - 1;
+            if (qx >= 0 && qx < ntxs && qy >= 0 && qy < ntys) nb = t - tl.y + qy * ntxs + qx;
+        }
+        const int nrec = P * ny * (int)sizeof(T);  // bytes of a field
+        // buffer offsets: the lane's column in the vector offset, the row
+        // (wave-uniform) in the scalar offset; every access below has
+        // ga >= 0 and 1 <= gj <= ny - 2, so both are non-negative
+        const int voff = ga * (int)sizeof(T);
+        // the chunk's two buffers: stage l writes buffer (l + 1) & 1
+        void *buf0, *buf1;
+        if constexpr (sizeof(T) == 8) {
+            buf0 = c.p[ct->ppe_base & 1];
+            buf1 = c.p[(ct->ppe_base + 1) & 1];
+        } else {
+            buf0 = c.e[0];
+            buf1 = c.e[1];
+        }
+        // the descriptors are built from wave-uniform (readfirstlane)
+        // pointers: held in scalar registers, with no per-lane loop around
+        // the buffer accesses
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf0), 0, nrec, kRsrc3);
+        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf1), 0, nrec, kRsrc3);
+        for (int l = 0; l < nexec; l++) {
+            int me = kexec - l * M;
+            if (me > M) me = M;
+            sweeps(2 * me);
+            const bool last = l + 1 == nexec;
+            const __amdgpu_buffer_rsrc_t ro = ((l + 1) & 1) ? rs1 : rs0;
+            // stage output at device scope: the border strips, or (last
+            // stage of the case) the whole stored region
             if (lane >= H / 2 && lane < 64 - H / 2) {
                 const bool lb = lane < H || lane >= 64 - H;
-    #pragma unroll
+#pragma unroll
                 for (int k = 0; k < NRW; k++) {
                     const int rr = w * NRW + k;
                     const int gj = jt0 + k;
                     if (rr < H || rr >= H + TY || gj > ny - 2) continue;
-                    if (border && !lb && rr >= 2 * H && rr < TY) continue;
-                    T *row = out + (size_t)gj * P;  // wave-uniform
-                    if (ga + 1 <= nx - 2)
-                        gstore_vec<V2>(row + ga, V2{R.pa[k], R.pb[k]});
-                    else if (ga <= nx - 2)
-                        GW(row)[ga] = R.pa[k];
-                }
-            }
-        };
-        if constexpr (PERS) {
-            using U2 = unsigned __attribute__((ext_vector_type(2)));
-            using U4 = unsigned __attribute__((ext_vector_type(4)));
-            const int nexec = (kexec + M - 1) / M;  // stages with sweeps for this case
-            const unsigned f0 = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // the neighbouring tiles of the case (lanes 0-7 of wave 0)
-            int nb = -1;
-            if (w == 0 && lane < 8) {
-                const int d = lane < 4 ? lane : lane + 1;  // 3 x 3 without the centre
-                const int ntys = (ny - 2 + TY - 1) / TY;
-                const int qx = tx + d % 3 - 1, qy = ty + d / 3 - 1;
-                if (qx >= 0 && qx < ntxs && qy >= 0 && qy < ntys) nb = t - tl.y + qy * ntxs + qx;
-            }
-            const int nrec = P * ny * (int)sizeof(T);  // bytes of a field
-            // buffer offsets: the lane's column in the vector offset, the row
-            // (wave-uniform) in the scalar offset; every access below has
-            // ga >= 0 and 1 <= gj <= ny - 2, so both are non-negative
-            const int voff = ga * (int)sizeof(T);
-            // the chunk's two buffers: stage l writes buffer (l + 1) & 1
-            void *buf0, *buf1;
-            if constexpr (sizeof(T) == 8) {
-                buf0 = c.p[ct->ppe_base & 1];
-                buf1 = c.p[(ct->ppe_base + 1) & 1];
-            } else {
-                buf0 = c.e[0];
-                buf1 = c.e[1];
-            }
-            // the descriptors are built from wave-uniform (readfirstlane)
-            // pointers: held in scalar registers, with no per-lane loop around
-            // the buffer accesses
-            const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf0), 0, nrec, kRsrc3);
-            const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(buf1), 0, nrec, kRsrc3);
-            for (int l = 0; l < nexec; l++) {
-                int me = kexec - l * M;
-                if (me > M) me = M;
-                sweeps(2 * me);
-                const bool last = l + 1 == nexec;
-                const __amdgpu_buffer_rsrc_t ro = ((l + 1) & 1) ? rs1 : rs0;
-                // stage output at device scope: the border strips, or (last
-                // stage of the case) the whole stored region
-                if (lane >= H / 2 && lane < 64 - H / 2) {
-                    const bool lb = lane < H || lane >= 64 - H;
-    #pragma unroll
-                    for (int k = 0; k < NRW; k++) {
-                        const int rr = w * NRW + k;
-                        const int gj = jt0 + k;
-                        if (rr < H || rr >= H + TY || gj > ny - 2) continue;
-                        if (!last && !lb && rr >= 2 * H && rr < TY) continue;
-                        const int srow = gj * P * (int)sizeof(T);  // uniform: scalar offset
-                        if constexpr (sizeof(T) == 4) {
-                            if (ga + 1 <= nx - 2)
-                                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro,
-                                                                      voff, srow, kDev);
-                            else if (ga <= nx - 2)
-                                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, voff, srow, kDev);
-                        } else {
-                            if (ga + 1 <= nx - 2)
-                                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, V2{R.pa[k], R.pb[k]}), ro,
-                                                                       voff, srow, kDev);
-                            else if (ga <= nx - 2)
-                                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, R.pa[k]), ro, voff, srow,
-                                                                      kDev);
-                        }
-                    }
-                }
-                if (last) break;
-                // publish stage l once every wave's stores are complete, wait
-                // for the neighbours' stage l
-                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-                __syncthreads();
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(&flags[t], f0 + (unsigned)(l + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                wait_neighbours(flags, nb, f0 + (unsigned)(l + 1), bar);
-                // after a timed-out wait anywhere in the launch, stop sweeping:
-                // the error buffers keep finite values, the step reports the
-                // failure and the solve resumes from the current pressure
-                const int bad =
-                    threadIdx.x == 0 ? (int)__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-                if (__builtin_amdgcn_readfirstlane(__syncthreads_or(bad))) break;
-                // halo ring: the cells of the tile outside its stored region
-                // that lie in the domain's interior (device-scope loads)
-                const bool lh = lane < H / 2 || lane >= 64 - H / 2;
-    #pragma unroll
-                for (int k = 0; k < NRW; k++) {
-                    const int rr = w * NRW + k;
-                    const int gj = jt0 + k;
-                    if (gj < 1 || gj > ny - 2) continue;
-                    if (!lh && rr >= H && rr < H + TY) continue;
-                    const int srow = gj * P * (int)sizeof(T);
+                    if (!last && !lb && rr >= 2 * H && rr < TY) continue;
+                    const int srow = gj * P * (int)sizeof(T);  // uniform: scalar offset
                     if constexpr (sizeof(T) == 4) {
-                        if (ina && inb) {
-                            const V2 v =
-                                __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
-                            R.pa[k] = v[0];
-                            R.pb[k] = v[1];
-                        } else {
-                            if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff, srow, kDev));
-                            if (inb)
-                                R.pb[k] =
-                                    __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff + 4, srow, kDev));
-                        }
+                        if (ga + 1 <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, V2{R.pa[k], R.pb[k]}), ro,
+                                                                  voff, srow, kDev);
+                        else if (ga <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(R.pa[k]), ro, voff, srow, kDev);
                     } else {
-                        if (ina && inb) {
-                            const V2 v =
-                                __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(ro, voff, srow, kDev));
-                            R.pa[k] = v[0];
-                            R.pb[k] = v[1];
-                        } else {
-                            if (ina)
-                                R.pa[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
-                            if (inb)
-                                R.pb[k] =
-                                    __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff + 8, srow, kDev));
-                        }
+                        if (ga + 1 <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, V2{R.pa[k], R.pb[k]}), ro,
+                                                                   voff, srow, kDev);
+                        else if (ga <= nx - 2)
+                            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, R.pa[k]), ro, voff, srow,
+                                                                  kDev);
                     }
                 }
             }
+            if (last) break;
+            // publish stage l once every wave's stores are complete, wait
+            // for the neighbours' stage l
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
             __syncthreads();
             if (threadIdx.x == 0)
-                __hip_atomic_store(&flags[t], f0 + (unsigned)nstage, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
+                __hip_atomic_store(&flags[t], f0 + (unsigned)(l + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wait_neighbours(flags, nb, f0 + (unsigned)(l + 1), bar);
+            // after a timed-out wait anywhere in the launch, stop sweeping:
+            // the error buffers keep finite values, the step reports the
+            // failure and the solve resumes from the current pressure
+            const int bad =
+                threadIdx.x == 0 ? (int)__hip_atomic_load(&bar->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            if (__builtin_amdgcn_readfirstlane(__syncthreads_or(bad))) break;
+            // halo ring: the cells of the tile outside its stored region
+            // that lie in the domain's interior (device-scope loads)
+            const bool lh = lane < H / 2 || lane >= 64 - H / 2;
+#pragma unroll
+            for (int k = 0; k < NRW; k++) {
+                const int rr = w * NRW + k;
+                const int gj = jt0 + k;
+                if (gj < 1 || gj > ny - 2) continue;
+                if (!lh && rr >= H && rr < H + TY) continue;
+                const int srow = gj * P * (int)sizeof(T);
+                if constexpr (sizeof(T) == 4) {
+                    if (ina && inb) {
+                        const V2 v =
+                            __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
+                        R.pa[k] = v[0];
+                        R.pb[k] = v[1];
+                    } else {
+                        if (ina) R.pa[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff, srow, kDev));
+                        if (inb)
+                            R.pb[k] =
+                                __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ro, voff + 4, srow, kDev));
+                    }
+                } else {
+                    if (ina && inb) {
+                        const V2 v =
+                            __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(ro, voff, srow, kDev));
+                        R.pa[k] = v[0];
+                        R.pb[k] = v[1];
+                    } else {
+                        if (ina)
+                            R.pa[k] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, srow, kDev));
+                        if (inb)
+                            R.pb[k] =
+                                __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(ro, voff + 8, srow, kDev));
+                    }
+                }
+            }
         }
-        sweeps(nst);
-        store(dst, false);
-    };
-    static_assert(!WT || sizeof(T) == 4, "weight tables are FP32");
-    body(std::integral_constant<bool, WT>{});
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&flags[t], f0 + (unsigned)nstage, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    sweeps(nst);
+    store(dst, false);
 }
 
 template <typename T, int M, int NW, int NRW>
