@@ -170,9 +170,11 @@ struct CaseDev {
     double *p[2], *rhs;
     float *e[2], *r32;             // mixed Poisson error / residual
     float *w32;                    // mixed: each cell's FP32 SOR weight om / sa (0: not updated), formed once per step
-    float *wtab;                   // mixed on a grid whose FP32 coefficients are each one value: the weight of
-                                   // each of the 64 patterns of flag bits 0-5 (in w32's first 64 slots, w32
-                                   // itself unused); null: the per-cell weights in w32
+    float *wtab;                   // mixed on a grid whose FP32 coefficients are each one value: the SOR
+                                   // weight of each of the 64 patterns of flag bits 0-5; null: per-cell
+                                   // weights (w32)
+    double *wtab64;                // FP64 Poisson on a grid whose coefficients are each one value: the same
+                                   // table in FP64; null: each launch forms the weights from the flags
     void *lf;                      // line solve factors (Poisson precision), line_row_stride(lB) per row; line cases only
     int lB, lS;                    // line solve partition of a row: S blocks of B cells
     uint8_t *flags;

@@ -152,9 +152,9 @@ static ArenaPlan plan_arena(int prec, int nx, int ny, int P, int ntiles, int nbo
         for (int b = 0; b < 2; b++) a.off_e[b] = fieldT(4);
         a.off_r32 = fieldT(4);
         a.off_w32 = fieldT(4);
-        a.off_wtab = o;
-        o = align256(o + sizeof(float) * 64);
     }
+    a.off_wtab = o;  // 64 weights, FP32 or FP64 (the Poisson precision)
+    o = align256(o + sizeof(double) * 64);
     if (line) {  // the line solve's factors, one stride per row, in the Poisson precision
         int B, S;
         line_partition(nx - 2, &B, &S);
@@ -404,20 +404,22 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
     std::vector<float> FX, FY;
     axis_vectors(s.dx, d.dt, d.re, d.conv_factor, true, VX, FX);
     axis_vectors(s.dy, d.dt, d.re, d.conv_factor, false, VY, FY);
-    // A mixed case whose FP32 Poisson coefficients are each one value over
-    // the faces that can open (east and north: cells 1..n-3; west and
-    // south: cells 2..n-2) has one weight per pattern of face flags: the
-    // sweep reads the cell's flag byte and looks the weight up instead of
-    // loading a per-cell weight.
-    auto flat = [](const std::vector<float> &F, int n) {
-        const float *cp = &F[(size_t)7 * n], *cm = &F[(size_t)8 * n];
+    // A case whose Poisson coefficients (in the Poisson precision) are each
+    // one value over the faces that can open (east and north: cells
+    // 1..n-3; west and south: cells 2..n-2) has one SOR weight per pattern
+    // of face flags: the register-tile sweep looks the weight up by the
+    // cell's flag byte instead of loading a per-cell weight (mixed) or
+    // dividing on every launch (FP64).
+    auto flat = [](const auto &V, int n, int icp) {
+        const auto *cp = &V[(size_t)icp * n], *cm = &V[(size_t)(icp + 1) * n];
         for (int i = 2; i <= n - 3; i++)
             if (cp[i] != cp[1]) return false;
         for (int i = 3; i <= n - 2; i++)
             if (cm[i] != cm[2]) return false;
         return true;
     };
-    c.wtab = (pprec_ == IF_MIXED && flat(FX, s.nx) && flat(FY, s.ny)) ? (float *)(A + a.off_wtab) : nullptr;
+    c.wtab = (pprec_ == IF_MIXED && flat(FX, s.nx, 7) && flat(FY, s.ny, 7)) ? (float *)(A + a.off_wtab) : nullptr;
+    c.wtab64 = (pprec_ == IF_FP64 && flat(VX, s.nx, 6) && flat(VY, s.ny, 6)) ? (double *)(A + a.off_wtab) : nullptr;
     double *md = (double *)(A + a.off_metd);
     float *mf = (float *)(A + a.off_metf);
     IFF_HIP(hipMemcpyAsync(md, VX.data(), VX.size() * 8, hipMemcpyHostToDevice, st_));
@@ -732,10 +734,11 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
-    wt_all_ = pprec_ == IF_MIXED;
+    wt_all_ = 1;
     for (int id : order_) {
         const Slot &sl = *slots_[id];
-        if (sl.desc.ppe_solver == IF_PPE_RBSOR && !sl.dev.wtab) wt_all_ = 0;
+        if (sl.desc.ppe_solver == IF_PPE_RBSOR && !(pprec_ == IF_FP64 ? (void *)sl.dev.wtab64 : (void *)sl.dev.wtab))
+            wt_all_ = 0;
     }
     // FP64 default shape, by measurement: 16-wave tiles as persistent
     // chunks of the smallest height (48, 64, 80 rows) whose tiles fit in
@@ -930,7 +933,10 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
         // ---- P3: faces + RHS
         launch_faces(st_, vprec_, d_cases_, d_tiles_base_, nt_base_, 1, 1);
         if (any_line) launch_line_factor(st_, pprec_, d_cases_, n, max_ring_);
-        if (pprec_ == IF_MIXED) launch_weights32(st_, d_cases_, n, max_ring_);
+        if (pprec_ == IF_MIXED)
+            launch_weights32(st_, d_cases_, n, max_ring_);
+        else
+            launch_weight_table64(st_, d_cases_, n);
 
         // ---- P4/P5: Poisson in chunks of K sweeps (unit 0 = the entry check)
         const int nunits = 1 + (max_ppe_ + K - 1) / K;

@@ -21,7 +21,7 @@ void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const Case
 // register-tile red-black SOR (sweep_reg.hip)
 bool reg_variant_ok(int prec, int waves, int ty, int m);
 int reg_out_cols(int m);
-// wt (FP32): every case of the tile table has a weight table (CaseDev::wtab)
+// wt: every red-black case of the tile table has a weight table (CaseDev::wtab, FP64: wtab64)
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                           const int2 *tiles, int nt, int s, int K, int wt);
 // The K sweeps of a chunk in one launch of the register-tile kernel, in
@@ -42,6 +42,7 @@ void launch_ppe_rearm(hipStream_t st, const CaseDev *cs, int ncase);
 // case, then one sweep (both colours) at chunk position s
 void launch_line_factor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_ny);
 void launch_weights32(hipStream_t st, const CaseDev *cs, int ncase, int max_n);
+void launch_weight_table64(hipStream_t st, const CaseDev *cs, int ncase);
 void launch_line_sor(hipStream_t st, int prec, const CaseDev *cs, int ncase, int max_n, int s, int K);
 void launch_project(hipStream_t st, int prec, const CaseDev *cs, const int2 *tiles, int nt);
 void launch_step_end(hipStream_t st, const CaseDev *cs, int ncase);

@@ -1261,6 +1261,33 @@ __global__ __launch_bounds__(256) void k_weights32(const CaseDev *cs)
     }
 }
 
+// The FP64 weight table of a case with one value per Poisson coefficient:
+// the weight of each pattern of flag bits 0-5, with the expression and
+// operand order of the FP64 sweep's per-launch weights (om / sa over the
+// open faces, IEEE division; 0 for a cell the sweep keeps), so a looked-up
+// weight has the same bits.  One block of 64 per case.
+__global__ __launch_bounds__(64) void k_weight_table64(const CaseDev *cs)
+{
+    const CaseDev &c = cs[blockIdx.x];
+    if (!c.wtab64 || !c.ctrl->active) return;
+    const unsigned f = threadIdx.x;
+    double w = 0.0;
+    if (f & F_FLUID) {
+        const double aE = (f & F_OPEN_E) ? G(c.x.cp)[1] : 0.0;
+        const double aW = (f & F_OPEN_W) ? G(c.x.cm)[2] : 0.0;
+        const double aN = (f & F_OPEN_N) ? G(c.y.cp)[1] : 0.0;
+        const double aS = (f & F_OPEN_S) ? G(c.y.cm)[2] : 0.0;
+        const double sa = (aE + aW) + (aN + aS);
+        if (sa != 0.0) w = sor_weight64(sa, c.omega);
+    }
+    GW(c.wtab64)[f] = w;
+}
+
+void launch_weight_table64(hipStream_t st, const CaseDev *cs, int ncase)
+{
+    hipLaunchKernelGGL(k_weight_table64, dim3((unsigned)ncase), dim3(64), 0, st, cs);
+}
+
 void launch_weights32(hipStream_t st, const CaseDev *cs, int ncase, int max_n)
 {
     hipLaunchKernelGGL(k_weights32, dim3((unsigned)(max_n - 2), (unsigned)ncase), dim3(256), 0, st, cs);

@@ -316,11 +316,10 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     const int xb = (ga + 1 != nx - 2 ? F_OPEN_E : 0) | (ga + 1 != 1 ? F_OPEN_W : 0);
     constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
 
-    // WT (FP32): the weights are looked up by flag pattern (wtab, which
-    // every case of the launch has) instead of loaded per cell (w32).  Two
-    // kernels rather than a runtime branch, which would hold both forms'
-    // registers at once.
-    static_assert(!WT || sizeof(T) == 4, "weight tables are FP32");
+    // WT: the weights are looked up by flag pattern (wtab / wtab64, which
+    // every case of the launch has) instead of loaded per cell (FP32: w32)
+    // or formed with a division on every launch (FP64).  Two kernels rather
+    // than a runtime branch, which would hold both forms' registers at once.
     RegTile<T, NRW> R;
     int lf[NRW];  // FP64: flags, slot a in bits 0-7, slot b in bits 8-15
     // FP32: the cells' weights om / sa, formed once per step (k_weights32),
@@ -329,8 +328,16 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // its entry across the wave (ds_bpermute)
     const float *w32 = c.w32;
     constexpr bool wuni = WT;
-    int wtv = 0;
-    if constexpr (wuni) wtv = __float_as_int(G(c.wtab)[lane]);
+    int wtv = 0, wtv_hi = 0;
+    if constexpr (wuni) {
+        if constexpr (sizeof(T) == 4) {
+            wtv = __float_as_int(G(c.wtab)[lane]);
+        } else {
+            const double d = G(c.wtab64)[lane];
+            wtv = __double2loint(d);
+            wtv_hi = __double2hiint(d);
+        }
+    }
     // Loads.  A tile lying inside the interior (the common case) loads
     // every row with aligned 8-byte vectors and no per-lane tests.
     const bool tile_in = c0 >= 1 && c0 + 127 <= nx - 2 && jt0 >= 1 && jt0 + NRW - 1 <= ny - 2;
@@ -458,14 +465,26 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
             if (offfast) smask |= 1u << k;
             // the weight om / sa over the open faces (for a fast-path cell the
             // same bits as om / (cx + cy)); 0 for a cell the sweep keeps:
-            // outside the interior, not fluid, or without an open face
+            // outside the interior, not fluid, or without an open face.  With
+            // a table: the entry of the cell's flag pattern, formed with the
+            // same expression (a cell outside the interior has flags 0)
             const int fa = f & 0xff, fb = f >> 8;
-            const T sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
-                           (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
-            const T sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
-                           (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
-            R.wa[k] = (rowint && (fa & F_FLUID) && sa_a != (T)0) ? sor_weight64(sa_a, om) : (T)0;
-            R.wb[k] = (rowint && (fb & F_FLUID) && sa_b != (T)0) ? sor_weight64(sa_b, om) : (T)0;
+            if constexpr (wuni) {
+                auto look = [&](int q) {
+                    const int i = (q & 63) << 2;
+                    return __hiloint2double(__builtin_amdgcn_ds_bpermute(i, wtv_hi),
+                                            __builtin_amdgcn_ds_bpermute(i, wtv));
+                };
+                R.wa[k] = look(fa);
+                R.wb[k] = look(fb);
+            } else {
+                const T sa_a = (face_coef<T, F_OPEN_E>(fa, cxpa) + face_coef<T, F_OPEN_W>(fa, cxma)) +
+                               (face_coef<T, F_OPEN_N>(fa, yp) + face_coef<T, F_OPEN_S>(fa, ym));
+                const T sa_b = (face_coef<T, F_OPEN_E>(fb, cxpb) + face_coef<T, F_OPEN_W>(fb, cxmb)) +
+                               (face_coef<T, F_OPEN_N>(fb, yp) + face_coef<T, F_OPEN_S>(fb, ym));
+                R.wa[k] = (rowint && (fa & F_FLUID) && sa_a != (T)0) ? sor_weight64(sa_a, om) : (T)0;
+                R.wb[k] = (rowint && (fb & F_FLUID) && sa_b != (T)0) ? sor_weight64(sa_b, om) : (T)0;
+            }
         }
     }
     const T c1 = (T)1 - om;
@@ -706,19 +725,18 @@ template <typename T, int M, int NW, int NRW>
 static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K, int wt)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    if constexpr (sizeof(T) == 4)
-        if (wt) {
-            hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles,
-                               nt, pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr);
-            return;
-        }
+    if (wt) {
+        hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
+                           pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr);
+        return;
+    }
     hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, false>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
                        pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr);
 }
 
 // Blocks of the persistent form that can be resident at once: resident
-// blocks per CU (occupancy calculator; FP32: the smaller of the two weight
-// forms) times the CUs, per shape.
+// blocks per CU (occupancy calculator; the smaller of the two weight forms)
+// times the CUs, per shape.
 template <typename T, int M, int NW, int NRW>
 static int pers_capacity()
 {
@@ -730,12 +748,10 @@ static int pers_capacity()
             hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
-        if constexpr (sizeof(T) == 4) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_rbsor_reg<T, M, NW, NRW, true, true>,
-                                                             64 * NW, 0) != hipSuccess)
-                return 0;
-            per_cu = std::min(per_cu, per_cu_t);
-        }
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_rbsor_reg<T, M, NW, NRW, true, true>, 64 * NW,
+                                                         0) != hipSuccess)
+            return 0;
+        per_cu = std::min(per_cu, per_cu_t);
         capacity = per_cu * cus;
     }
     return capacity;
@@ -746,12 +762,11 @@ static void pers_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int n
                      unsigned *flags, int wt)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    if constexpr (sizeof(T) == 4)
-        if (wt) {
-            hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles,
-                               nt, pad, 0, K, bar, flags);
-            return;
-        }
+    if (wt) {
+        hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
+                           pad, 0, K, bar, flags);
+        return;
+    }
     hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, false>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
                        pad, 0, K, bar, flags);
 }

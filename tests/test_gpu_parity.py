@@ -520,6 +520,46 @@ def test_heaving_cylinder_matches_oracle(require_gpu):
     gc.close()
 
 
+@pytest.mark.parametrize("kind,ty,m", [(3, 32, 4), (2, 32, 2), (4, 48, 4), (4, 80, 4), (3, 64, 8)])
+def test_fp64_weight_table_matches_oracle(require_gpu, kind, ty, m):
+    """FP64 on a grid with one value per Poisson coefficient (spacing 1/32):
+    the register-tile kernel looks each cell's weight up in a table formed
+    with the oracle's expression instead of dividing on every launch, per
+    stage and as persistent chunks; sweep counts equal to the oracle's and
+    fields to 1e-10, with a star (cells of every face pattern) and with a
+    heaving body (patterns change every step).  A stretched grid takes the
+    division."""
+    import math
+    xf, yf, bodies = CASES["stars"]
+    _say("fp64 weight table", kind, ty, m)
+    oc, gc = _pair(xf, yf, bodies, dt=0.01, check_every=12)
+    gc.tune(kind, ty, m, 12)
+    for _ in range(2):
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["weight_table"] == 1 and g["ppe_sweeps"] == o[2], (g, o)
+    _compare_fields(oc, gc, 1e-10)
+    gc.close()
+    y0, A, f, dt = 2.0, 0.5, 0.2, 0.02
+    prm = dict(dt=dt, re=100.0, conv_factor=0.5, ubc=1.0, vbc=0.0, ad_itermax=25, ad_tol=1e-6,
+               ppe_solver=1, ppe_omega=1.8, ppe_itermax=100000, ppe_tol=1e-8, check_every=12)
+    oc = O.PhysCase(xf, xf, [dict(kind=O.CIRCLE, xc=1.5, yc=y0, r=0.4, vb=A * 2 * math.pi * f)], prm)
+    gc = iff.Case(xf, xf, bodies=[iff.heaving_circle(1.5, y0, 0.4, A, f)], **prm)
+    gc.tune(kind, ty, m, 12)
+    for n in range(3):
+        oc.heave({0: (y0, A, f)}, (n + 1) * dt)
+        o = oc.step(1)[-1]
+        g = gc.step(1)
+        assert g["weight_table"] == 1 and g["ppe_sweeps"] == o[2], (n, g, o)
+        _compare_fields(oc, gc, 1e-10)
+    gc.close()
+    xs, ys, bs = CASES["circle-stretched"]
+    gs = iff.Case(xs, ys, bodies=bs, dt=0.01, re=100.0, ppe_omega=1.8, ppe_tol=1e-9, check_every=12)
+    gs.tune(kind, ty, m, 12)
+    assert gs.step(1)["weight_table"] == 0
+    gs.close()
+
+
 def test_mixed_weights_follow_moving_flags(require_gpu):
     """Mixed precision with a heaving body: the register-tile kernel loads
     the FP32 SOR weights formed once per step from the flags, the LDS-tiled
