@@ -668,7 +668,8 @@ def serving_c5(iff, configs, a):
     def pick(s):
         return dict(cases_per_s=round(s["cases_per_s"], 4), p50_s=round(s["p50_s"], 3), p99_s=round(s["p99_s"], 3),
                     mcell_steps_per_s=round(s["mcell_steps_per_s"], 2), max_div=s["max_div"],
-                    max_resident=s["max_resident"], wall_s=round(s["wall_s"], 2), completed=s["completed"])
+                    max_resident=s["max_resident"], wall_s=round(s["wall_s"], 2), completed=s["completed"],
+                    autotunes=s["autotunes"], last_autotune=s["autotune"])
     sizes = [len(c["xf"]) - 1 for _, c in trace]
     return dict(value=round(summ["cases_per_s"], 4), unit="cases/s", requests=len(trace),
                 at_50_req_per_s=pick(summ), sub_saturating=dict(pick(summ2), arrival_scale=round(scale, 2),

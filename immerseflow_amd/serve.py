@@ -64,7 +64,11 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
     """Run a request trace [(arrival_s, case_dict), ...] to completion.
 
     tune: (kernel, tile_y, sweeps_per_launch, check_every), or "auto" to run
-    the autotuner once on the resident batch after its first batched step.
+    the autotuner on the resident batch after its first batched step, and
+    again after any batched step whose largest resident grid has more than
+    twice the cells of the largest the last tuning saw (the large cases
+    take most of the time, and the shape that suits a few small grids
+    does not suit them).
     Returns (records, summary).  records: one dict per request with
     arrival / admitted / finished times (s, trace clock), latency, state,
     steps, cells and the final max |div|; with keep_fields (True, or a
@@ -76,6 +80,8 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
     else:   # until the autotuner has run: the library's measured default shape
         batch.tune(*((3, 32, 4, 160) if precision == IF_FP64 else (4, 64, 8, 160)))
     tuned = None
+    tuned_cells = 0   # the largest resident grid at the last tuning
+    ntune = 0
     order = sorted(range(len(trace)), key=lambda k: trace[k][0])
     pending = collections.deque()
     active = {}
@@ -106,10 +112,15 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
             continue
         batch.step(1)
         batched_steps += 1
-        if auto and tuned is None and any(batch.state(sl)[0] == 0 for sl in active):
-            tuned = batch.autotune()
-            if log:
-                log("autotuned: %s" % tuned)
+        if auto:
+            running = [sl for sl in active if batch.state(sl)[0] == 0]
+            big = max(((len(active[sl][3]["xf"]) - 1) * (len(active[sl][3]["yf"]) - 1) for sl in running), default=0)
+            if big > 2 * tuned_cells:
+                tuned = batch.autotune()
+                tuned_cells = big
+                ntune += 1
+                if log:
+                    log("autotuned (largest grid %d cells): %s" % (big, tuned))
         now = time.perf_counter() - t0
         for slot in list(active):
             state, steps_done, st = batch.state(slot)
@@ -136,5 +147,5 @@ def serve(trace, precision=IF_MIXED, hbm_budget=64 << 30, tune="auto",
                    p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
                    mcell_steps_per_s=sum(r["cells"] * r["steps"] for r in ok) / wall / 1e6,
                    max_div=max((r["max_div"] for r in ok), default=None),
-                   batched_steps=batched_steps, max_resident=max_resident, autotune=tuned)
+                   batched_steps=batched_steps, max_resident=max_resident, autotune=tuned, autotunes=ntune)
     return records, summary
