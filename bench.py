@@ -96,14 +96,15 @@ def log(*a):
 
 def kernel_name(kind, ty, m, t, pers=False, wt=False):
     """The Poisson sweep kernel a tune() choice launches (as rocprofv3 names
-    it); wt: the FP32 weights are looked up by flag pattern."""
+    it); wt: the weights are looked up by flag pattern (weight mode 1, else
+    0: no case of the launch takes the uniform-grid form)."""
     if kind == 0:
         return "k_rbsor<%s, %d, %d>" % (t, ty, m)
     if kind == 1:
         return "k_rbsor_stream<%s, %d, %d>" % (t, m, ty)
     nw = {2: 4, 3: 8, 4: 16}[kind]
     b = lambda x: "true" if x else "false"
-    return "k_rbsor_reg<%s, %d, %d, %d, %s, %s>" % (t, m, nw, (ty + 4 * m) // nw, b(pers), b(wt))
+    return "k_rbsor_reg<%s, %d, %d, %d, %s, %d>" % (t, m, nw, (ty + 4 * m) // nw, b(pers), 1 if wt else 0)
 
 
 def reg_updates_per_sweep(NX, NY, kind, ty, m, pers=False):

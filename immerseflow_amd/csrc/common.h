@@ -175,6 +175,12 @@ struct CaseDev {
                                    // weights (w32)
     double *wtab64;                // FP64 Poisson on a grid whose coefficients are each one value: the same
                                    // table in FP64; null: each launch forms the weights from the flags
+    int uni;                       // mixed red-black case with one FP32 coefficient value per axis on every
+                                   // face that can open (aE = aW = uax, aN = aS = uax * uratio): the FP32
+                                   // sweeps take the five-operation form (sor_cell_u) with K = w uax, and
+                                   // r32 holds the residual times uinv = 1 / uax; only such cases get wtab
+                                   // (then the K of each flag pattern)
+    float uax, uratio, uinv;
     void *lf;                      // line solve factors (Poisson precision), line_row_stride(lB) per row; line cases only
     int lB, lS;                    // line solve partition of a row: S blocks of B cells
     uint8_t *flags;

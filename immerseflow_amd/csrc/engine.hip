@@ -418,7 +418,17 @@ if_status Engine::build_slot(Slot &s, const if_case_desc &d)
             if (cm[i] != cm[2]) return false;
         return true;
     };
-    c.wtab = (pprec_ == IF_MIXED && flat(FX, s.nx, 7) && flat(FY, s.ny, 7)) ? (float *)(A + a.off_wtab) : nullptr;
+    // mixed: the five-operation FP32 form needs one value per axis on both
+    // sides of a cell (aE = aW, aN = aS), which the table form then assumes
+    c.uni = pprec_ == IF_MIXED && d.ppe_solver == IF_PPE_RBSOR && flat(FX, s.nx, 7) && flat(FY, s.ny, 7) &&
+            s.nx >= 4 && s.ny >= 4 && FX[(size_t)7 * s.nx + 1] == FX[(size_t)8 * s.nx + 2] &&
+            FY[(size_t)7 * s.ny + 1] == FY[(size_t)8 * s.ny + 2];
+    if (c.uni) {
+        c.uax = FX[(size_t)7 * s.nx + 1];
+        c.uratio = FY[(size_t)7 * s.ny + 1] / c.uax;
+        c.uinv = 1.0f / c.uax;
+    }
+    c.wtab = c.uni ? (float *)(A + a.off_wtab) : nullptr;
     c.wtab64 = (pprec_ == IF_FP64 && flat(VX, s.nx, 6) && flat(VY, s.ny, 6)) ? (double *)(A + a.off_wtab) : nullptr;
     double *md = (double *)(A + a.off_metd);
     float *mf = (float *)(A + a.off_metf);
@@ -734,11 +744,16 @@ if_status Engine::refresh()
     for (auto &s : slots_)
         if (s && s->state == 0) order_.push_back(s->id);
     const int n = (int)order_.size();
-    wt_all_ = 1;
-    for (int id : order_) {
-        const Slot &sl = *slots_[id];
-        if (sl.desc.ppe_solver == IF_PPE_RBSOR && !(pprec_ == IF_FP64 ? (void *)sl.dev.wtab64 : (void *)sl.dev.wtab))
-            wt_all_ = 0;
+    {  // the weight mode of the sweep launches (launch_ppe_sweep_reg)
+        int rb = 0, tab = 0, uni = 0;
+        for (int id : order_) {
+            const Slot &sl = *slots_[id];
+            if (sl.desc.ppe_solver != IF_PPE_RBSOR) continue;
+            rb++;
+            tab += (pprec_ == IF_FP64 ? (void *)sl.dev.wtab64 : (void *)sl.dev.wtab) != nullptr;
+            uni += sl.dev.uni;
+        }
+        wt_all_ = tab == rb ? 1 : uni > 0 ? 2 : 0;
     }
     // FP64 default shape, by measurement: 16-wave tiles as persistent
     // chunks of the smallest height (48, 64, 80 rows) whose tiles fit in
@@ -1045,9 +1060,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             s.last.total_ppe_sweeps = c.tot_ppe;
             s.last.max_div = c.max_div;
             s.last.failed = c.failed;
-            s.last.weight_table = wt_all_;
+            s.last.weight_table = wt_all_ == 1;
             if (agg) {
-                agg->weight_table = wt_all_;
+                agg->weight_table = wt_all_ == 1;
                 agg->steps = std::max(agg->steps, 1 + it);
                 agg->ad_iters = c.ad_iter;
                 agg->ad_res = c.ad_res;

@@ -21,7 +21,9 @@ void launch_ppe_sweep_stream(hipStream_t st, int prec, int ty, int m, const Case
 // register-tile red-black SOR (sweep_reg.hip)
 bool reg_variant_ok(int prec, int waves, int ty, int m);
 int reg_out_cols(int m);
-// wt: every red-black case of the tile table has a weight table (CaseDev::wtab, FP64: wtab64)
+// wt, the weight mode: 1 every red-black case of the tile table has a weight
+// table (CaseDev::wtab, FP64: wtab64); 0 none uses the five-operation FP32
+// form (CaseDev::uni); 2 (FP32) some do, some do not
 void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, const CaseDev *cs,
                           const int2 *tiles, int nt, int s, int K, int wt);
 // The K sweeps of a chunk in one launch of the register-tile kernel, in

@@ -382,7 +382,7 @@ template <typename T, int TY, int M, int h>
 __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
                                               const uint8_t (*sf)[kTileX + 4 * M],
                                               const T *mxp, const T *mxm, const T *myp, const T *mym,
-                                              int gpar, int cpar, T om)
+                                              int gpar, int cpar, T om, int uni, T ax, T ratio)
 {
     constexpr int H = 2 * M;
     constexpr int R = H - 1 - h;
@@ -407,6 +407,13 @@ __device__ __forceinline__ void rb_half_sweep(T (*sp)[kTileX + 4 * M], const T (
         const T aS = (f & F_OPEN_S) ? mym[ly] : (T)0;
         const T sa = (aE + aW) + (aN + aS);
         if (sa == (T)0) continue;
+        if constexpr (sizeof(T) == 4) {
+            if (uni) {
+                sp[ly][lx] = sor_cell_u_sa(sp[ly][lx], sr[ly][lx], sp[ly][lx + 1], sp[ly][lx - 1], sp[ly + 1][lx],
+                                           sp[ly - 1][lx], sa, om, ax, ratio);
+                continue;
+            }
+        }
         sp[ly][lx] = sor_cell<T>(sp[ly][lx], sr[ly][lx], aE, sp[ly][lx + 1], aW, sp[ly][lx - 1], aN,
                                  sp[ly + 1][lx], aS, sp[ly - 1][lx], sa, om, cpar + lx);
     }
@@ -416,14 +423,14 @@ template <typename T, int TY, int M, int h>
 __device__ __forceinline__ void rb_half_sweeps(T (*sp)[kTileX + 4 * M], const T (*sr)[kTileX + 4 * M],
                                                const uint8_t (*sf)[kTileX + 4 * M], const T *mxp,
                                                const T *mxm, const T *myp, const T *mym, int gpar, int cpar,
-                                               T om, int nst)
+                                               T om, int nst, int uni, T ax, T ratio)
 {
     if constexpr (h < 2 * M) {
         if (h < nst) {
-            rb_half_sweep<T, TY, M, h>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om);
+            rb_half_sweep<T, TY, M, h>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om, uni, ax, ratio);
             __syncthreads();
         }
-        rb_half_sweeps<T, TY, M, h + 1>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om, nst);
+        rb_half_sweeps<T, TY, M, h + 1>(sp, sr, sf, mxp, mxm, myp, mym, gpar, cpar, om, nst, uni, ax, ratio);
     }
 }
 
@@ -542,7 +549,13 @@ __global__ __launch_bounds__(256) void k_rbsor(const CaseDev *cs, const int2 *ti
     // parity of the tile's first cell, and of its first column (LDS column
     // lx is global column i0 - H + lx)
     const int gpar = (i0 - H) + (j0 - H);
-    rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, (i0 - H) & 1, om, 2 * meff);
+    T uax = (T)0, uratio = (T)0;
+    if constexpr (sizeof(T) == 4) {
+        uax = c.uax;
+        uratio = c.uratio;
+    }
+    rb_half_sweeps<T, TY, M, 0>(sp, sr, sf, mxp, mxm, myp, mym, gpar & 1, (i0 - H) & 1, om, 2 * meff,
+                                sizeof(T) == 4 && c.uni, uax, uratio);
 
     // write the tile (all interior cells: non-fluid values pass through)
     constexpr int VW = 16 / sizeof(T);  // 16-byte stores; i0 is 256-byte aligned
@@ -1154,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_ppe_update_resid(const CaseDev *cs, con
                 if (ap != 0.0) {
                     const double sum = (aE * pE + aW * pW) + (aN * pN + aS * pS);
                     const double rr = c.rhs[id] - (ap * pc + sum);
-                    r32 = (float)rr;
+                    r32 = c.uni ? (float)rr * c.uinv : (float)rr;
                     m = fmax(m, absinf(rr));
                 }
             }
@@ -1218,10 +1231,10 @@ __global__ void k_ppe_rearm(const CaseDev *cs, int ncase)
 // for a cell the sweep keeps (not fluid, or no open face).  Once per step,
 // after the flags are final: the register-tile kernel loads it instead of
 // forming it from the flags on every launch.  One block per row.  On a grid
-// whose coefficients are each one value (wtab) the weight depends on the
-// flags alone: the first block also writes the weight of each pattern of
-// flag bits 0-5, with the same expression, for launches whose cases all
-// have such a table.
+// whose coefficients are one value per axis (wtab) the weight depends on
+// the flags alone: the first block also writes K = w ax of each pattern of
+// flag bits 0-5 (the five-operation form's weight), with the same
+// expression, for launches whose cases all have such a table.
 __global__ __launch_bounds__(256) void k_weights32(const CaseDev *cs)
 {
     const CaseDev &c = cs[blockIdx.y];
@@ -1239,7 +1252,7 @@ __global__ __launch_bounds__(256) void k_weights32(const CaseDev *cs)
                 const float sa = (aE + aW) + (aN + aS);
                 if (sa != 0.0f) w = sor_weight(sa, om);
             }
-            GW(c.wtab)[f] = w;
+            GW(c.wtab)[f] = w * c.uax;  // K = w ax of the five-operation form
         }
     }
     const int j = 1 + (int)blockIdx.x;

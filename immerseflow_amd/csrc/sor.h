@@ -79,6 +79,31 @@ __device__ __forceinline__ void gstore_vec(void *p, VT v)
 // FP32 relaxation weight om / sa of a cell (sa != 0).
 __device__ __forceinline__ float sor_weight(float sa, float om) { return om * __builtin_amdgcn_rcpf(sa); }
 
+// FP32 update of a case whose face coefficients are one value per axis
+// (aE = aW = ax, aN = aS = ay on every face that can open; CaseDev::uni):
+// with K = w ax, wr = K r' (r' = r / ax, the form in which the residual
+// is stored for such a case) and ratio = ay / ax (one value per case),
+//   s = ratio pN + (ratio pS + (pW + pE));  p' = K s + ((1 - om) pc - wr)
+// five operations where the general form takes six (one accumulation
+// chain, so no more registers live per row than the general form).  A closed face's
+// neighbour is a non-fluid or ghost cell, whose error is zero (each solve
+// starts from zero and never updates it), so the plain sums equal the
+// flagged ones.  Every FP32 sweep kernel evaluates exactly this for such a
+// case, and the general form for every other.
+__device__ __forceinline__ float sor_cell_u(float pc, float wr, float pW, float pE, float pN, float pS, float K,
+                                            float ratio, float c1)
+{
+    const float s = __builtin_fmaf(ratio, pN, __builtin_fmaf(ratio, pS, pW + pE));
+    return __builtin_fmaf(K, s, __builtin_fmaf(c1, pc, -wr));
+}
+// The same from the cell's sa and r' (kernels that form the weight per update).
+__device__ __forceinline__ float sor_cell_u_sa(float pc, float rs, float pE, float pW, float pN, float pS, float sa,
+                                               float om, float ax, float ratio)
+{
+    const float K = sor_weight(sa, om) * ax;
+    return sor_cell_u(pc, K * rs, pW, pE, pN, pS, K, ratio, 1.0f - om);
+}
+
 // FP64 relaxation weight and update (IEEE division; c1 = 1 - om).
 __device__ __forceinline__ double sor_weight64(double sa, double om) { return om / sa; }
 __device__ __forceinline__ double sor_cell_w64(double pc, double r, double sum, double w, double c1)

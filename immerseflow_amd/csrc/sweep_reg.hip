@@ -120,17 +120,20 @@ __device__ __forceinline__ double reg_cell(int f, double pc, double r, double pE
     }
 }
 
-// One row's cells of a half-sweep.  c1 = 1 - om; the cells' weights are in R.
-template <typename T, int NRW, int Q, bool FAST, int k>
+// One row's cells of a half-sweep.  c1 = 1 - om; the cells' weights are in R
+// (UNI, FP32: K = w ax, the five-operation form with ratio = ay / ax).
+template <typename T, int NRW, int Q, bool FAST, int k, bool UNI = false>
 __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb, T nva, T nvb, T cxpa,
-                                        T cxma, T cxpb, T cxmb, T c1)
+                                        T cxma, T cxpb, T cxmb, T c1, T ratio = (T)0)
 {
     constexpr int ks = k > 0 ? k - 1 : 0, kn = k < NRW - 1 ? k + 1 : 0;
     if constexpr ((k & 1) == Q) {
         const T pS = k == 0 ? sva : R.pa[ks];
         const T pN = k == NRW - 1 ? nva : R.pa[kn];
         const T pE = R.pb[k], pW = from_west(R.pb[k]);
-        if constexpr (sizeof(T) == 4)  // slot a: odd column, west neighbour in lane l-1
+        if constexpr (sizeof(T) == 4 && UNI)
+            R.pa[k] = sor_cell_u(R.pa[k], R.ra[k], pW, pE, pN, pS, R.wa[k], ratio, c1);
+        else if constexpr (sizeof(T) == 4)  // slot a: odd column, west neighbour in lane l-1
             R.pa[k] = reg_cell_w(R.pa[k], R.ra[k], cxma, pW, cxpa, pE, pN, pS, R.cyp[k], R.cym[k], R.wa[k], c1);
         else
             R.pa[k] = reg_cell<FAST>(f & 0xff, R.pa[k], R.ra[k], pE, pW, pN, pS, cxpa, cxma, R.cyp[k], R.cym[k],
@@ -139,7 +142,9 @@ __device__ __forceinline__ void reg_row(RegTile<T, NRW> &R, int f, T sva, T svb,
         const T pS = k == 0 ? svb : R.pb[ks];
         const T pN = k == NRW - 1 ? nvb : R.pb[kn];
         const T pE = from_east(R.pa[k]), pW = R.pa[k];
-        if constexpr (sizeof(T) == 4)  // slot b: even column, east neighbour in lane l+1
+        if constexpr (sizeof(T) == 4 && UNI)
+            R.pb[k] = sor_cell_u(R.pb[k], R.rb[k], pW, pE, pN, pS, R.wb[k], ratio, c1);
+        else if constexpr (sizeof(T) == 4)  // slot b: even column, east neighbour in lane l+1
             R.pb[k] = reg_cell_w(R.pb[k], R.rb[k], cxpb, pE, cxmb, pW, pN, pS, R.cyp[k], R.cym[k], R.wb[k], c1);
         else
             R.pb[k] = reg_cell<FAST>(f >> 8, R.pb[k], R.rb[k], pE, pW, pN, pS, cxpb, cxmb, R.cyp[k], R.cym[k],
@@ -223,7 +228,7 @@ __device__ __forceinline__ void wait_neighbours(const unsigned *flags, int nb, u
 // the checks read the same buffer either way.  A block is at most one
 // stage ahead of its neighbours, so the two buffers never hold data a
 // neighbour still needs when they are overwritten.
-template <typename T, int M, int NW, int NRW, bool PERS, bool WT>
+template <typename T, int M, int NW, int NRW, bool PERS, int WM>
 __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
                                                        int ntiles_pad, int s, int K, GridBar *bar,
                                                        unsigned *flags)
@@ -316,10 +321,15 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     const int xb = (ga + 1 != nx - 2 ? F_OPEN_E : 0) | (ga + 1 != 1 ? F_OPEN_W : 0);
     constexpr int kFaces = F_FLUID | F_OPEN_E | F_OPEN_W | F_OPEN_N | F_OPEN_S;
 
-    // WT: the weights are looked up by flag pattern (wtab / wtab64, which
-    // every case of the launch has) instead of loaded per cell (FP32: w32)
-    // or formed with a division on every launch (FP64).  Two kernels rather
-    // than a runtime branch, which would hold both forms' registers at once.
+    // WM, the weight mode of the launch (kernels rather than runtime
+    // branches, which would hold both forms' registers at once):
+    //   1: every case has a weight table (wtab / wtab64): the weights are
+    //      looked up by flag pattern instead of loaded per cell (FP32: w32)
+    //      or formed with a division on every launch (FP64); FP32 takes the
+    //      five-operation form (every such case is uni);
+    //   0: per-cell weights, the general form;
+    //   2 (FP32): per-cell weights, the form each case's uni flag picks (a
+    //      batch mixing uniform and stretched grids).
     RegTile<T, NRW> R;
     int lf[NRW];  // FP64: flags, slot a in bits 0-7, slot b in bits 8-15
     // FP32: the cells' weights om / sa, formed once per step (k_weights32),
@@ -327,8 +337,13 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // flag pattern: lane q holds the table's entry q, and each cell reads
     // its entry across the wave (ds_bpermute)
     const float *w32 = c.w32;
-    constexpr bool wuni = WT;
-    int wtv = 0, wtv_hi = 0;
+    static_assert(WM == 0 || WM == 1 || (WM == 2 && sizeof(T) == 4), "weight mode");
+    constexpr bool wuni = WM == 1;
+    // FP32 with one coefficient value per axis: the five-operation update,
+    // K = w ax held in place of w
+    const bool uni = sizeof(T) == 4 && (WM == 1 || (WM == 2 && c.uni));
+    const T uax = sizeof(T) == 4 ? (T)c.uax : (T)0, uratio = sizeof(T) == 4 ? (T)c.uratio : (T)0;
+    int wtv = 0, wtv_hi = 0;  // FP32: K = w ax; FP64: w (low and high halves)
     if constexpr (wuni) {
         if constexpr (sizeof(T) == 4) {
             wtv = __float_as_int(G(c.wtab)[lane]);
@@ -445,9 +460,14 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
         R.cyp[k] = uniform(yp);
         R.cym[k] = uniform(ym);
         if constexpr (sizeof(T) == 4) {
+            // the five-operation form holds K = w ax in place of w; r32
+            // holds r' = r / ax for such a case, so K r' stands for w r
             if constexpr (wuni) {  // a cell outside the interior has flags 0: weight 0
                 R.wa[k] = __int_as_float(__builtin_amdgcn_ds_bpermute((lf[k] & 63) << 2, wtv));
                 R.wb[k] = __int_as_float(__builtin_amdgcn_ds_bpermute(((lf[k] >> 8) & 63) << 2, wtv));
+            } else if (uni) {
+                R.wa[k] = R.wa[k] * uax;
+                R.wb[k] = R.wb[k] * uax;
             }
             R.ra[k] = R.wa[k] * R.ra[k];
             R.rb[k] = R.wb[k] * R.rb[k];
@@ -517,16 +537,18 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // persistent form (one workgroup per CU, every wave waiting at the
     // barrier for the slowest SIMD) runs every wave.
     auto live = [&](int h) { return PERS || (w * NRW + NRW - 1 > h && NW * NRW - 1 - w * NRW > h); };
-    auto half32 = [&]<int Q>(int h) {
+    auto half32 = [&]<int Q, bool UNI>(int h) {
         const int b = h & 1;
         if (live(h)) {
             const V2 sv = xch[b][w][1][lane], nv = xch[b][w + 2][0][lane];
             [&]<int... K>(std::integer_sequence<int, K...>) {
-                (reg_row<T, NRW, Q, true, K + 1>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1),
+                (reg_row<T, NRW, Q, true, K + 1, UNI>(R, 0, (T)0, (T)0, (T)0, (T)0, cxpa, cxma, cxpb, cxmb, c1,
+                                                      uratio),
                  ...);
             }(std::make_integer_sequence<int, NRW - 2>{});
-            reg_row<T, NRW, Q, true, 0>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
-            reg_row<T, NRW, Q, true, NRW - 1>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1);
+            reg_row<T, NRW, Q, true, 0, UNI>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1, uratio);
+            reg_row<T, NRW, Q, true, NRW - 1, UNI>(R, 0, sv[0], sv[1], nv[0], nv[1], cxpa, cxma, cxpb, cxmb, c1,
+                                                   uratio);
             xch[b ^ 1][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
             xch[b ^ 1][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         }
@@ -534,19 +556,19 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     };
     // nst is even: the half-sweeps run in pairs whose colour order is fixed
     // per wave, so each loop body is straight-line code
-    auto pairs = [&](int n) {
+    auto pairs = [&]<bool UNI>(int n) {
         xch[0][w + 1][0][lane] = V2{R.pa[0], R.pb[0]};
         xch[0][w + 1][1][lane] = V2{R.pa[NRW - 1], R.pb[NRW - 1]};
         __syncthreads();
         if (Z == 0) {
             for (int h = 0; h < n; h += 2) {
-                half32.template operator()<0>(h);
-                half32.template operator()<1>(h + 1);
+                half32.template operator()<0, UNI>(h);
+                half32.template operator()<1, UNI>(h + 1);
             }
         } else {
             for (int h = 0; h < n; h += 2) {
-                half32.template operator()<1>(h);
-                half32.template operator()<0>(h + 1);
+                half32.template operator()<1, UNI>(h);
+                half32.template operator()<0, UNI>(h + 1);
             }
         }
     };
@@ -555,7 +577,14 @@ __global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const 
     // register budget)
     auto sweeps = [&](int n) {
         if constexpr (sizeof(T) == 4) {
-            pairs(n);
+            if constexpr (WM == 2) {
+                if (uni)
+                    pairs.template operator()<true>(n);
+                else
+                    pairs.template operator()<false>(n);
+            } else {
+                pairs.template operator()<WM == 1>(n);
+            }
         } else {
             for (int h = 0; h < n; h++) {
                 const int b = h & 1;
@@ -725,13 +754,16 @@ template <typename T, int M, int NW, int NRW>
 static void reg_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int nt, int s, int K, int wt)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    if (wt) {
-        hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
-                           pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr);
-        return;
-    }
-    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, false>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
-                       pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr);
+#define IFF_RL(WM_)                                                                                             \
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, false, WM_>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt, \
+                       pad, s, K, (GridBar *)nullptr, (unsigned *)nullptr)
+    if (wt == 1)
+        IFF_RL(1);
+    else if (sizeof(T) == 4 && wt == 2)
+        IFF_RL(sizeof(T) == 4 ? 2 : 0);
+    else
+        IFF_RL(0);
+#undef IFF_RL
 }
 
 // Blocks of the persistent form that can be resident at once: resident
@@ -743,15 +775,21 @@ static int pers_capacity()
     static int capacity = -1;  // one device type per process
     if (capacity < 0) {
         int per_cu = 0, per_cu_t = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbsor_reg<T, M, NW, NRW, true, false>,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbsor_reg<T, M, NW, NRW, true, 0>,
                                                          64 * NW, 0) != hipSuccess ||
             hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_rbsor_reg<T, M, NW, NRW, true, true>, 64 * NW,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_rbsor_reg<T, M, NW, NRW, true, 1>, 64 * NW,
                                                          0) != hipSuccess)
             return 0;
         per_cu = std::min(per_cu, per_cu_t);
+        if constexpr (sizeof(T) == 4) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_rbsor_reg<T, M, NW, NRW, true, 2>,
+                                                             64 * NW, 0) != hipSuccess)
+                return 0;
+            per_cu = std::min(per_cu, per_cu_t);
+        }
         capacity = per_cu * cus;
     }
     return capacity;
@@ -762,13 +800,16 @@ static void pers_one(hipStream_t st, const CaseDev *cs, const int2 *tiles, int n
                      unsigned *flags, int wt)
 {
     const int pad = ((nt + 7) / 8) * 8;
-    if (wt) {
-        hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, true>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
-                           pad, 0, K, bar, flags);
-        return;
-    }
-    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, false>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt,
-                       pad, 0, K, bar, flags);
+#define IFF_PLW(WM_)                                                                                           \
+    hipLaunchKernelGGL((k_rbsor_reg<T, M, NW, NRW, true, WM_>), dim3(pad), dim3(64 * NW), 0, st, cs, tiles, nt, \
+                       pad, 0, K, bar, flags)
+    if (wt == 1)
+        IFF_PLW(1);
+    else if (sizeof(T) == 4 && wt == 2)
+        IFF_PLW(sizeof(T) == 4 ? 2 : 0);
+    else
+        IFF_PLW(0);
+#undef IFF_PLW
 }
 
 // the shapes with a persistent form: (waves, sweeps per stage, tile_y,

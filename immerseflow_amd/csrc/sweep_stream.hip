@@ -62,6 +62,9 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
         cpx_ = c.xf.cp; cmx_ = c.xf.cm; cpy_ = c.yf.cp; cmy_ = c.yf.cm;
         om = c.omega_f;
     }
+    // FP32, one coefficient value per axis: the five-operation update
+    const bool uni = sizeof(T) == 4 && c.uni;
+    const float uax = c.uax, uratio = c.uratio;
     const int ntxs = (nx - 2 + WO - 1) / WO;
     const int tx = tl.y % ntxs, ty = tl.y / ntxs;
     const int i0 = 1 + tx * WO, j0 = 1 + ty * TY;
@@ -116,7 +119,10 @@ __global__ __launch_bounds__(256) void k_rbsor_stream(const CaseDev *cs, const i
                         const T aS = (f & F_OPEN_S) ? MY[sj] : (T)0;
                         const T sa = (aE + aW) + (aN + aS);
                         if (sa != (T)0) {
-                            Pr[sj] = sor_cell<T>(pc, Rr[sj], aE, pE, aW, pW, aN, Pr[sn], aS, Pr[ss], sa, om, gi);
+                            if (uni)
+                                Pr[sj] = sor_cell_u_sa(pc, Rr[sj], pE, pW, Pr[sn], Pr[ss], sa, om, uax, uratio);
+                            else
+                                Pr[sj] = sor_cell<T>(pc, Rr[sj], aE, pE, aW, pW, aN, Pr[sn], aS, Pr[ss], sa, om, gi);
                         }
                     }
                 }
