@@ -54,8 +54,11 @@ HBM_PEAK_GBS = 8000.0
 # half of it: 256 CUs x 64 lanes x 2 FLOP (FMA) x 2.4 GHz.
 FP32_PEAK_TFLOPS = 157.3
 FP32_NONPACKED_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12
-FLOP_PER_UPDATE = 11     # FP32 red-black update: 1 multiply + 5 FMA
-VALU_PER_UPDATE = 6      # vector instructions per update (lane shift folded in)
+# FP32 red-black update, FLOP and vector instructions (the lane shift folded
+# in): the general form (1 multiply + 5 FMA) and the form of a grid with one
+# coefficient value per axis (1 add + 4 FMA)
+FLOP_PER_UPDATE = {False: 11, True: 9}
+VALU_PER_UPDATE = {False: 6, True: 5}
 
 
 def parse():
@@ -232,21 +235,26 @@ def headline_c3(iff, configs, a):
     upd = int(reg_updates_per_sweep(NX, NY, tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], pers) * spl)
     useful = cells * spl
     kname = kernel_name(tr["kernel"], tr["tile_y"], tr["sweeps_per_launch"], "float", pers, st["weight_table"])
-    achieved = upd * FLOP_PER_UPDATE / avg_s / 1e12 if avg_s > 0 else 0.0
+    uf = bool(st["weight_table"])  # the headline grid is uniform: the five-operation form
+    fpu = FLOP_PER_UPDATE[uf]
+    achieved = upd * fpu / avg_s / 1e12 if avg_s > 0 else 0.0
     roof = dict(bound="valu", achieved=round(achieved, 2), peak=round(FP32_NONPACKED_TFLOPS, 2), unit="TFLOP/s",
                 frac=round(achieved / FP32_NONPACKED_TFLOPS, 4), traffic=None, kernel=kname,
                 published_fp32_peak=FP32_PEAK_TFLOPS,
                 frac_of_published=round(achieved / FP32_PEAK_TFLOPS, 4),
                 updates_per_launch=upd, interior_updates_per_launch=useful,
                 recompute_factor=round(upd / useful, 3), sweeps_per_launch=spl,
-                flop_per_update=FLOP_PER_UPDATE, valu_per_update=VALU_PER_UPDATE,
+                flop_per_update=fpu, valu_per_update=VALU_PER_UPDATE[uf],
                 mean_launch_us=round(avg_s * 1e6, 3), launches_timed=launches,
-                useful_tflops=round(useful * FLOP_PER_UPDATE / avg_s / 1e12, 2) if avg_s > 0 else None,
+                useful_tflops=round(useful * fpu / avg_s / 1e12, 2) if avg_s > 0 else None,
                 note="achieved = cell updates the launch executes (the temporal-blocking halo recompute "
-                     "included) x 11 FP32 FLOP (1 multiply + 5 FMA: 6 VALU instructions) / mean launch time "
-                     "from device time stamps around each chunk; peak = 256 CUs x 64 lanes x 2 FLOP x 2.4 GHz, "
+                     "included, the halo half-sweeps that no longer hold exact rows left out) x FP32 FLOP per "
+                     "update (%s) / mean launch time from device time stamps around each chunk; peak = 256 CUs "
+                     "x 64 lanes x 2 FLOP x 2.4 GHz, "
                      "the FP32 rate without packed instructions, which this build does not emit (the "
-                     "published 157.3 TFLOP/s counts packed FP32)")
+                     "published 157.3 TFLOP/s counts packed FP32)"
+                     % ("9: 1 add + 4 FMA, the form of a grid with one coefficient value per axis" if uf
+                        else "11: 1 multiply + 5 FMA"))
     tp = os.path.join(ROOT, "profiles", "traffic_rbsor_reg_c3.json")
     tj = {}
     if os.path.exists(tp):

@@ -212,7 +212,7 @@ static if_status tune(Engine &e, int kind, int ty, int m, int k)
     if (kind == 4 && !reg_variant_ok(pr, 16, ty, m))
         return fail(IF_ERR_ARG, "tune: 16-wave register-tile sweep needs 4 sweeps with tile_y 48/64/80 or 8 "
                                 "sweeps with tile_y 32/64; mixed precision also 8 sweeps with tile_y 96, 9 "
-                                "with 92 or 10 with 88");
+                                "with 92, 10 with 88, 11 with 84 or 12 with 80");
     if (kind < 0 || kind > 4) return fail(IF_ERR_ARG, "tune: unknown sweep kernel");
     if (k < 1) return fail(IF_ERR_ARG, "tune: check_every must be >= 1");
     e.sweep_kind = kind;

@@ -228,10 +228,15 @@ __device__ __forceinline__ void wait_neighbours(const unsigned *flags, int nb, u
 // the checks read the same buffer either way.  A block is at most one
 // stage ahead of its neighbours, so the two buffers never hold data a
 // neighbour still needs when they are overwritten.
+//
+// The second launch bound holds the FP32 per-stage 128 x 128 blocks (16
+// waves x 8 rows) at 8 waves per SIMD, i.e. 64 VGPRs and two blocks per CU:
+// left free, some of them take a few more registers and run at one block
+// per CU, a third slower.
 template <typename T, int M, int NW, int NRW, bool PERS, int WM>
-__global__ __launch_bounds__(64 * NW) void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles,
-                                                       int ntiles_pad, int s, int K, GridBar *bar,
-                                                       unsigned *flags)
+__global__ __launch_bounds__(64 * NW, (sizeof(T) == 4 && !PERS && NW == 16 && NRW == 8 && WM != 2) ? 8 : 1)
+void k_rbsor_reg(const CaseDev *cs, const int2 *tiles, int ntiles, int ntiles_pad, int s, int K, GridBar *bar,
+                 unsigned *flags)
 {
     constexpr int H = 2 * M;
     constexpr int TW = 128 - 2 * H;   // stored columns per tile
@@ -866,9 +871,9 @@ void launch_ppe_sweep_pers(hipStream_t st, int prec, int waves, int ty, int m, c
 // tiles) trades a wider halo for half the loads per sweep.  The FP64 tile
 // of 16 waves x 8 rows does not fit the 128 registers a 1024-lane
 // workgroup allows, so that shape is FP32 (mixed precision) only, as are
-// its deeper forms: 9 and 10 sweeps on the same 128 x 128 cells (92 and 88
-// stored rows and columns), fewer loads per sweep for more recomputed
-// halo.
+// its deeper forms: 9 to 12 sweeps on the same 128 x 128 cells (92, 88, 84
+// and 80 stored rows and columns), fewer loads per sweep for more
+// recomputed halo.
 bool reg_variant_ok(int prec, int waves, int ty, int m)
 {
     if (waves == 4) return (m == 1 || m == 2 || m == 4) && (ty == 16 || ty == 32 || ty == 48);
@@ -878,7 +883,8 @@ bool reg_variant_ok(int prec, int waves, int ty, int m)
     if (waves == 16)
         return (m == 4 && (ty == 48 || ty == 64 || ty == 80)) ||
                (m == 8 && (ty == 32 || ty == 64 || (ty == 96 && prec != IF_FP64))) ||
-               (prec != IF_FP64 && ((m == 9 && ty == 92) || (m == 10 && ty == 88) || (m == 12 && ty == 80)));
+               (prec != IF_FP64 && ((m == 9 && ty == 92) || (m == 10 && ty == 88) || (m == 11 && ty == 84) ||
+                                    (m == 12 && ty == 80)));
     return false;
 }
 
@@ -907,6 +913,7 @@ void launch_ppe_sweep_reg(hipStream_t st, int prec, int waves, int ty, int m, co
         IFF_RG(float, 16, 8, 96)
         IFF_RG(float, 16, 9, 92)
         IFF_RG(float, 16, 10, 88)
+        IFF_RG(float, 16, 11, 84)
         IFF_RG(float, 16, 12, 80)
     }
 #undef IFF_RG_ALL

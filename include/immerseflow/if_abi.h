@@ -244,9 +244,9 @@ if_status if_case_ghosts(if_case *c, int32_t *out, if_ghost *recs,
  * kernel 3: the same with eight wave64s per tile, tile height 24/40/56
  * (2 sweeps per launch), 32/48/64 (4) or 32/48/64 (8); kernel 4: sixteen
  * wave64s per tile, tile height 48/64/80 (4 sweeps per launch) or 32/64/96
- * (8), and in mixed precision only also 96 (8), 92 (9) and 88 (10; the
- * tile computes 128 x 128 cells, stores 128 - 4m of them in each direction
- * and recomputes the rest).  Several red-black sweeps per launch is
+ * (8), and in mixed precision only also 96 (8), 92 (9), 88 (10), 84 (11)
+ * and 80 (12; the tile computes 128 x 128 cells, stores 128 - 4m of them in
+ * each direction and recomputes the rest).  Several red-black sweeps per launch is
  * temporal blocking; check_every is the number of sweeps between
  * convergence checks.  Results do not depend on the kernel, the tile
  * height or the sweeps per launch.  Defaults: kernel 4, 64 rows, 8 sweeps
