@@ -144,11 +144,14 @@ typedef struct {
     int persistent_timeouts; /* steps whose persistent Poisson chunk timed out
                                 (blocks not all resident) and was solved again
                                 with one launch per stage */
-    int weight_table;       /* last step: the FP32 red-black sweeps looked each
-                               cell's weight up by its face flags (every
-                               red-black case resident has a grid whose FP32
-                               coefficients are each one value) instead of
-                               loading a per-cell weight */
+    int weight_table;       /* last step: the red-black sweeps looked each
+                               cell's weight up by its face flags, because
+                               every red-black case resident has a grid with
+                               one coefficient value per axis in the Poisson
+                               precision; mixed: instead of loading a per-cell
+                               weight, with the uniform grid's five-operation
+                               update; FP64: instead of dividing on every
+                               launch.  Results are the same either way. */
 } if_step_stats;
 
 typedef struct {
