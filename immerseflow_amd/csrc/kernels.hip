@@ -1191,9 +1191,19 @@ __global__ __launch_bounds__(256) void k_ppe_finalize(const CaseDev *cs, int nca
     Ctrl *ct = c.ctrl;
     if (!ct->active || ct->ppe_done) return;
     const int nt = c.ntx * c.nty;
-    double m = 0.0;
-    for (int k = threadIdx.x; k < nt; k += 256) m = fmax(m, c.part[k]);
-    m = block_max(m);
+    // four independent loads in flight per thread (the maximum does not
+    // depend on the order)
+    const double *part = c.part;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int k = threadIdx.x;
+    for (; k + 768 < nt; k += 1024) {
+        m0 = fmax(m0, part[k]);
+        m1 = fmax(m1, part[k + 256]);
+        m2 = fmax(m2, part[k + 512]);
+        m3 = fmax(m3, part[k + 768]);
+    }
+    for (; k < nt; k += 256) m0 = fmax(m0, part[k]);
+    double m = block_max(fmax(fmax(m0, m1), fmax(m2, m3)));
     if (threadIdx.x == 0) {
         int kexec = c.ppe_itermax - ct->ppe_sweeps;
         if (kexec > K) kexec = K;
