@@ -113,7 +113,7 @@ private:
     int2 *d_tiles_base_ = nullptr, *d_tiles_sweep_ = nullptr;
     int nt_base_ = 0, nt_sweep_ = 0, cap_tiles_base_ = 0, cap_tiles_sweep_ = 0;
     int tiles_sweep_ty_ = 0;
-    int wt_all_ = 0;                   // sweep weight mode: 1 every red-black case resident has a weight table,
+    int wmode_ = 0;                   // sweep weight mode: 1 every red-black case resident has a weight table,
                                        // 2 (FP32) some take the five-operation form, 0 none
     int max_ng_ = 0, max_ring_ = 0, max_cells_ = 0, max_ad_ = 0, max_ppe_ = 0;
     bool any_moving_ = false;

@@ -655,12 +655,12 @@ void Engine::enqueue_sweeps(int K, int M, bool any_rb, bool any_jac, bool any_li
     if (stamp) launch_stamp(st_, d_stamps_, kStamps);
     if (any_rb && pers)  // the whole chunk in one launch
         launch_ppe_sweep_pers(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_,
-                              K, d_bar_, d_flags_, wt_all_);
+                              K, d_bar_, d_flags_, wmode_);
     for (int s = 0; s < K; s++) {
         if (any_rb && !pers && s % M == 0) {
             if (sweep_kind >= 2)
                 launch_ppe_sweep_reg(st_, pprec_, reg_waves(sweep_kind), tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_,
-                                     nt_sweep_, s, K, wt_all_);
+                                     nt_sweep_, s, K, wmode_);
             else if (sweep_kind == 1)
                 launch_ppe_sweep_stream(st_, pprec_, tiles_sweep_ty_, M, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
             else
@@ -753,7 +753,7 @@ if_status Engine::refresh()
             tab += (pprec_ == IF_FP64 ? (void *)sl.dev.wtab64 : (void *)sl.dev.wtab) != nullptr;
             uni += sl.dev.uni;
         }
-        wt_all_ = tab == rb ? 1 : uni > 0 ? 2 : 0;
+        wmode_ = tab == rb ? 1 : uni > 0 ? 2 : 0;
     }
     // FP64 default shape, by measurement: 16-wave tiles as persistent
     // chunks of the smallest height (48, 64, 80 rows) whose tiles fit in
@@ -1060,9 +1060,9 @@ if_status Engine::step(int nsteps, int *finished, if_step_stats *agg)
             s.last.total_ppe_sweeps = c.tot_ppe;
             s.last.max_div = c.max_div;
             s.last.failed = c.failed;
-            s.last.weight_table = wt_all_ == 1;
+            s.last.weight_table = wmode_ == 1;
             if (agg) {
-                agg->weight_table = wt_all_ == 1;
+                agg->weight_table = wmode_ == 1;
                 agg->steps = std::max(agg->steps, 1 + it);
                 agg->ad_iters = c.ad_iter;
                 agg->ad_res = c.ad_res;
@@ -1150,7 +1150,7 @@ if_status Engine::autotune(if_tune_result *out)
     auto sweep_launch = [&](const Shape &c, int s, int K) {
         if (c.kind >= 2)
             launch_ppe_sweep_reg(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_,
-                                 s, K, wt_all_);
+                                 s, K, wmode_);
         else if (c.kind == 1)
             launch_ppe_sweep_stream(st_, pprec_, c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, s, K);
         else
@@ -1199,7 +1199,7 @@ if_status Engine::autotune(if_tune_result *out)
         auto timed = [&]() {
             if (pc)
                 launch_ppe_sweep_pers(st_, pprec_, reg_waves(c.kind), c.ty, c.m, d_cases_, d_tiles_sweep_, nt_sweep_, kp,
-                                      d_bar_, d_flags_, wt_all_);
+                                      d_bar_, d_flags_, wmode_);
             else
                 sweep_launch(c, s0, 2 * c.m);
         };
