@@ -12,19 +12,26 @@
 //   * north/south neighbours: the lane's own registers, except at the
 //     wave's first and last row, whose neighbours come from the adjacent
 //     waves through a small LDS exchange (one barrier per half-sweep).
-// Rows whose cells are all fluid with four open faces (nearly all of them)
-// take a fast path whose coefficients are the metric itself (x: per lane,
-// y: per row); a row holding any other cell evaluates every lane with its
-// face flags.  Both paths compute the same expression (sor.h), so the
-// result does not depend on the path, the tile shape or the kernel.
+// FP64 (the oracle's update, bit for bit): rows whose cells are all fluid
+// with four open faces (nearly all of them) take a fast path whose
+// coefficients are the metric itself (x: per lane, y: per row); a row
+// holding any other cell evaluates every lane with its face flags.  FP32
+// (the mixed path's error sweeps): every cell takes one branch-free form,
+// the general one or, on a grid with one coefficient value per axis, the
+// five-operation one.  Every kernel computes the same expression per case
+// (sor.h), so the result does not depend on the path, the tile shape, the
+// kernel or the batch a case is in.  The cells' weights come from a table
+// indexed by the flag byte (uniform grids) or per cell (FP32: loaded once
+// per launch; FP64: formed once per launch); WM, the launch's weight mode,
+// picks the form at compile time.
 //
 // M sweeps per launch on the tile with its 2M halo (temporal blocking):
 // after half-sweep h only the tile plus a ring of 2M-1-h cells is still
 // exact, so after 2M half-sweeps the interior (TW x TY) is, and only it is
 // stored.  Reads come from buffer src and writes go to dst (ping-pong), so
 // tiles never read halos that other tiles are writing.  HBM traffic is one
-// read of the unknown, the right side and the flags, plus the halo, and
-// one write of the unknown per M sweeps.
+// read of the unknown, the right side and the flags (or FP32 weights),
+// plus the halo, and one write of the unknown per M sweeps.
 #include "common.h"
 #include "kernels.h"
 #include "sor.h"
