@@ -18,9 +18,10 @@
  *    fractional step (divergence RHS, Poisson, projection, divergence check)
  *    with a sharp-interface ghost-cell immersed boundary.  The reference has
  *    no code for these parts (PPESolver.cu does not compile,
- *    AD_PPE_Correction.cu is empty, iBlankComputeKernel is a stub), so each
- *    such function is "parity unpinned" against the reference and is held by
- *    closed forms, invariants and brute force in tests/ instead.
+ *    AD_PPE_Correction.cu is empty, iBlankComputeKernel is a stub), so no
+ *    reference output exists to compare them with; each such function is
+ *    pinned by closed forms, invariants and brute force in tests/ instead
+ *    (test_oracle_physics.py, test_oracle_moving.py).
  *
  * Layouts follow the reference (src/include/preSim.cu:10): cell arrays are
  * nx*ny with idx = i + j*nx, nx/ny INCLUDING the ghost ring; east-face
